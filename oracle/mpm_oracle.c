@@ -1,0 +1,972 @@
+/*
+ * mpm_oracle.c -- CPU FP64 restatement of MPMWorkpieceSolver::step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP solver; never shipped,
+ * never called by the product library).
+ *
+ * Reference: Up25-pathan/edegpredict-engine-rotary, src/MPMWorkpieceSolver.cu
+ * and include/MPMWorkpieceSolver.cuh.  Every function cites the lines it
+ * restates.  Deliberate, documented differences from the reference binary:
+ *   - the grid heat diffusion is Jacobi (reads the pre-diffusion field, writes
+ *     a new one).  The reference kernel (MPMWorkpieceSolver.cu:208-262) updates
+ *     temperatureMass in place while neighbouring threads read it, a data race
+ *     whose outcome depends on scheduling; its own comment (:195-207) states the
+ *     Jacobi update T_new = T + alpha*dt*lap(T).
+ *   - compaction keeps the surviving particles in their previous order; the
+ *     reference's atomic-counter compaction (:464-473) leaves them in an
+ *     unspecified order.
+ *   - the analytic tool modes (wedge, drill) and tool velocity/friction are the
+ *     grid-node form of ContactSolver.cu:797-923; in point-sample mode the grid
+ *     update is exactly :136-193.
+ *
+ * Parity pins for each function are listed in tests/test_oracle.py.
+ */
+#include "mpm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+struct orc_solver {
+    orc_params prm;
+    orc_particle *p;
+    int n, cap;
+    orc_grid g;
+    orc_node *grid;
+    orc_node *snap;       /* post-P2G copy of the grid */
+    int nnodes;
+    double *tm_new;       /* Jacobi buffer for the diffusion */
+    double *tool_xyz;     /* point samples */
+    int ntool;
+    orc_tool_desc tool;
+    orc_tool_pose pose;
+    double tip[3];
+    double time;
+    int64_t step;
+    double max_stress, max_temp, ke, contact_force;
+    int compactions;
+};
+
+static int g_threads = 1;
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+
+static double clampd(double v, double lo, double hi) { return fmax(lo, fmin(hi, v)); }  /* :14-16 */
+
+/* equivalentStress, MPMWorkpieceSolver.cu:18-24 */
+double orc_equivalent_stress(double xx, double yy, double zz, double xy, double xz, double yz) {
+    double s1 = xx - yy, s2 = yy - zz, s3 = zz - xx;
+    return sqrt(0.5 * (s1 * s1 + s2 * s2 + s3 * s3 + 6.0 * (xy * xy + xz * xz + yz * yz)));
+}
+
+/* johnsonCookStress, MPMWorkpieceSolver.cu:43-50 */
+double orc_jc_flow(const orc_params *m, double eps_p, double eps_rate, double temp) {
+    double strainTerm = m->jc_A + m->jc_B * pow(fmax(eps_p, 0.0), m->jc_n);
+    double rateRatio = fmax(eps_rate / fmax(m->ref_strain_rate, 1e-9), 1.0);
+    double rateTerm = 1.0 + m->jc_C * log(rateRatio);
+    double tStar = clampd((temp - m->ref_temp) / fmax(m->melt_temp - m->ref_temp, 1.0), 0.0, 1.0);
+    double thermalTerm = fmax(0.01, 1.0 - pow(tStar, m->jc_m));
+    return strainTerm * rateTerm * thermalTerm;
+}
+
+/* computeWeights, MPMWorkpieceSolver.cu:52-90 (gridIndex :26-28) */
+void orc_weights(const orc_grid *g, double x, double y, double z,
+                 int32_t idx[8], double w[8], double grad[24]) {
+    double rx = (x - g->minx) / g->cell;
+    double ry = (y - g->miny) / g->cell;
+    double rz = (z - g->minz) / g->cell;
+    int bx = (int)floor(rx), by = (int)floor(ry), bz = (int)floor(rz);
+    double fx = rx - bx, fy = ry - by, fz = rz - bz;
+    int n = 0;
+    for (int dz = 0; dz <= 1; ++dz)
+        for (int dy = 0; dy <= 1; ++dy)
+            for (int dx = 0; dx <= 1; ++dx, ++n) {
+                int i = bx + dx, j = by + dy, k = bz + dz;
+                if (i < 0 || j < 0 || k < 0 || i >= g->nx || j >= g->ny || k >= g->nz) {
+                    idx[n] = -1;
+                    w[n] = 0.0;
+                    grad[3 * n] = grad[3 * n + 1] = grad[3 * n + 2] = 0.0;
+                    continue;
+                }
+                double wx = dx ? fx : 1.0 - fx;
+                double wy = dy ? fy : 1.0 - fy;
+                double wz = dz ? fz : 1.0 - fz;
+                double dwx = dx ? 1.0 / g->cell : -1.0 / g->cell;
+                double dwy = dy ? 1.0 / g->cell : -1.0 / g->cell;
+                double dwz = dz ? 1.0 / g->cell : -1.0 / g->cell;
+                idx[n] = (k * g->ny + j) * g->nx + i;
+                w[n] = wx * wy * wz;
+                grad[3 * n] = dwx * wy * wz;
+                grad[3 * n + 1] = wx * dwy * wz;
+                grad[3 * n + 2] = wx * wy * dwz;
+            }
+}
+
+/* rebuildGridFromHostBounds, MPMWorkpieceSolver.cu:578-584 (given the AABB) */
+void orc_rebuild_params(double cell, const double mn[3], const double mx[3], orc_grid *g) {
+    double pad = cell * 4.0;
+    g->cell = cell;
+    g->minx = mn[0] - pad;
+    g->miny = mn[1] - pad;
+    g->minz = mn[2] - pad;
+    int nx = (int)ceil((mx[0] - mn[0] + 2 * pad) / cell) + 1;
+    int ny = (int)ceil((mx[1] - mn[1] + 2 * pad) / cell) + 1;
+    int nz = (int)ceil((mx[2] - mn[2] + 2 * pad) / cell) + 1;
+    g->nx = nx > 4 ? nx : 4;
+    g->ny = ny > 4 ? ny : 4;
+    g->nz = nz > 4 ? nz : 4;
+}
+
+/* chip-fracture pseudo-random direction, MPMWorkpieceSolver.cu:709-712 */
+void orc_chip_kick(int32_t id, double r[3]) {
+    uint32_t seed = (uint32_t)((uint32_t)id * 2654435761U);
+    r[0] = (double)((seed >> 16) & 0xFF) / 255.0 - 0.5;
+    r[1] = (double)((seed >> 8) & 0xFF) / 255.0 - 0.5;
+    r[2] = (double)(seed & 0xFF) / 255.0 - 0.5;
+}
+
+/* ---------------------------------------------------------------- tools -- */
+
+/* World -> tool frame: translate by -center, rotate by -angle about z
+ * (ContactSolver.cu:825-829 uses the same Z-rotation convention). */
+static void to_tool_frame(const orc_tool_pose *p, double x, double y, double z, double q[3]) {
+    double c = cos(p->angle), s = sin(p->angle);
+    double dx = x - p->center[0], dy = y - p->center[1];
+    q[0] = dx * c + dy * s;
+    q[1] = -dx * s + dy * c;
+    q[2] = z - p->center[2];
+}
+
+/* Wedge: rake plane through the edge (normal (cos g, 0, sin g)) intersected
+ * with the flank plane (normal (-sin a, 0, -cos a)); cutting direction +x,
+ * edge along y.  Interior where both plane distances are negative. */
+static double phi_wedge(const orc_tool_desc *d, const double q[3]) {
+    double g = d->rake_angle, a = d->clearance_angle;
+    double pr = cos(g) * q[0] + sin(g) * q[2];
+    double pf = -sin(a) * q[0] - cos(a) * q[2];
+    return fmax(pr, pf);
+}
+
+/* Two-flute helical drill, tip at the origin, body along +z.
+ * body = cone(half-angle = point/2) & cylinder(r <= R) & (z <= L);
+ * two helical flute grooves of radius rf at distance df from the axis,
+ * phase k*pi + z*tan(helix)/R, are subtracted. */
+static double phi_drill(const orc_tool_desc *d, const double q[3]) {
+    double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+    double beta = 0.5 * d->point_angle;
+    double sb = sin(beta), cb = cos(beta);
+    double t = r * sb + q[2] * cb;
+    double cone = (t < 0.0) ? sqrt(r * r + q[2] * q[2]) : (r * cb - q[2] * sb);
+    double body = fmax(fmax(cone, r - d->radius), q[2] - d->flute_length);
+    double psi = q[2] * tan(d->helix_angle) / d->radius;
+    double gmin = 1e300;
+    for (int k = 0; k < 2; ++k) {
+        double ang = psi + k * 3.14159265358979323846;
+        double cx = d->flute_offset * cos(ang), cy = d->flute_offset * sin(ang);
+        double ex = q[0] - cx, ey = q[1] - cy;
+        double gk = sqrt(ex * ex + ey * ey) - d->flute_radius;
+        gmin = fmin(gmin, gk);
+    }
+    return fmax(body, -gmin);
+}
+
+static double phi_local(const orc_tool_desc *d, const double q[3]) {
+    if (d->kind == ORC_TOOL_WEDGE) return phi_wedge(d, q);
+    if (d->kind == ORC_TOOL_DRILL) return phi_drill(d, q);
+    return 1e300;
+}
+
+double orc_tool_phi(const orc_tool_desc *d, const orc_tool_pose *p, double x, double y, double z) {
+    double q[3];
+    to_tool_frame(p, x, y, z, q);
+    return phi_local(d, q);
+}
+
+/* Normal = normalised central-difference gradient of phi in world space. */
+void orc_tool_normal(const orc_tool_desc *d, const orc_tool_pose *p, double x, double y, double z,
+                     double n[3]) {
+    double e = d->grad_eps;
+    double gx = (orc_tool_phi(d, p, x + e, y, z) - orc_tool_phi(d, p, x - e, y, z)) / (2.0 * e);
+    double gy = (orc_tool_phi(d, p, x, y + e, z) - orc_tool_phi(d, p, x, y - e, z)) / (2.0 * e);
+    double gz = (orc_tool_phi(d, p, x, y, z + e) - orc_tool_phi(d, p, x, y, z - e)) / (2.0 * e);
+    double len = sqrt(gx * gx + gy * gy + gz * gz);
+    if (len > 1e-12) {
+        n[0] = gx / len; n[1] = gy / len; n[2] = gz / len;
+    } else {
+        n[0] = n[1] = n[2] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------ lifecycle -- */
+
+/* Config defaults the hot path reads (Config.h:23-45,74-80,128-157,343-346,366-391)
+ * mapped as MPMWorkpieceSolver::initialize does (MPMWorkpieceSolver.cu:428-451). */
+void orc_params_default(orc_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->density = 4430.0;
+    p->young = 113.8e9;
+    p->poisson = 0.34;
+    p->specific_heat = 526.3;
+    p->melt_temp = 1660.0;
+    p->ref_temp = 25.0;
+    p->jc_A = 880e6; p->jc_B = 290e6; p->jc_n = 0.47; p->jc_C = 0.015; p->jc_m = 1.0;
+    p->ref_strain_rate = 1.0;
+    p->failure_strain = 0.3;
+    p->damage_threshold = 1.0;
+    p->critical_plastic_strain = 0.15;
+    p->thermal_chip_ratio = 0.25;
+    p->cell_size = 0.0001;
+    p->tool_contact_radius = 1.5 * p->cell_size;
+    p->contact_damping = 0.15;              /* MPMWorkpieceSolver.cuh:85 */
+    p->max_contact_temp_rise = 50.0;
+    p->thermal_conductivity = 42.6;         /* MPMWorkpieceSolver.cuh:87 */
+    p->kinematic_hardening = 0.0;           /* MPMWorkpieceSolver.cuh:88 */
+    p->thermal_clamp_enabled = 1;
+    p->has_config = 1;
+    p->particle_spacing_factor = 0.8;
+    p->fixture_thickness = 0.002;
+    p->min_time_step = 1e-12;
+    p->max_time_step = 1e-4;
+    p->max_particles = 100000;
+}
+
+orc_solver *orc_create(const orc_params *p) {
+    orc_solver *s = (orc_solver *)calloc(1, sizeof(orc_solver));
+    s->prm = *p;
+    s->g.cell = p->cell_size;
+    s->g.nx = s->g.ny = s->g.nz = 1;
+    s->max_temp = p->ref_temp;    /* reset() leaves m_maxTemperature = refTemp */
+    s->tool.kind = ORC_TOOL_NONE;
+    return s;
+}
+
+void orc_destroy(orc_solver *s) {
+    if (!s) return;
+    free(s->p); free(s->grid); free(s->snap); free(s->tm_new); free(s->tool_xyz);
+    free(s);
+}
+
+static void ensure_particles(orc_solver *s, int n) {
+    if (n > s->cap) {
+        s->p = (orc_particle *)realloc(s->p, (size_t)n * sizeof(orc_particle));
+        s->cap = n;
+    }
+}
+
+static orc_particle *push_particle(orc_solver *s) {
+    if (s->n + 1 > s->cap) ensure_particles(s, 2 * s->cap + 1024);
+    return &s->p[s->n++];
+}
+
+/* allocateGrid, MPMWorkpieceSolver.cu:562-568 (reallocate on size change) */
+static void alloc_grid(orc_solver *s) {
+    int n = s->g.nx * s->g.ny * s->g.nz;
+    if (n == s->nnodes && s->grid) return;
+    free(s->grid); free(s->snap); free(s->tm_new);
+    s->nnodes = n;
+    s->grid = (orc_node *)calloc((size_t)n, sizeof(orc_node));
+    s->snap = (orc_node *)calloc((size_t)n, sizeof(orc_node));
+    s->tm_new = (double *)calloc((size_t)n, sizeof(double));
+}
+
+/* rebuildGridFromHostBounds, MPMWorkpieceSolver.cu:570-586 */
+static void rebuild_grid(orc_solver *s) {
+    if (s->n <= 0) return;
+    double mn[3] = {1e30, 1e30, 1e30}, mx[3] = {-1e30, -1e30, -1e30};
+    for (int i = 0; i < s->n; ++i) {
+        const orc_particle *q = &s->p[i];
+        if (q->status == ORC_INACTIVE) continue;
+        mn[0] = fmin(mn[0], q->x); mn[1] = fmin(mn[1], q->y); mn[2] = fmin(mn[2], q->z);
+        mx[0] = fmax(mx[0], q->x); mx[1] = fmax(mx[1], q->y); mx[2] = fmax(mx[2], q->z);
+    }
+    orc_rebuild_params(s->g.cell, mn, mx, &s->g);
+    alloc_grid(s);
+}
+
+int orc_set_particles(orc_solver *s, const orc_particle_arrays *a) {
+    ensure_particles(s, a->n);
+    s->n = a->n;
+    for (int i = 0; i < a->n; ++i) {
+        orc_particle *q = &s->p[i];
+        memset(q, 0, sizeof(*q));
+        q->x = a->pos[3 * i]; q->y = a->pos[3 * i + 1]; q->z = a->pos[3 * i + 2];
+        q->vx = a->vel[3 * i]; q->vy = a->vel[3 * i + 1]; q->vz = a->vel[3 * i + 2];
+        if (a->fext) { q->fx = a->fext[3 * i]; q->fy = a->fext[3 * i + 1]; q->fz = a->fext[3 * i + 2]; }
+        q->mass = a->mass[i]; q->volume = a->volume[i];
+        q->density = a->density ? a->density[i] : s->prm.density;
+        q->temperature = a->temperature[i];
+        q->plasticStrain = a->plastic_strain[i]; q->strainRate = a->strain_rate[i];
+        q->damage = a->damage[i]; q->residualStress = a->residual_stress[i];
+        for (int c = 0; c < 6; ++c) {
+            q->s[c] = a->stress[6 * i + c];
+            q->a[c] = a->backstress ? a->backstress[6 * i + c] : 0.0;
+        }
+        q->id = a->id[i];
+        q->status = a->status[i];
+    }
+    rebuild_grid(s);   /* as initializeParticleBox does before uploadParticles */
+    return 0;
+}
+
+int orc_get_particles(orc_solver *s, orc_particle_arrays *a) {
+    int n = a->n < s->n ? a->n : s->n;
+    for (int i = 0; i < n; ++i) {
+        const orc_particle *q = &s->p[i];
+        a->pos[3 * i] = q->x; a->pos[3 * i + 1] = q->y; a->pos[3 * i + 2] = q->z;
+        a->vel[3 * i] = q->vx; a->vel[3 * i + 1] = q->vy; a->vel[3 * i + 2] = q->vz;
+        if (a->fext) { a->fext[3 * i] = q->fx; a->fext[3 * i + 1] = q->fy; a->fext[3 * i + 2] = q->fz; }
+        a->mass[i] = q->mass; a->volume[i] = q->volume;
+        if (a->density) a->density[i] = q->density;
+        a->temperature[i] = q->temperature;
+        a->plastic_strain[i] = q->plasticStrain; a->strain_rate[i] = q->strainRate;
+        a->damage[i] = q->damage; a->residual_stress[i] = q->residualStress;
+        for (int c = 0; c < 6; ++c) {
+            a->stress[6 * i + c] = q->s[c];
+            if (a->backstress) a->backstress[6 * i + c] = q->a[c];
+        }
+        a->id[i] = q->id;
+        a->status[i] = q->status;
+    }
+    a->n = n;
+    return n;
+}
+
+int orc_particle_count(const orc_solver *s) { return s->n; }
+
+/* initializeParticleBox, MPMWorkpieceSolver.cu:588-614 */
+void orc_init_particle_box(orc_solver *s, const double mn[3], const double mx[3], double spacing) {
+    int maxp = s->prm.has_config ? s->prm.max_particles : 100000;
+    double vol = spacing * spacing * spacing;
+    int id = 0;
+    s->n = 0;
+    for (double z = mx[2]; z >= mn[2] - 1e-12 && s->n < maxp; z -= spacing)
+        for (double y = mn[1]; y <= mx[1] + 1e-12 && s->n < maxp; y += spacing)
+            for (double x = mn[0]; x <= mx[0] + 1e-12 && s->n < maxp; x += spacing) {
+                orc_particle *q = push_particle(s);
+                memset(q, 0, sizeof(*q));
+                q->x = x; q->y = y; q->z = z;
+                q->mass = s->prm.density * vol;
+                q->volume = vol;
+                q->density = s->prm.density;
+                q->temperature = s->prm.ref_temp;
+                q->id = id++;
+                q->status = ORC_ACTIVE;
+                if (s->prm.has_config && z <= mn[2] + s->prm.fixture_thickness)
+                    q->status = ORC_FIXED_BOUNDARY;
+            }
+    rebuild_grid(s);
+}
+
+/* initializeCylindricalWorkpiece, MPMWorkpieceSolver.cu:616-648 (axis ignored as there) */
+void orc_init_cylinder(orc_solver *s, const double c[3], double radius, double length,
+                       double spacing, int axis) {
+    (void)axis;
+    int maxp = s->prm.has_config ? s->prm.max_particles : 100000;
+    double vol = spacing * spacing * spacing;
+    int id = 0;
+    double minZ = c[2] - length * 0.5, maxZ = c[2] + length * 0.5;
+    s->n = 0;
+    for (double z = maxZ; z >= minZ - 1e-12 && s->n < maxp; z -= spacing)
+        for (double y = c[1] - radius; y <= c[1] + radius + 1e-12 && s->n < maxp; y += spacing)
+            for (double x = c[0] - radius; x <= c[0] + radius + 1e-12 && s->n < maxp; x += spacing) {
+                double dx = x - c[0], dy = y - c[1];
+                if (dx * dx + dy * dy > radius * radius) continue;
+                orc_particle *q = push_particle(s);
+                memset(q, 0, sizeof(*q));
+                q->x = x; q->y = y; q->z = z;
+                q->mass = s->prm.density * vol;
+                q->volume = vol;
+                q->density = s->prm.density;
+                q->temperature = s->prm.ref_temp;
+                q->id = id++;
+                q->status = ORC_ACTIVE;
+                if (s->prm.has_config && z <= minZ + s->prm.fixture_thickness)
+                    q->status = ORC_FIXED_BOUNDARY;
+            }
+    rebuild_grid(s);
+}
+
+/* setToolMesh, MPMWorkpieceSolver.cu:793-810: every stride-th node, stride = max(1, N/4000) */
+void orc_set_tool_samples(orc_solver *s, const double *xyz, int n, double contact_radius) {
+    s->prm.tool_contact_radius = fmax(contact_radius, s->g.cell);
+    int stride = n / 4000;
+    if (stride < 1) stride = 1;
+    free(s->tool_xyz);
+    s->tool_xyz = NULL;
+    s->ntool = 0;
+    if (n > 0) {
+        s->tool_xyz = (double *)malloc(sizeof(double) * 3 * (size_t)((n + stride - 1) / stride));
+        for (int i = 0; i < n; i += stride) {
+            s->tool_xyz[3 * s->ntool] = xyz[3 * i];
+            s->tool_xyz[3 * s->ntool + 1] = xyz[3 * i + 1];
+            s->tool_xyz[3 * s->ntool + 2] = xyz[3 * i + 2];
+            s->ntool++;
+        }
+    }
+    s->tool.kind = s->ntool > 0 ? ORC_TOOL_POINTS : ORC_TOOL_NONE;
+}
+
+void orc_set_tool_analytic(orc_solver *s, const orc_tool_desc *d) { s->tool = *d; }
+void orc_set_tool_pose(orc_solver *s, const orc_tool_pose *p) { s->pose = *p; }
+
+/* setToolTipPosition, MPMWorkpieceSolver.cuh:133-135 */
+void orc_set_tool_tip(orc_solver *s, double x, double y, double z) {
+    s->tip[0] = x; s->tip[1] = y; s->tip[2] = z;
+}
+
+/* applyExternalForce + applyMPMExternalForceKernel, MPMWorkpieceSolver.cu:812-818, :402-414 */
+void orc_apply_external_force(orc_solver *s, const double c[3], double radius, const double f[3]) {
+    for (int i = 0; i < s->n; ++i) {
+        orc_particle *q = &s->p[i];
+        if (q->status != ORC_ACTIVE) continue;
+        double dx = q->x - c[0], dy = q->y - c[1], dz = q->z - c[2];
+        if (dx * dx + dy * dy + dz * dz <= radius * radius) {
+            q->fx += f[0]; q->fy += f[1]; q->fz += f[2];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- step --- */
+
+/* mpmP2GKernel, MPMWorkpieceSolver.cu:98-134 (serial: atomics become plain adds;
+ * with several threads each thread fills a private grid that is summed after). */
+static void p2g(orc_solver *s) {
+    const orc_grid *g = &s->g;
+    int nt = g_threads;
+    orc_node *priv = NULL;
+    if (nt > 1) priv = (orc_node *)calloc((size_t)(nt - 1) * s->nnodes, sizeof(orc_node));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) if (nt > 1)
+#endif
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        orc_node *G = (t == 0) ? s->grid : priv + (size_t)(t - 1) * s->nnodes;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int pi = 0; pi < s->n; ++pi) {
+            const orc_particle *p = &s->p[pi];
+            if (p->status == ORC_INACTIVE) continue;
+            int32_t ids[8];
+            double ws[8], gr[24];
+            orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
+            for (int n = 0; n < 8; ++n) {
+                int gid = ids[n];
+                if (gid < 0 || ws[n] <= 0.0) continue;
+                orc_node *nd = &G[gid];
+                double m = p->mass * ws[n];
+                nd->mass += m;
+                nd->vx += p->vx * m;
+                nd->vy += p->vy * m;
+                nd->vz += p->vz * m;
+                nd->tm += p->temperature * m;
+                if (p->status == ORC_ACTIVE || p->status == ORC_FIXED_BOUNDARY) {
+                    double gx = gr[3 * n], gy = gr[3 * n + 1], gz = gr[3 * n + 2];
+                    double fX = -(p->s[0] * gx + p->s[3] * gy + p->s[4] * gz) * p->volume;
+                    double fY = -(p->s[3] * gx + p->s[1] * gy + p->s[5] * gz) * p->volume;
+                    double fZ = -(p->s[4] * gx + p->s[5] * gy + p->s[2] * gz) * p->volume;
+                    nd->fx += fX + p->fx * ws[n];
+                    nd->fy += fY + p->fy * ws[n];
+                    nd->fz += fZ + p->fz * ws[n];
+                } else if (p->status == ORC_CHIP) {
+                    nd->fx += p->fx * ws[n];
+                    nd->fy += p->fy * ws[n];
+                    nd->fz += p->fz * ws[n];
+                }
+            }
+        }
+    }
+    if (priv) {
+        for (int t = 1; t < nt; ++t) {
+            const orc_node *P = priv + (size_t)(t - 1) * s->nnodes;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+            for (int i = 0; i < s->nnodes; ++i) {
+                orc_node *d = &s->grid[i];
+                d->mass += P[i].mass; d->vx += P[i].vx; d->vy += P[i].vy; d->vz += P[i].vz;
+                d->fx += P[i].fx; d->fy += P[i].fy; d->fz += P[i].fz; d->tm += P[i].tm;
+            }
+        }
+        free(priv);
+    }
+}
+
+/* mpmGridUpdateKernel, MPMWorkpieceSolver.cu:136-193; analytic tools follow the
+ * grid-node form of ContactSolver.cu:797-923 (normal from the SDF gradient,
+ * relative velocity against the moving tool, Coulomb friction). */
+static void grid_update(orc_solver *s, double fixture, double dt) {
+    const orc_grid *g = &s->g;
+    const orc_params *mat = &s->prm;
+    double force = 0.0;
+    const int nt = g_threads;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static) reduction(+ : force) if (nt > 1)
+#endif
+    for (int idx = 0; idx < s->nnodes; ++idx) {
+        orc_node *node = &s->grid[idx];
+        if (node->mass <= 1e-18) continue;
+        node->vx /= node->mass;
+        node->vy /= node->mass;
+        node->vz /= node->mass;
+        int i = idx % g->nx, t = idx / g->nx, j = t % g->ny, k = t / g->ny;
+        double px = g->minx + i * g->cell, py = g->miny + j * g->cell, pz = g->minz + k * g->cell;
+        int fixed = pz <= g->minz + fixture + g->cell;
+        if (fixed) {
+            node->fixed = 1;
+            node->vx = node->vy = node->vz = 0.0;
+            continue;
+        }
+        node->vx += (node->fx / node->mass) * dt;
+        node->vy += (node->fy / node->mass) * dt;
+        node->vz += (node->fz / node->mass) * dt;
+
+        if (s->tool.kind == ORC_TOOL_POINTS && s->ntool > 0) {
+            double r2 = mat->tool_contact_radius * mat->tool_contact_radius;
+            double bn[3] = {0, 0, 0};
+            int hit = 0;
+            for (int q = 0; q < s->ntool; ++q) {
+                double dx = px - s->tool_xyz[3 * q], dy = py - s->tool_xyz[3 * q + 1],
+                       dz = pz - s->tool_xyz[3 * q + 2];
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < r2 && d2 > 1e-20) {
+                    r2 = d2;
+                    double len = sqrt(d2);   /* Vec3::normalized, Types.h:185-188 */
+                    if (len > 1e-12) { bn[0] = dx / len; bn[1] = dy / len; bn[2] = dz / len; }
+                    else { bn[0] = bn[1] = bn[2] = 0.0; }
+                    hit = 1;
+                }
+            }
+            if (hit) {
+                double vn = node->vx * bn[0] + node->vy * bn[1] + node->vz * bn[2];
+                if (vn < 0.0) {
+                    force += fabs(node->mass * vn / fmax(dt, 1e-15));
+                    node->vx -= vn * bn[0];
+                    node->vy -= vn * bn[1];
+                    node->vz -= vn * bn[2];
+                    double heat = 0.5 * node->mass * vn * vn * mat->contact_damping;
+                    double dT = heat / fmax(node->mass * mat->specific_heat, 1e-12);
+                    if (mat->thermal_clamp_enabled) dT = fmin(dT, mat->max_contact_temp_rise);
+                    node->tm += dT * node->mass;
+                }
+            }
+        } else if (s->tool.kind == ORC_TOOL_WEDGE || s->tool.kind == ORC_TOOL_DRILL) {
+            double phi = orc_tool_phi(&s->tool, &s->pose, px, py, pz);
+            if (phi < s->tool.band) {
+                double n[3];
+                orc_tool_normal(&s->tool, &s->pose, px, py, pz, n);
+                if (n[0] != 0.0 || n[1] != 0.0 || n[2] != 0.0) {
+                    const orc_tool_pose *P = &s->pose;
+                    double tvx = P->vel[0] - P->omega * (py - P->center[1]);
+                    double tvy = P->vel[1] + P->omega * (px - P->center[0]);
+                    double tvz = P->vel[2];
+                    double rx = node->vx - tvx, ry = node->vy - tvy, rz = node->vz - tvz;
+                    double vn = rx * n[0] + ry * n[1] + rz * n[2];
+                    if (vn < 0.0) {
+                        force += fabs(node->mass * vn / fmax(dt, 1e-15));
+                        double ux = rx - vn * n[0], uy = ry - vn * n[1], uz = rz - vn * n[2];
+                        double ut2 = ux * ux + uy * uy + uz * uz;
+                        double ut = sqrt(ut2);
+                        double sc = 1.0;
+                        if (s->tool.friction > 0.0 && ut > 0.0)
+                            sc = fmax(0.0, 1.0 + s->tool.friction * vn / ut);
+                        node->vx = tvx + sc * ux;
+                        node->vy = tvy + sc * uy;
+                        node->vz = tvz + sc * uz;
+                        double removed = vn * vn + (1.0 - sc * sc) * ut2;
+                        double heat = 0.5 * node->mass * removed * mat->contact_damping;
+                        double dT = heat / fmax(node->mass * mat->specific_heat, 1e-12);
+                        if (mat->thermal_clamp_enabled) dT = fmin(dT, mat->max_contact_temp_rise);
+                        node->tm += dT * node->mass;
+                    }
+                }
+            }
+        }
+    }
+    s->contact_force = force;
+}
+
+/* mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, in its race-free
+ * Jacobi form (see the file header). */
+static void diffuse(orc_solver *s, double alpha, double dt) {
+    const orc_grid *g = &s->g;
+    const int nt = g_threads;
+    const orc_node *G = s->grid;
+    double h = g->cell, inv_h2 = 1.0 / (h * h);
+    int nxy = g->nx * g->ny;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
+#endif
+    for (int idx = 0; idx < s->nnodes; ++idx) {
+        const orc_node *node = &G[idx];
+        s->tm_new[idx] = node->tm;
+        if (node->mass < 1e-15) continue;
+        double T = node->tm / node->mass;
+        int i = idx % g->nx, t = idx / g->nx, j = t % g->ny, k = t / g->ny;
+        double lap = 0.0;
+        if (i > 0) lap += (G[idx - 1].tm / fmax(G[idx - 1].mass, 1e-30) - T);
+        if (i < g->nx - 1) lap += (G[idx + 1].tm / fmax(G[idx + 1].mass, 1e-30) - T);
+        if (j > 0) lap += (G[idx - g->nx].tm / fmax(G[idx - g->nx].mass, 1e-30) - T);
+        if (j < g->ny - 1) lap += (G[idx + g->nx].tm / fmax(G[idx + g->nx].mass, 1e-30) - T);
+        if (k > 0) lap += (G[idx - nxy].tm / fmax(G[idx - nxy].mass, 1e-30) - T);
+        if (k < g->nz - 1) lap += (G[idx + nxy].tm / fmax(G[idx + nxy].mass, 1e-30) - T);
+        double T_new = T + alpha * dt * lap * inv_h2;
+        s->tm_new[idx] = fmax(T_new, 0.0) * node->mass;
+    }
+    for (int idx = 0; idx < s->nnodes; ++idx) s->grid[idx].tm = s->tm_new[idx];
+}
+
+/* The ACTIVE-particle constitutive block of mpmG2PKernel,
+ * MPMWorkpieceSolver.cu:308-397 (with f zeroing at :399 left to the caller). */
+static void constitutive_active(const orc_params *mat, orc_particle *p, const double L[9], double dt) {
+    double shear = mat->young / (2.0 * (1.0 + mat->poisson));
+    double bulk = mat->young / (3.0 * (1.0 - 2.0 * mat->poisson));
+    double Dxx = L[0], Dyy = L[4], Dzz = L[8];
+    double Dxy = 0.5 * (L[1] + L[3]);
+    double Dxz = 0.5 * (L[2] + L[6]);
+    double Dyz = 0.5 * (L[5] + L[7]);
+    double tr = Dxx + Dyy + Dzz;
+    double devXX = Dxx - tr / 3.0, devYY = Dyy - tr / 3.0, devZZ = Dzz - tr / 3.0;
+    double eqRate = sqrt(2.0 / 3.0 *
+        (devXX * devXX + devYY * devYY + devZZ * devZZ + 2.0 * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz)));
+    p->strainRate = eqRate;
+    double pressureInc = bulk * tr * dt;
+    p->s[0] += (2.0 * shear * devXX) * dt + pressureInc;
+    p->s[1] += (2.0 * shear * devYY) * dt + pressureInc;
+    p->s[2] += (2.0 * shear * devZZ) * dt + pressureInc;
+    p->s[3] += 2.0 * shear * Dxy * dt;
+    p->s[4] += 2.0 * shear * Dxz * dt;
+    p->s[5] += 2.0 * shear * Dyz * dt;
+    double sx = p->s[0] - p->a[0], sy = p->s[1] - p->a[1], sz = p->s[2] - p->a[2];
+    double sxy = p->s[3] - p->a[3], sxz = p->s[4] - p->a[4], syz = p->s[5] - p->a[5];
+    double seq = orc_equivalent_stress(sx, sy, sz, sxy, sxz, syz);
+    double flow = orc_jc_flow(mat, p->plasticStrain, p->strainRate, p->temperature);
+    if (seq > flow && seq > 1e-9) {
+        double scale = flow / seq;
+        p->s[0] = p->a[0] + scale * sx;
+        p->s[1] = p->a[1] + scale * sy;
+        p->s[2] = p->a[2] + scale * sz;
+        p->s[3] = p->a[3] + scale * sxy;
+        p->s[4] = p->a[4] + scale * sxz;
+        p->s[5] = p->a[5] + scale * syz;
+        double dp = (seq - flow) / fmax(3.0 * shear, 1.0);
+        if (mat->kinematic_hardening > 0.0 && dp > 0.0) {
+            double ns = mat->kinematic_hardening * dp / seq;
+            p->a[0] += ns * sx; p->a[1] += ns * sy; p->a[2] += ns * sz;
+            p->a[3] += ns * sxy; p->a[4] += ns * sxz; p->a[5] += ns * syz;
+        }
+        p->plasticStrain += dp;
+        double dT = 0.9 * flow * dp / fmax(mat->density * mat->specific_heat, 1.0);
+        if (mat->thermal_clamp_enabled) dT = fmin(dT, mat->max_contact_temp_rise);
+        p->temperature = fmin(p->temperature + dT, mat->melt_temp);
+    }
+    if (eqRate > 0.0) p->damage += (eqRate * dt) / fmax(mat->failure_strain, 1e-3);
+    if (p->damage > 0.7 * mat->damage_threshold && p->damage < mat->damage_threshold) {
+        double degrade = (mat->damage_threshold - p->damage) / (mat->damage_threshold * 0.3);
+        degrade = fmax(0.0, degrade);
+        for (int c = 0; c < 6; ++c) { p->s[c] *= degrade; p->a[c] *= degrade; }
+        p->residualStress = fmax(p->residualStress,
+            orc_equivalent_stress(p->s[0], p->s[1], p->s[2], p->s[3], p->s[4], p->s[5]));
+    }
+}
+
+/* Chip transition at the end of mpmG2PKernel, MPMWorkpieceSolver.cu:384-397 */
+static void chip_transition(const orc_params *mat, orc_particle *p) {
+    if (p->status == ORC_ACTIVE &&
+        (p->damage >= mat->damage_threshold || p->plasticStrain >= mat->critical_plastic_strain ||
+         p->temperature >= mat->melt_temp * mat->thermal_chip_ratio)) {
+        p->status = ORC_CHIP;
+        p->residualStress = fmax(p->residualStress,
+            orc_equivalent_stress(p->s[0], p->s[1], p->s[2], p->s[3], p->s[4], p->s[5]));
+        for (int c = 0; c < 6; ++c) { p->s[c] = 0.0; p->a[c] = 0.0; }
+        p->damage = fmax(p->damage, mat->damage_threshold);
+    }
+}
+
+void orc_constitutive(const orc_params *p, orc_particle *pt, const double L[9], double dt) {
+    if (pt->status == ORC_ACTIVE) constitutive_active(p, pt, L, dt);
+    chip_transition(p, pt);
+}
+
+/* mpmG2PKernel, MPMWorkpieceSolver.cu:264-400 */
+static void g2p(orc_solver *s, double dt) {
+    const orc_grid *g = &s->g;
+    const orc_params *mat = &s->prm;
+    const int nt = g_threads;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
+#endif
+    for (int pi = 0; pi < s->n; ++pi) {
+        orc_particle *p = &s->p[pi];
+        if (p->status == ORC_INACTIVE) continue;
+        int32_t ids[8];
+        double ws[8], gr[24];
+        orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
+        double v[3] = {0, 0, 0}, temp = 0.0, wsum = 0.0;
+        double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int n = 0; n < 8; ++n) {
+            int gid = ids[n];
+            if (gid < 0 || ws[n] <= 0.0) continue;
+            const orc_node *nd = &s->grid[gid];
+            if (nd->mass <= 1e-18) continue;
+            double nv[3] = {nd->vx, nd->vy, nd->vz};
+            v[0] += nv[0] * ws[n]; v[1] += nv[1] * ws[n]; v[2] += nv[2] * ws[n];
+            temp += (nd->tm / nd->mass) * ws[n];
+            wsum += ws[n];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) L[3 * a + b] += nv[a] * gr[3 * n + b];
+        }
+        if (p->status == ORC_FIXED_BOUNDARY) {
+            p->vx = p->vy = p->vz = 0.0;
+            p->fx = p->fy = p->fz = 0.0;
+            continue;
+        }
+        p->vx = v[0]; p->vy = v[1]; p->vz = v[2];
+        p->x += p->vx * dt;
+        p->y += p->vy * dt;
+        p->z += p->vz * dt;
+        if (wsum > 1e-8) p->temperature = fmin(fmax(temp / wsum, mat->ref_temp), mat->melt_temp);
+        if (p->status == ORC_ACTIVE) constitutive_active(mat, p, L, dt);
+        chip_transition(mat, p);
+        p->fx = p->fy = p->fz = 0.0;
+    }
+}
+
+/* chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725 */
+static void chip_fracture(orc_solver *s) {
+    const orc_params *mat = &s->prm;
+    double cell = s->g.cell, tipz = s->tip[2];
+    double dth = mat->damage_threshold;
+    for (int pi = 0; pi < s->n; ++pi) {
+        orc_particle *p = &s->p[pi];
+        if (p->status == ORC_INACTIVE) continue;
+        if (p->status == ORC_ACTIVE) {
+            double cutDepth = cell * 1.5;
+            if (p->z < tipz - cutDepth && p->plasticStrain > 1e-6) {
+                p->status = ORC_CHIP;
+                p->residualStress = fmax(p->residualStress,
+                    sqrt(p->s[0] * p->s[0] + p->s[1] * p->s[1] + p->s[2] * p->s[2]));
+                for (int c = 0; c < 6; ++c) { p->s[c] = 0.0; p->a[c] = 0.0; }
+                p->damage = fmax(p->damage, dth);
+            }
+            continue;
+        }
+        if (p->status == ORC_CHIP) {
+            if (p->damage < dth * 1.5) continue;
+            double r[3];
+            orc_chip_kick(p->id, r);
+            double speed = sqrt(p->vx * p->vx + p->vy * p->vy + p->vz * p->vz);
+            double mag = speed * 0.3 + 0.5;
+            p->vx += r[0] * mag;
+            p->vy += r[1] * mag;
+            p->vz += r[2] * mag;
+            p->damage = dth * 0.5;
+        }
+    }
+}
+
+/* updateMetrics, MPMWorkpieceSolver.cu:855-868 */
+static void update_metrics(orc_solver *s) {
+    s->max_stress = 0.0;
+    s->max_temp = s->prm.ref_temp;
+    s->ke = 0.0;
+    for (int i = 0; i < s->n; ++i) {
+        const orc_particle *p = &s->p[i];
+        if (p->status == ORC_INACTIVE) continue;
+        double st = orc_equivalent_stress(p->s[0], p->s[1], p->s[2], p->s[3], p->s[4], p->s[5]);
+        s->max_stress = fmax(s->max_stress, st);
+        s->max_temp = fmax(s->max_temp, p->temperature);
+        s->ke += 0.5 * p->mass * (p->vx * p->vx + p->vy * p->vy + p->vz * p->vz);
+    }
+}
+
+void orc_sync_metrics(orc_solver *s) { update_metrics(s); }
+
+/* compactParticles, MPMWorkpieceSolver.cu:475-527 (order-preserving) */
+void orc_compact(orc_solver *s) {
+    if (s->n <= 0) return;
+    int k = 0;
+    for (int i = 0; i < s->n; ++i)
+        if (s->p[i].status != ORC_INACTIVE) s->p[k++] = s->p[i];
+    if (k <= 0) return;
+    s->n = k;
+    s->compactions++;
+}
+
+/* MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791 */
+void orc_step(orc_solver *s, double dt) {
+    if (s->n <= 0 || !s->grid) return;
+    if (s->step % 25 == 0) rebuild_grid(s);
+    memset(s->grid, 0, sizeof(orc_node) * (size_t)s->nnodes);   /* clearMPMGridKernel :92-96 */
+    p2g(s);
+    memcpy(s->snap, s->grid, sizeof(orc_node) * (size_t)s->nnodes);
+    double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.002;
+    grid_update(s, fixture, dt);
+    if (s->prm.specific_heat > 0.0) {
+        double alpha = s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat);
+        if (alpha > 0.0) diffuse(s, alpha, dt);
+    }
+    g2p(s, dt);
+    if (s->step % 2 == 0) chip_fracture(s);
+    /* kinematic tool: the pose advances with its own velocity */
+    s->pose.center[0] += s->pose.vel[0] * dt;
+    s->pose.center[1] += s->pose.vel[1] * dt;
+    s->pose.center[2] += s->pose.vel[2] * dt;
+    s->pose.angle += s->pose.omega * dt;
+    s->time += dt;
+    ++s->step;
+    if (s->step % 25 == 0) update_metrics(s);
+    if (s->step % 1000 == 0 && s->step > 0) orc_compact(s);
+}
+
+void orc_run(orc_solver *s, int nsteps, double dt) {
+    for (int i = 0; i < nsteps; ++i) orc_step(s, dt);
+}
+
+/* getStableTimeStep, MPMWorkpieceSolver.cu:820-828 */
+double orc_stable_dt(const orc_solver *s) {
+    double wave = sqrt(fmax(s->prm.young / fmax(s->prm.density, 1.0), 1.0));
+    double dt = 0.35 * s->g.cell / wave;
+    if (s->prm.has_config) dt = fmax(s->prm.min_time_step, fmin(s->prm.max_time_step, dt));
+    return dt;
+}
+
+void orc_get_metrics(const orc_solver *s, orc_metrics *m) {
+    m->max_stress = s->max_stress;
+    m->max_temperature = s->max_temp;
+    m->kinetic_energy = s->ke;
+    m->contact_force = s->contact_force;
+    m->time = s->time;
+    m->step = s->step;
+}
+
+/* reset, MPMWorkpieceSolver.cu:830-841 */
+void orc_reset(orc_solver *s) {
+    s->n = 0;
+    s->ntool = 0;
+    s->tool.kind = ORC_TOOL_NONE;
+    s->time = 0.0;
+    s->step = 0;
+    s->max_stress = 0.0;
+    s->max_temp = s->prm.ref_temp;
+    s->ke = 0.0;
+    s->compactions = 0;
+    s->tip[0] = s->tip[1] = s->tip[2] = 0.0;
+}
+
+/* getBounds, MPMWorkpieceSolver.cu:843-853 (over the current particles) */
+void orc_get_bounds(const orc_solver *s, double b[6]) {
+    b[0] = b[1] = b[2] = 1e30;
+    b[3] = b[4] = b[5] = -1e30;
+    for (int i = 0; i < s->n; ++i) {
+        const orc_particle *p = &s->p[i];
+        if (p->status == ORC_INACTIVE) continue;
+        b[0] = fmin(b[0], p->x); b[1] = fmin(b[1], p->y); b[2] = fmin(b[2], p->z);
+        b[3] = fmax(b[3], p->x); b[4] = fmax(b[4], p->y); b[5] = fmax(b[5], p->z);
+    }
+    if (s->n == 0) b[0] = b[1] = b[2] = b[3] = b[4] = b[5] = 0.0;
+}
+
+/* getTemperatureAt, MPMWorkpieceSolver.cu:936-946 */
+double orc_temperature_at(const orc_solver *s, double x, double y, double z) {
+    double best = 1e30, temp = s->prm.ref_temp;
+    for (int i = 0; i < s->n; ++i) {
+        const orc_particle *p = &s->p[i];
+        double dx = p->x - x, dy = p->y - y, dz = p->z - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < best) { best = d2; temp = p->temperature; }
+    }
+    return temp;
+}
+
+/* applyHeatFlux, MPMWorkpieceSolver.cu:918-934 */
+void orc_apply_heat_flux(orc_solver *s, double x, double y, double z, double q) {
+    double best = 1e30;
+    int nearest = -1;
+    for (int i = 0; i < s->n; ++i) {
+        const orc_particle *p = &s->p[i];
+        double dx = p->x - x, dy = p->y - y, dz = p->z - z;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < best) { best = d2; nearest = i; }
+    }
+    if (nearest >= 0 && best <= s->g.cell * s->g.cell) {
+        orc_particle *p = &s->p[nearest];
+        p->temperature = fmin(p->temperature + q / fmax(p->mass * s->prm.specific_heat, 1e-12),
+                              s->prm.melt_temp);
+    }
+}
+
+/* setParticleTemperatures, MPMWorkpieceSolver.cu:906-916 */
+void orc_set_particle_temperatures(orc_solver *s, const double *t, int count) {
+    if (count <= 0 || count > s->n) return;
+    for (int i = 0; i < count; ++i) s->p[i].temperature = t[i];
+}
+
+void orc_get_grid(const orc_solver *s, orc_grid *g) { *g = s->g; }
+int orc_grid_node_count(const orc_solver *s) { return s->nnodes; }
+
+/* Grid after the last step: post-update velocity and post-diffusion T*m. */
+int orc_get_grid_state(const orc_solver *s, double *mass, double *vel, double *tm, int32_t *fixed) {
+    for (int i = 0; i < s->nnodes; ++i) {
+        const orc_node *n = &s->grid[i];
+        if (mass) mass[i] = n->mass;
+        if (vel) { vel[3 * i] = n->vx; vel[3 * i + 1] = n->vy; vel[3 * i + 2] = n->vz; }
+        if (tm) tm[i] = n->tm;
+        if (fixed) fixed[i] = n->fixed;
+    }
+    return s->nnodes;
+}
+
+int orc_get_grid_snapshot(const orc_solver *s, double *mass, double *mom, double *force, double *tm) {
+    for (int i = 0; i < s->nnodes; ++i) {
+        const orc_node *n = &s->snap[i];
+        if (mass) mass[i] = n->mass;
+        if (mom) { mom[3 * i] = n->vx; mom[3 * i + 1] = n->vy; mom[3 * i + 2] = n->vz; }
+        if (force) { force[3 * i] = n->fx; force[3 * i + 1] = n->fy; force[3 * i + 2] = n->fz; }
+        if (tm) tm[i] = n->tm;
+    }
+    return s->nnodes;
+}
+
+/* ------------------------------------------------------- input generator -- */
+
+static uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double uniform01(uint64_t seed, uint64_t counter) {
+    return (double)(splitmix64(seed ^ (counter * 0xD1B54A32D192ED03ull)) >> 11) * 0x1.0p-53;
+}
+
+/* Slab of cx*cy*cz cells, 8 particles per cell (one per 2x2x2 sub-cell) at
+ * sub-cell origin + (h/2)*U[0,1)^3, positions rounded to the nearest float. */
+void orc_gen_jittered_slab(uint64_t seed, int cx, int cy, int cz, double h, const double origin[3],
+                           double density, double fixture, double *pos, double *mass,
+                           double *volume, int32_t *id, int32_t *status) {
+    double vol = h * h * h / 8.0;
+    for (int64_t c = 0; c < (int64_t)cx * cy * cz; ++c) {
+        int ci = (int)(c % cx), cj = (int)((c / cx) % cy), ck = (int)(c / ((int64_t)cx * cy));
+        for (int k = 0; k < 8; ++k) {
+            int64_t pi = c * 8 + k;
+            int sub[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
+            int cell[3] = {ci, cj, ck};
+            for (int a = 0; a < 3; ++a) {
+                double u = uniform01(seed, (uint64_t)(pi * 3 + a));
+                double x = origin[a] + ((double)cell[a] + 0.5 * sub[a] + 0.5 * u) * h;
+                pos[3 * pi + a] = (double)(float)x;
+            }
+            mass[pi] = density * vol;
+            volume[pi] = vol;
+            id[pi] = (int32_t)pi;
+            status[pi] = (pos[3 * pi + 2] <= origin[2] + fixture) ? ORC_FIXED_BOUNDARY : ORC_ACTIVE;
+        }
+    }
+}

@@ -1,0 +1,171 @@
+/*
+ * mpm_oracle.h -- CPU FP64 restatement of the reference MPM workpiece step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP solver:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product library never links or calls it.
+ *
+ * It restates src/MPMWorkpieceSolver.cu (Up25-pathan/edegpredict-engine-rotary)
+ * in plain C99, array-of-structs, FP64, serial by default (OpenMP optional for
+ * the CPU baseline).  Each function in mpm_oracle.c cites the reference
+ * file:line it follows.
+ */
+#ifndef MPM_ORACLE_H
+#define MPM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ParticleStatus values, include/Types.h:37-43 */
+enum { ORC_ACTIVE = 0, ORC_INACTIVE = 1, ORC_CHIP = 2, ORC_BOUNDARY = 3, ORC_FIXED_BOUNDARY = 4 };
+
+/* Tool kinds (none / reference point samples / analytic wedge / analytic drill) */
+enum { ORC_TOOL_NONE = 0, ORC_TOOL_POINTS = 1, ORC_TOOL_WEDGE = 2, ORC_TOOL_DRILL = 3 };
+
+/* MPMParticleGPU, include/MPMWorkpieceSolver.cuh:18-45 (field order kept) */
+typedef struct {
+    double x, y, z;
+    double vx, vy, vz;
+    double fx, fy, fz;
+    double mass, volume, density, temperature;
+    double plasticStrain, strainRate, damage, residualStress;
+    double s[6];   /* stress xx yy zz xy xz yz */
+    double a[6];   /* backstress xx yy zz xy xz yz */
+    int32_t id;
+    int32_t status;
+} orc_particle;
+
+/* MPMGridNodeGPU, include/MPMWorkpieceSolver.cuh:47-57 */
+typedef struct {
+    double mass, vx, vy, vz, fx, fy, fz, tm;
+    int32_t fixed;
+} orc_node;
+
+/* MPMGridParams, include/MPMWorkpieceSolver.cuh:59-63 */
+typedef struct {
+    double cell;
+    double minx, miny, minz;
+    int32_t nx, ny, nz;
+} orc_grid;
+
+/* Solver parameters: MPMMaterialParams (MPMWorkpieceSolver.cuh:65-90) after
+ * MPMWorkpieceSolver::initialize (MPMWorkpieceSolver.cu:422-458), plus the
+ * Config fields the step reads through its stored Config pointer. */
+typedef struct {
+    double density, young, poisson, specific_heat, melt_temp, ref_temp;
+    double jc_A, jc_B, jc_n, jc_C, jc_m, ref_strain_rate;
+    double failure_strain, damage_threshold, critical_plastic_strain, thermal_chip_ratio;
+    double tool_contact_radius, contact_damping, max_contact_temp_rise;
+    double thermal_conductivity, kinematic_hardening;
+    int32_t thermal_clamp_enabled;
+    int32_t has_config;            /* m_config != nullptr */
+    double cell_size;              /* smoothingRadius */
+    double particle_spacing_factor;
+    double fixture_thickness;      /* MachineSetupParams::fixtureLayerThickness */
+    double min_time_step, max_time_step;
+    int32_t max_particles;
+    int32_t pad_;
+} orc_params;
+
+/* Analytic tool description (grid-node SDF collision, extension of
+ * ContactSolver.cu:797-923 semantics to grid nodes). */
+typedef struct {
+    int32_t kind;
+    int32_t pad_;
+    double band;             /* contact when phi < band */
+    double friction;         /* Coulomb coefficient */
+    double grad_eps;         /* central-difference step for the normal */
+    /* wedge */
+    double rake_angle, clearance_angle;
+    /* drill */
+    double radius, helix_angle, point_angle, flute_offset, flute_radius, flute_length;
+} orc_tool_desc;
+
+typedef struct {
+    double center[3];
+    double angle;            /* rotation about +z (rad) */
+    double vel[3];           /* linear velocity of the tool frame (m/s) */
+    double omega;            /* spin about the +z axis through center (rad/s) */
+} orc_tool_pose;
+
+typedef struct {
+    int32_t n;
+    int32_t pad_;
+    double *pos, *vel, *fext;               /* 3n each; fext may be NULL */
+    double *mass, *volume, *density, *temperature;
+    double *plastic_strain, *strain_rate, *damage, *residual_stress;
+    double *stress, *backstress;            /* 6n each; backstress may be NULL */
+    int32_t *id, *status;
+} orc_particle_arrays;
+
+typedef struct {
+    double max_stress, max_temperature, kinetic_energy, contact_force;
+    double time;
+    int64_t step;
+} orc_metrics;
+
+typedef struct orc_solver orc_solver;
+
+void orc_params_default(orc_params *p);
+orc_solver *orc_create(const orc_params *p);
+void orc_destroy(orc_solver *s);
+void orc_set_threads(int n);
+
+int orc_set_particles(orc_solver *s, const orc_particle_arrays *a);
+int orc_get_particles(orc_solver *s, orc_particle_arrays *a);
+int orc_particle_count(const orc_solver *s);
+
+void orc_init_particle_box(orc_solver *s, const double mn[3], const double mx[3], double spacing);
+void orc_init_cylinder(orc_solver *s, const double c[3], double radius, double length,
+                       double spacing, int axis);
+
+void orc_set_tool_samples(orc_solver *s, const double *xyz, int n, double contact_radius);
+void orc_set_tool_analytic(orc_solver *s, const orc_tool_desc *d);
+void orc_set_tool_pose(orc_solver *s, const orc_tool_pose *p);
+void orc_set_tool_tip(orc_solver *s, double x, double y, double z);
+void orc_apply_external_force(orc_solver *s, const double c[3], double radius, const double f[3]);
+
+void orc_step(orc_solver *s, double dt);
+void orc_run(orc_solver *s, int nsteps, double dt);
+double orc_stable_dt(const orc_solver *s);
+void orc_get_metrics(const orc_solver *s, orc_metrics *m);
+void orc_sync_metrics(orc_solver *s);
+void orc_compact(orc_solver *s);
+void orc_reset(orc_solver *s);
+void orc_get_bounds(const orc_solver *s, double b[6]);
+double orc_temperature_at(const orc_solver *s, double x, double y, double z);
+void orc_apply_heat_flux(orc_solver *s, double x, double y, double z, double q);
+void orc_set_particle_temperatures(orc_solver *s, const double *t, int count);
+
+void orc_get_grid(const orc_solver *s, orc_grid *g);
+/* Post-P2G grid snapshot of the last step (mass, momentum xyz, force xyz, T*m). */
+int orc_get_grid_snapshot(const orc_solver *s, double *mass, double *mom, double *force, double *tm);
+int orc_grid_node_count(const orc_solver *s);
+/* Grid after the last step (post-update velocity, post-diffusion T*m). */
+int orc_get_grid_state(const orc_solver *s, double *mass, double *vel, double *tm, int32_t *fixed);
+
+/* Single-function entry points for known-answer tests. */
+void orc_weights(const orc_grid *g, double x, double y, double z,
+                 int32_t idx[8], double w[8], double grad[24]);
+double orc_equivalent_stress(double xx, double yy, double zz, double xy, double xz, double yz);
+double orc_jc_flow(const orc_params *p, double eps_p, double eps_rate, double temp);
+void orc_rebuild_params(double cell, const double mn[3], const double mx[3], orc_grid *g);
+void orc_chip_kick(int32_t id, double r[3]);
+double orc_tool_phi(const orc_tool_desc *d, const orc_tool_pose *p, double x, double y, double z);
+void orc_tool_normal(const orc_tool_desc *d, const orc_tool_pose *p, double x, double y, double z,
+                     double n[3]);
+/* One G2P constitutive update of a single particle from a prescribed
+ * velocity gradient (row = velocity component, column = d/dx_b). */
+void orc_constitutive(const orc_params *p, orc_particle *pt, const double L[9], double dt);
+/* Jittered slab generator (config inputs), float-exact positions. */
+void orc_gen_jittered_slab(uint64_t seed, int cx, int cy, int cz, double h, const double origin[3],
+                           double density, double fixture, double *pos, double *mass,
+                           double *volume, int32_t *id, int32_t *status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
