@@ -1,0 +1,271 @@
+// mpmw_device.hpp -- device-side data views and per-particle / per-node math
+// of the MPM workpiece step, templated on the arithmetic type R (float for the
+// FP32 bulk path, double for the full-FP64 path).
+//
+// Semantics follow src/MPMWorkpieceSolver.cu of the reference (cited per
+// function); layout is MI355X-first: structure-of-arrays particle state,
+// structure-of-arrays grid accumulators, a float4/double4 node record for the
+// gather, and all reductions done as fixed-order block partials.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpmw {
+
+enum : int32_t { kActive = 0, kInactive = 1, kChip = 2, kBoundary = 3, kFixed = 4 };
+enum : int32_t { kToolNone = 0, kToolPoints = 1, kToolWedge = 2, kToolDrill = 3 };
+
+template <typename R> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+template <typename R> using vec4_t = typename Vec4<R>::type;
+
+// Material constants after MPMWorkpieceSolver::initialize
+// (MPMWorkpieceSolver.cu:428-451), in kernel precision.
+template <typename R> struct Material {
+    R shear, bulk, density, cp, melt, ref_temp;
+    R jcA, jcB, jcN, jcC, jcM, ref_rate;
+    R failure_strain, dth, eps_crit, chip_ratio;
+    R rc, c_damp, dT_max, H;
+    int32_t clamp;
+};
+
+// MPMGridParams (MPMWorkpieceSolver.cuh:59-63) plus the integer fixture
+// threshold: nodes with k <= kfix satisfy the reference's FP64 test
+// pos.z <= minZ + fixture + h (MPMWorkpieceSolver.cu:153), evaluated on the host.
+template <typename R> struct GridView {
+    R minx, miny, minz, h, inv_h;
+    int32_t nx, ny, nz, kfix;
+    int64_t nn;
+};
+
+// Particle state, structure of arrays.  a0 == nullptr means all backstress is
+// zero; fx == nullptr means no external force.
+template <typename R> struct Particles {
+    R *x, *y, *z, *vx, *vy, *vz;
+    R *s0, *s1, *s2, *s3, *s4, *s5;
+    R *a0, *a1, *a2, *a3, *a4, *a5;
+    R *fx, *fy, *fz;
+    R *eps, *rate, *dmg, *sres, *T, *m, *V;
+    int32_t *status, *id;
+    int32_t n;
+};
+
+// Grid arrays.  acc: 8 SoA channels (mass, p.xyz, f.xyz, T*m) that P2G
+// accumulates into.  node: (v.xyz, T) for the gather, with T = NaN marking a
+// node of mass <= 1e-18 (skipped by G2P, MPMWorkpieceSolver.cu:286).
+// gm/gtm: post-update mass and T*m read by the diffusion stencil.
+template <typename R> struct Grid {
+    R *acc;
+    vec4_t<R> *node;
+    R *gm, *gtm;
+    R *snap;  // optional copy of acc (8 channels) for parity inspection
+};
+
+// Tool state for the grid update.
+template <typename R> struct Tool {
+    int32_t kind, nsamples;
+    const R *sx, *sy, *sz;            // point samples (SoA)
+    R lo[3], hi[3];                   // sample AABB grown by r_c (conservative)
+    R rc2;
+    // analytic
+    R cx, cy, cz, cos_a, sin_a;       // pose
+    R vx, vy, vz, omega;              // tool velocity
+    R band, mu, eps;
+    R sin_rake, cos_rake, sin_clear, cos_clear;
+    R radius, tan_helix, sin_half_point, cos_half_point, flute_offset, flute_radius, flute_length;
+};
+
+// --------------------------------------------------------------- helpers --
+
+template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
+template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return fmin(a, b); }
+
+// equivalentStress, MPMWorkpieceSolver.cu:18-24
+template <typename R>
+__device__ __forceinline__ R von_mises(R xx, R yy, R zz, R xy, R xz, R yz) {
+    R s1 = xx - yy, s2 = yy - zz, s3 = zz - xx;
+    return sqrt(R(0.5) * (s1 * s1 + s2 * s2 + s3 * s3 + R(6) * (xy * xy + xz * xz + yz * yz)));
+}
+
+// johnsonCookStress, MPMWorkpieceSolver.cu:43-50
+template <typename R>
+__device__ __forceinline__ R jc_flow(const Material<R> &M, R eps, R rate, R T) {
+    R strainTerm = M.jcA + M.jcB * pow(rmax(eps, R(0)), M.jcN);
+    R rateRatio = rmax(rate / rmax(M.ref_rate, R(1e-9)), R(1));
+    R rateTerm = R(1) + M.jcC * log(rateRatio);
+    R tStar = rmax(R(0), rmin(R(1), (T - M.ref_temp) / rmax(M.melt - M.ref_temp, R(1))));
+    R thermalTerm = rmax(R(0.01), R(1) - pow(tStar, M.jcM));
+    return strainTerm * rateTerm * thermalTerm;
+}
+
+// Base cell and fractions of computeWeights, MPMWorkpieceSolver.cu:54-62.
+template <typename R>
+__device__ __forceinline__ void base_and_frac(const GridView<R> &g, R x, R y, R z, int &bx, int &by,
+                                              int &bz, R &fx, R &fy, R &fz) {
+    R rx = (x - g.minx) / g.h, ry = (y - g.miny) / g.h, rz = (z - g.minz) / g.h;
+    R flx = floor(rx), fly = floor(ry), flz = floor(rz);
+    bx = (int)flx; by = (int)fly; bz = (int)flz;
+    fx = rx - (R)bx; fy = ry - (R)by; fz = rz - (R)bz;
+}
+
+// One of the 8 trilinear nodes (dz outer, dy, dx inner), :64-89.
+template <typename R>
+__device__ __forceinline__ bool corner(const GridView<R> &g, int bx, int by, int bz, R fx, R fy, R fz,
+                                       int n, int64_t &idx, R &w, R &gx, R &gy, R &gz) {
+    int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
+    int i = bx + dx, j = by + dy, k = bz + dz;
+    if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
+    R wx = dx ? fx : R(1) - fx;
+    R wy = dy ? fy : R(1) - fy;
+    R wz = dz ? fz : R(1) - fz;
+    R dwx = dx ? g.inv_h : -g.inv_h;
+    R dwy = dy ? g.inv_h : -g.inv_h;
+    R dwz = dz ? g.inv_h : -g.inv_h;
+    idx = ((int64_t)k * g.ny + j) * g.nx + i;
+    w = wx * wy * wz;
+    gx = dwx * wy * wz;
+    gy = wx * dwy * wz;
+    gz = wx * wy * dwz;
+    return w > R(0);
+}
+
+// --------------------------------------------------------------- tools ----
+
+template <typename R> __device__ __forceinline__ void tool_frame(const Tool<R> &t, R x, R y, R z, R q[3]) {
+    R dx = x - t.cx, dy = y - t.cy;
+    q[0] = dx * t.cos_a + dy * t.sin_a;
+    q[1] = -dx * t.sin_a + dy * t.cos_a;
+    q[2] = z - t.cz;
+}
+
+template <typename R> __device__ __forceinline__ R phi_local(const Tool<R> &t, const R q[3]) {
+    if (t.kind == kToolWedge) {
+        R pr = t.cos_rake * q[0] + t.sin_rake * q[2];
+        R pf = -t.sin_clear * q[0] - t.cos_clear * q[2];
+        return rmax(pr, pf);
+    }
+    // two-flute helical drill
+    R r = sqrt(q[0] * q[0] + q[1] * q[1]);
+    R tt = r * t.sin_half_point + q[2] * t.cos_half_point;
+    R cone = (tt < R(0)) ? sqrt(r * r + q[2] * q[2]) : (r * t.cos_half_point - q[2] * t.sin_half_point);
+    R body = rmax(rmax(cone, r - t.radius), q[2] - t.flute_length);
+    R psi = q[2] * t.tan_helix / t.radius;
+    R gmin = R(1e30);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        R ang = psi + R(k) * R(3.14159265358979323846);
+        R fcx = t.flute_offset * cos(ang), fcy = t.flute_offset * sin(ang);
+        R ex = q[0] - fcx, ey = q[1] - fcy;
+        gmin = rmin(gmin, sqrt(ex * ex + ey * ey) - t.flute_radius);
+    }
+    return rmax(body, -gmin);
+}
+
+template <typename R> __device__ __forceinline__ R tool_phi(const Tool<R> &t, R x, R y, R z) {
+    R q[3];
+    tool_frame(t, x, y, z, q);
+    return phi_local(t, q);
+}
+
+// ---------------------------------------------------------- constitutive --
+
+// Register copy of one particle's mutable state.
+template <typename R> struct PState {
+    R x, y, z, vx, vy, vz;
+    R s[6], a[6];
+    R eps, rate, dmg, sres, T;
+    int32_t status;
+};
+
+// ACTIVE-particle block of mpmG2PKernel, MPMWorkpieceSolver.cu:308-382.
+template <typename R>
+__device__ __forceinline__ void constitutive_active(const Material<R> &M, PState<R> &p, const R L[9], R dt,
+                                                    bool has_back) {
+    R Dxx = L[0], Dyy = L[4], Dzz = L[8];
+    R Dxy = R(0.5) * (L[1] + L[3]);
+    R Dxz = R(0.5) * (L[2] + L[6]);
+    R Dyz = R(0.5) * (L[5] + L[7]);
+    R tr = Dxx + Dyy + Dzz;
+    R devXX = Dxx - tr / R(3), devYY = Dyy - tr / R(3), devZZ = Dzz - tr / R(3);
+    R eqRate = sqrt(R(2) / R(3) *
+                    (devXX * devXX + devYY * devYY + devZZ * devZZ + R(2) * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz)));
+    p.rate = eqRate;
+    R pressureInc = M.bulk * tr * dt;
+    p.s[0] += (R(2) * M.shear * devXX) * dt + pressureInc;
+    p.s[1] += (R(2) * M.shear * devYY) * dt + pressureInc;
+    p.s[2] += (R(2) * M.shear * devZZ) * dt + pressureInc;
+    p.s[3] += R(2) * M.shear * Dxy * dt;
+    p.s[4] += R(2) * M.shear * Dxz * dt;
+    p.s[5] += R(2) * M.shear * Dyz * dt;
+    R sh[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sh[c] = p.s[c] - p.a[c];
+    R seq = von_mises(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]);
+    R flow = jc_flow(M, p.eps, p.rate, p.T);
+    if (seq > flow && seq > R(1e-9)) {
+        R scale = flow / seq;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p.s[c] = p.a[c] + scale * sh[c];
+        R dp = (seq - flow) / rmax(R(3) * M.shear, R(1));
+        if (has_back && M.H > R(0) && dp > R(0)) {
+            R ns = M.H * dp / seq;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) p.a[c] += ns * sh[c];
+        }
+        p.eps += dp;
+        R dT = R(0.9) * flow * dp / rmax(M.density * M.cp, R(1));
+        if (M.clamp) dT = rmin(dT, M.dT_max);
+        p.T = rmin(p.T + dT, M.melt);
+    }
+    if (eqRate > R(0)) p.dmg += (eqRate * dt) / rmax(M.failure_strain, R(1e-3));
+    if (p.dmg > R(0.7) * M.dth && p.dmg < M.dth) {
+        R degrade = (M.dth - p.dmg) / (M.dth * R(0.3));
+        degrade = rmax(R(0), degrade);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { p.s[c] *= degrade; p.a[c] *= degrade; }
+        p.sres = rmax(p.sres, von_mises(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]));
+    }
+}
+
+// Chip transition, MPMWorkpieceSolver.cu:384-397.
+template <typename R> __device__ __forceinline__ void chip_transition(const Material<R> &M, PState<R> &p) {
+    if (p.status == kActive && (p.dmg >= M.dth || p.eps >= M.eps_crit || p.T >= M.melt * M.chip_ratio)) {
+        p.status = kChip;
+        p.sres = rmax(p.sres, von_mises(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]));
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { p.s[c] = R(0); p.a[c] = R(0); }
+        p.dmg = rmax(p.dmg, M.dth);
+    }
+}
+
+// chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725.
+template <typename R>
+__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, int32_t id, R zcut) {
+    if (p.status == kActive) {
+        if (p.z < zcut && p.eps > R(1e-6)) {
+            p.status = kChip;
+            p.sres = rmax(p.sres, sqrt(p.s[0] * p.s[0] + p.s[1] * p.s[1] + p.s[2] * p.s[2]));
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { p.s[c] = R(0); p.a[c] = R(0); }
+            p.dmg = rmax(p.dmg, M.dth);
+        }
+        return;
+    }
+    if (p.status == kChip) {
+        if (p.dmg < M.dth * R(1.5)) return;
+        uint32_t seed = (uint32_t)id * 2654435761u;
+        R rx = (R)((seed >> 16) & 0xFFu) / R(255) - R(0.5);
+        R ry = (R)((seed >> 8) & 0xFFu) / R(255) - R(0.5);
+        R rz = (R)(seed & 0xFFu) / R(255) - R(0.5);
+        R speed = sqrt(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+        R mag = speed * R(0.3) + R(0.5);
+        p.vx += rx * mag;
+        p.vy += ry * mag;
+        p.vz += rz * mag;
+        p.dmg = M.dth * R(0.5);
+    }
+}
+
+}  // namespace mpmw

@@ -1,0 +1,499 @@
+// mpmw_kernels.hpp -- HIP kernels of the MPM workpiece step for gfx950.
+//
+// Kernel map (reference kernel -> here):
+//   clearMPMGridKernel          -> the grid update zeroes the accumulators it has read
+//   mpmP2GKernel                -> k_p2g (standalone) and the P2G tail of k_g2p
+//   mpmGridUpdateKernel         -> k_grid_update
+//   mpmDiffuseTemperatureKernel -> k_diffuse (race-free Jacobi form)
+//   mpmG2PKernel + chipFractureAndSeparationKernel -> k_g2p (one particle pass)
+//   updateMetrics / rebuild AABB host loops -> block partials in k_g2p / k_reduce
+#pragma once
+
+#include "mpmw_device.hpp"
+
+namespace mpmw {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kRed = 9;  // AABB min xyz, -max xyz, max von Mises, max T, kinetic energy
+
+// ------------------------------------------------------------ reductions --
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Fixed-order block reduction of the kRed particle statistics into
+// part[blockIdx.x * kRed + c].  Slots 0-5 and 6-7 reduce with min/max, slot 8
+// with a sum whose order depends only on thread and block indices.
+__device__ __forceinline__ void block_reduce_stats(double v[kRed], double *part) {
+    __shared__ double sh[kBlock / kWave][kRed];
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int c = 0; c < kRed; ++c) {
+        double r = (c < 6) ? wave_min(v[c]) : (c < 8 ? wave_max(v[c]) : wave_sum(v[c]));
+        if (lane == 0) sh[wid][c] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRed) {
+        const int c = threadIdx.x;
+        double r = sh[0][c];
+        for (int w = 1; w < kBlock / kWave; ++w)
+            r = (c < 6) ? fmin(r, sh[w][c]) : (c < 8 ? fmax(r, sh[w][c]) : r + sh[w][c]);
+        part[(int64_t)blockIdx.x * kRed + c] = r;
+    }
+}
+
+__device__ __forceinline__ void stats_identity(double v[kRed], double ref_temp) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] = 1e30;
+    v[6] = 0.0;
+    v[7] = ref_temp;
+    v[8] = 0.0;
+}
+
+// Single-block fixed-order reduction of nparts partial records of width w.
+// mode per column: 0 min, 1 max, 2 sum.
+__global__ void __launch_bounds__(kBlock) k_finalize(const double *part, int nparts, int w,
+                                                     const int32_t *modes, double *out) {
+    __shared__ double sh[kBlock / kWave];
+    for (int c = 0; c < w; ++c) {
+        const int mode = modes[c];
+        double r = mode == 0 ? 1e300 : (mode == 1 ? -1e300 : 0.0);
+        for (int i = threadIdx.x; i < nparts; i += kBlock) {
+            double x = part[(int64_t)i * w + c];
+            r = mode == 0 ? fmin(r, x) : (mode == 1 ? fmax(r, x) : r + x);
+        }
+        r = mode == 0 ? wave_min(r) : (mode == 1 ? wave_max(r) : wave_sum(r));
+        if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = r;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = sh[0];
+            for (int k = 1; k < kBlock / kWave; ++k)
+                t = mode == 0 ? fmin(t, sh[k]) : (mode == 1 ? fmax(t, sh[k]) : t + sh[k]);
+            out[c] = t;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R> __global__ void k_fill(R *p, int64_t n, R v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// AABB + metrics over non-INACTIVE particles (rebuildGridFromHostBounds
+// :572-577 and updateMetrics :855-868).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_temp, double *part) {
+    double v[kRed];
+    stats_identity(v, ref_temp);
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += gridDim.x * kBlock) {
+        if (P.status[i] == kInactive) continue;
+        double x = P.x[i], y = P.y[i], z = P.z[i];
+        v[0] = fmin(v[0], x); v[1] = fmin(v[1], y); v[2] = fmin(v[2], z);
+        v[3] = fmin(v[3], -x); v[4] = fmin(v[4], -y); v[5] = fmin(v[5], -z);
+        double vm = (double)von_mises<R>(P.s0[i], P.s1[i], P.s2[i], P.s3[i], P.s4[i], P.s5[i]);
+        v[6] = fmax(v[6], vm);
+        v[7] = fmax(v[7], (double)P.T[i]);
+        double vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+        v[8] += 0.5 * (double)P.m[i] * (vx * vx + vy * vy + vz * vz);
+    }
+    block_reduce_stats(v, part);
+}
+
+// ------------------------------------------------------------------ P2G ---
+
+// Scatter of one particle, mpmP2GKernel :110-133, with device-wide atomics.
+template <typename R>
+__device__ __forceinline__ void p2g_particle(const GridView<R> &g, R *acc, R x, R y, R z, R vx, R vy, R vz,
+                                             R mass, R vol, R T, const R s[6], R fex, R fey, R fez,
+                                             int32_t status) {
+    int bx, by, bz;
+    R fx, fy, fz;
+    base_and_frac(g, x, y, z, bx, by, bz, fx, fy, fz);
+    const bool stress_force = (status == kActive || status == kFixed);
+    const bool ext_only = (status == kChip);
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        int64_t idx;
+        R w, gx, gy, gz;
+        if (!corner(g, bx, by, bz, fx, fy, fz, n, idx, w, gx, gy, gz)) continue;
+        R m = mass * w;
+        unsafeAtomicAdd(acc + idx, m);
+        unsafeAtomicAdd(acc + g.nn + idx, vx * m);
+        unsafeAtomicAdd(acc + 2 * g.nn + idx, vy * m);
+        unsafeAtomicAdd(acc + 3 * g.nn + idx, vz * m);
+        unsafeAtomicAdd(acc + 7 * g.nn + idx, T * m);
+        if (stress_force) {
+            R fX = -(s[0] * gx + s[3] * gy + s[4] * gz) * vol;
+            R fY = -(s[3] * gx + s[1] * gy + s[5] * gz) * vol;
+            R fZ = -(s[4] * gx + s[5] * gy + s[2] * gz) * vol;
+            unsafeAtomicAdd(acc + 4 * g.nn + idx, fX + fex * w);
+            unsafeAtomicAdd(acc + 5 * g.nn + idx, fY + fey * w);
+            unsafeAtomicAdd(acc + 6 * g.nn + idx, fZ + fez * w);
+        } else if (ext_only) {
+            unsafeAtomicAdd(acc + 4 * g.nn + idx, fex * w);
+            unsafeAtomicAdd(acc + 5 * g.nn + idx, fey * w);
+            unsafeAtomicAdd(acc + 6 * g.nn + idx, fez * w);
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, GridView<R> g, R *acc) {
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= P.n) return;
+    int32_t st = P.status[i];
+    if (st == kInactive) return;
+    R s[6] = {P.s0[i], P.s1[i], P.s2[i], P.s3[i], P.s4[i], P.s5[i]};
+    R fex = P.fx ? P.fx[i] : R(0), fey = P.fx ? P.fy[i] : R(0), fez = P.fx ? P.fz[i] : R(0);
+    p2g_particle(g, acc, P.x[i], P.y[i], P.z[i], P.vx[i], P.vy[i], P.vz[i], P.m[i], P.V[i], P.T[i], s, fex,
+                 fey, fez, st);
+}
+
+// ---------------------------------------------------------- grid update ---
+
+// mpmGridUpdateKernel, MPMWorkpieceSolver.cu:136-193 (point samples) and its
+// analytic-SDF form with tool velocity and Coulomb friction.  Reads and zeroes
+// the accumulators; writes node velocity, mass and T*m.  When `direct_T` is
+// set (no diffusion pass follows) it also writes the gather temperature.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, Material<R> M, Tool<R> tool,
+                                                        R dt, int direct_T, double *force_part) {
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    double cf = 0.0;
+    if (idx < g.nn) {
+        R mass = G.acc[idx];
+        R vx = G.acc[g.nn + idx], vy = G.acc[2 * g.nn + idx], vz = G.acc[3 * g.nn + idx];
+        R fxn = G.acc[4 * g.nn + idx], fyn = G.acc[5 * g.nn + idx], fzn = G.acc[6 * g.nn + idx];
+        R tm = G.acc[7 * g.nn + idx];
+        if (G.snap) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) G.snap[c * g.nn + idx] = G.acc[c * g.nn + idx];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) G.acc[c * g.nn + idx] = R(0);
+        bool live = mass > R(1e-18);
+        if (live) {
+            vx /= mass; vy /= mass; vz /= mass;
+            const int i = (int)(idx % g.nx);
+            const int64_t t = idx / g.nx;
+            const int j = (int)(t % g.ny), k = (int)(t / g.ny);
+            if (k <= g.kfix) {
+                vx = vy = vz = R(0);
+            } else {
+                vx += (fxn / mass) * dt;
+                vy += (fyn / mass) * dt;
+                vz += (fzn / mass) * dt;
+                const R px = g.minx + (R)i * g.h, py = g.miny + (R)j * g.h, pz = g.minz + (R)k * g.h;
+                if (tool.kind == kToolPoints) {
+                    if (px >= tool.lo[0] && px <= tool.hi[0] && py >= tool.lo[1] && py <= tool.hi[1] &&
+                        pz >= tool.lo[2] && pz <= tool.hi[2]) {
+                        R r2 = tool.rc2;
+                        R bdx = 0, bdy = 0, bdz = 0;
+                        bool hit = false;
+                        for (int s = 0; s < tool.nsamples; ++s) {
+                            R dx = px - tool.sx[s], dy = py - tool.sy[s], dz = pz - tool.sz[s];
+                            R d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 < r2 && d2 > R(1e-20)) {
+                                r2 = d2;
+                                bdx = dx; bdy = dy; bdz = dz;
+                                hit = true;
+                            }
+                        }
+                        if (hit) {
+                            R len = sqrt(bdx * bdx + bdy * bdy + bdz * bdz);
+                            R nx = R(0), ny = R(0), nz = R(0);
+                            if (len > R(1e-12)) { nx = bdx / len; ny = bdy / len; nz = bdz / len; }
+                            R vn = vx * nx + vy * ny + vz * nz;
+                            if (vn < R(0)) {
+                                cf += (double)fabs(mass * vn / rmax(dt, R(1e-15)));
+                                vx -= vn * nx; vy -= vn * ny; vz -= vn * nz;
+                                R heat = R(0.5) * mass * vn * vn * M.c_damp;
+                                R dT = heat / rmax(mass * M.cp, R(1e-12));
+                                if (M.clamp) dT = rmin(dT, M.dT_max);
+                                tm += dT * mass;
+                            }
+                        }
+                    }
+                } else if (tool.kind == kToolWedge || tool.kind == kToolDrill) {
+                    R phi = tool_phi(tool, px, py, pz);
+                    if (phi < tool.band) {
+                        const R e = tool.eps;
+                        R gx = (tool_phi(tool, px + e, py, pz) - tool_phi(tool, px - e, py, pz)) / (R(2) * e);
+                        R gy = (tool_phi(tool, px, py + e, pz) - tool_phi(tool, px, py - e, pz)) / (R(2) * e);
+                        R gz = (tool_phi(tool, px, py, pz + e) - tool_phi(tool, px, py, pz - e)) / (R(2) * e);
+                        R len = sqrt(gx * gx + gy * gy + gz * gz);
+                        if (len > R(1e-12)) {
+                            R nx = gx / len, ny = gy / len, nz = gz / len;
+                            R tvx = tool.vx - tool.omega * (py - tool.cy);
+                            R tvy = tool.vy + tool.omega * (px - tool.cx);
+                            R tvz = tool.vz;
+                            R rx = vx - tvx, ry = vy - tvy, rz = vz - tvz;
+                            R vn = rx * nx + ry * ny + rz * nz;
+                            if (vn < R(0)) {
+                                cf += (double)fabs(mass * vn / rmax(dt, R(1e-15)));
+                                R ux = rx - vn * nx, uy = ry - vn * ny, uz = rz - vn * nz;
+                                R ut2 = ux * ux + uy * uy + uz * uz;
+                                R ut = sqrt(ut2);
+                                R sc = R(1);
+                                if (tool.mu > R(0) && ut > R(0)) sc = rmax(R(0), R(1) + tool.mu * vn / ut);
+                                vx = tvx + sc * ux;
+                                vy = tvy + sc * uy;
+                                vz = tvz + sc * uz;
+                                R removed = vn * vn + (R(1) - sc * sc) * ut2;
+                                R heat = R(0.5) * mass * removed * M.c_damp;
+                                R dT = heat / rmax(mass * M.cp, R(1e-12));
+                                if (M.clamp) dT = rmin(dT, M.dT_max);
+                                tm += dT * mass;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        vec4_t<R> nd;
+        nd.x = vx; nd.y = vy; nd.z = vz;
+        nd.w = live ? (direct_T ? tm / mass : R(0)) : R(__builtin_nan(""));
+        G.node[idx] = nd;
+        G.gm[idx] = mass;
+        G.gtm[idx] = tm;
+    }
+    // fixed-order block sum of the contact force
+    __shared__ double sh[kBlock / kWave];
+    cf = wave_sum(cf);
+    if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = cf;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = sh[0];
+        for (int w = 1; w < kBlock / kWave; ++w) t += sh[w];
+        force_part[blockIdx.x] = t;
+    }
+}
+
+// mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, Jacobi form:
+// reads the post-update field (gm, gtm), writes the gather temperature.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R alpha, R dt) {
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= g.nn) return;
+    const R mass = G.gm[idx];
+    if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
+    R tm = G.gtm[idx];
+    if (mass >= R(1e-15)) {
+        const R T = tm / mass;
+        const R inv_h2 = R(1) / (g.h * g.h);
+        const int i = (int)(idx % g.nx);
+        const int64_t t = idx / g.nx;
+        const int j = (int)(t % g.ny), k = (int)(t / g.ny);
+        const int64_t nxy = (int64_t)g.nx * g.ny;
+        R lap = R(0);
+        if (i > 0) lap += (G.gtm[idx - 1] / rmax(G.gm[idx - 1], R(1e-30)) - T);
+        if (i < g.nx - 1) lap += (G.gtm[idx + 1] / rmax(G.gm[idx + 1], R(1e-30)) - T);
+        if (j > 0) lap += (G.gtm[idx - g.nx] / rmax(G.gm[idx - g.nx], R(1e-30)) - T);
+        if (j < g.ny - 1) lap += (G.gtm[idx + g.nx] / rmax(G.gm[idx + g.nx], R(1e-30)) - T);
+        if (k > 0) lap += (G.gtm[idx - nxy] / rmax(G.gm[idx - nxy], R(1e-30)) - T);
+        if (k < g.nz - 1) lap += (G.gtm[idx + nxy] / rmax(G.gm[idx + nxy], R(1e-30)) - T);
+        const R T_new = T + alpha * dt * lap * inv_h2;
+        tm = rmax(T_new, R(0)) * mass;
+    }
+    G.node[idx].w = tm / mass;
+}
+
+// ------------------------------------------------------------------ G2P ---
+
+struct G2PFlags {
+    int chip;      // run the chip separation / fracture pass (even step)
+    int p2g;       // scatter into the accumulators for the next step
+    int stats;     // write AABB + metrics block partials
+};
+
+// mpmG2PKernel :264-400 fused with chipFractureAndSeparationKernel :667-725
+// and, when flags.p2g, with the next step's mpmP2GKernel :98-134.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
+                                                R zcut, G2PFlags flags, double ref_temp, double *part) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double stats[kRed];
+    if (flags.stats) stats_identity(stats, ref_temp);
+    if (i < P.n) {
+        PState<R> p;
+        p.status = P.status[i];
+        if (p.status != kInactive) {
+            p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+            int bx, by, bz;
+            R fx, fy, fz;
+            base_and_frac(g, p.x, p.y, p.z, bx, by, bz, fx, fy, fz);
+            R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
+            R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                int64_t idx;
+                R w, gx, gy, gz;
+                if (!corner(g, bx, by, bz, fx, fy, fz, n, idx, w, gx, gy, gz)) continue;
+                const vec4_t<R> nd = G.node[idx];
+                if (nd.w != nd.w) continue;  // mass <= 1e-18
+                v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
+                temp += nd.w * w;
+                wsum += w;
+                L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
+                L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
+                L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
+            }
+            const bool has_back = P.a0 != nullptr;
+            p.T = P.T[i];
+            p.eps = P.eps[i];
+            p.dmg = P.dmg[i];
+            p.sres = P.sres[i];
+            p.rate = P.rate[i];
+            p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+            p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+            if (has_back) {
+                p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
+                p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) p.a[c] = R(0);
+            }
+            if (p.status == kFixed) {
+                p.vx = p.vy = p.vz = R(0);
+            } else {
+                p.vx = v0; p.vy = v1; p.vz = v2;
+                p.x += p.vx * dt;
+                p.y += p.vy * dt;
+                p.z += p.vz * dt;
+                if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
+                if (p.status == kActive) constitutive_active(M, p, L, dt, has_back);
+                chip_transition(M, p);
+            }
+            if (flags.chip) chip_pass(M, p, P.id[i], zcut);
+            // write back
+            P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
+            P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
+            P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
+            P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
+            if (has_back) {
+                P.a0[i] = p.a[0]; P.a1[i] = p.a[1]; P.a2[i] = p.a[2];
+                P.a3[i] = p.a[3]; P.a4[i] = p.a[4]; P.a5[i] = p.a[5];
+            }
+            P.eps[i] = p.eps; P.rate[i] = p.rate; P.dmg[i] = p.dmg; P.sres[i] = p.sres; P.T[i] = p.T;
+            P.status[i] = p.status;
+            if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }
+            if (flags.stats) {
+                const double x = p.x, y = p.y, z = p.z;
+                stats[0] = x; stats[1] = y; stats[2] = z;
+                stats[3] = -x; stats[4] = -y; stats[5] = -z;
+                stats[6] = fmax(0.0, (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]));
+                stats[7] = fmax(ref_temp, (double)p.T);
+                const double vx = p.vx, vy = p.vy, vz = p.vz;
+                stats[8] = 0.5 * (double)P.m[i] * (vx * vx + vy * vy + vz * vz);
+            }
+            if (flags.p2g) {
+                // f_ext was zeroed above, so the next step's P2G sees none
+                p2g_particle(g, G.acc, p.x, p.y, p.z, p.vx, p.vy, p.vz, P.m[i], P.V[i], p.T, p.s, R(0), R(0),
+                             R(0), p.status);
+            }
+        }
+    }
+    if (flags.stats) block_reduce_stats(stats, part);
+}
+
+// ------------------------------------------------------------ small ops ---
+
+// applyMPMExternalForceKernel, MPMWorkpieceSolver.cu:402-414
+template <typename R>
+__global__ void k_external_force(Particles<R> P, R cx, R cy, R cz, R r2, R fx, R fy, R fz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n || P.status[i] != kActive) return;
+    R dx = P.x[i] - cx, dy = P.y[i] - cy, dz = P.z[i] - cz;
+    if (dx * dx + dy * dy + dz * dz <= r2) {
+        P.fx[i] += fx;
+        P.fy[i] += fy;
+        P.fz[i] += fz;
+    }
+}
+
+// setParticleTempKernel, MPMWorkpieceSolver.cu:899-904
+template <typename R> __global__ void k_set_temperature(R *T, const double *src, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) T[i] = (R)src[i];
+}
+
+// Nearest particle to a point: per-block (d2, index) with the lowest index
+// winning ties, as the host loops of getTemperatureAt/applyHeatFlux do.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_nearest(Particles<R> P, double qx, double qy, double qz,
+                                                    double *best_d2, int32_t *best_i) {
+    __shared__ double sd[kBlock];
+    __shared__ int32_t si[kBlock];
+    double bd = 1e30;
+    int32_t bi = -1;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += gridDim.x * kBlock) {
+        double dx = (double)P.x[i] - qx, dy = (double)P.y[i] - qy, dz = (double)P.z[i] - qz;
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < bd) { bd = d2; bi = i; }
+    }
+    sd[threadIdx.x] = bd;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            double d = sd[threadIdx.x + o];
+            int32_t k = si[threadIdx.x + o];
+            if (k >= 0 && (d < sd[threadIdx.x] || (d == sd[threadIdx.x] && (si[threadIdx.x] < 0 || k < si[threadIdx.x])))) {
+                sd[threadIdx.x] = d;
+                si[threadIdx.x] = k;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        best_d2[blockIdx.x] = sd[0];
+        best_i[blockIdx.x] = si[0];
+    }
+}
+
+// Combine the per-block nearest candidates (single block) and, for
+// applyHeatFlux (:928-932), heat the winner when within one cell.
+template <typename R>
+__global__ void k_nearest_finalize(Particles<R> P, const double *best_d2, const int32_t *best_i, int nparts,
+                                   double *out_d2, int32_t *out_i, int apply, double q, double cp, double melt,
+                                   double h2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double bd = 1e30;
+    int32_t bi = -1;
+    for (int b = 0; b < nparts; ++b) {
+        double d = best_d2[b];
+        int32_t k = best_i[b];
+        if (k >= 0 && (d < bd || (d == bd && (bi < 0 || k < bi)))) { bd = d; bi = k; }
+    }
+    *out_d2 = bd;
+    *out_i = bi;
+    if (apply && bi >= 0 && bd <= h2) {
+        double T = (double)P.T[bi];
+        double m = (double)P.m[bi];
+        P.T[bi] = (R)fmin(T + q / fmax(m * cp, 1e-12), melt);
+    }
+}
+
+// Gather every particle array through a permutation: dst[i] = src[perm[i]].
+template <typename T> __global__ void k_gather(T *dst, const T *src, const int32_t *perm, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+}  // namespace mpmw

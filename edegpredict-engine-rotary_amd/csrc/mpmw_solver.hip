@@ -1,0 +1,1371 @@
+// mpmw_solver.hip -- host orchestration of the MI355X MPM workpiece step and
+// the C ABI declared in include/mpmw.h.
+//
+// Replaces MPMWorkpieceSolver (src/MPMWorkpieceSolver.cu:418-946 of the
+// reference).  Differences in how, not what:
+//   * particle state lives in HBM as structure-of-arrays in the solver's
+//     precision (FP32 bulk or FP64) and never round-trips to the host during
+//     stepping; the grid rebuild every 25 steps reads back only the AABB
+//     (the reference downloads and re-uploads every particle, :729-733);
+//   * G2P, the chip pass and the next step's P2G run as one particle pass
+//     (k_g2p), so each particle is read and written once per step;
+//   * all reductions (AABB, metrics, contact force) are fixed-order block
+//     partials, so their results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpmw.h"
+#include "mpmw_kernels.hpp"
+#include "mpmw_sort.hpp"
+
+using namespace mpmw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(MPMW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+// Particle fields.  Real-valued fields first, then the two int32 fields.
+enum Field {
+    F_X, F_Y, F_Z, F_VX, F_VY, F_VZ,
+    F_S0, F_S1, F_S2, F_S3, F_S4, F_S5,
+    F_EPS, F_RATE, F_DMG, F_SRES, F_T, F_M, F_V, F_DENS,
+    F_A0, F_A1, F_A2, F_A3, F_A4, F_A5,
+    F_FX, F_FY, F_FZ,
+    F_STATUS, F_ID,
+    NF
+};
+constexpr int kNumReal = F_STATUS;
+
+inline int blocks_for(int64_t n, int b = kBlock) { return (int)std::max<int64_t>(1, (n + b - 1) / b); }
+
+}  // namespace
+
+struct mpmw_solver {
+    mpmw_params prm{};
+    hipStream_t stream = nullptr;
+    bool fp64 = false;
+    size_t rsz = 4;
+
+    // particles
+    int32_t n = 0, cap = 0;
+    void *f[NF] = {};
+    void *alt[NF] = {};    // second buffer set for permutations
+    int32_t alt_cap = 0;
+    bool has_back = false, has_fext = false;
+
+    // grid
+    double minx = 0, miny = 0, minz = 0, h = 0;
+    int32_t nx = 1, ny = 1, nz = 1, kfix = -1;
+    int64_t nn = 0, nn_cap = 0;
+    void *acc = nullptr, *node = nullptr, *gm = nullptr, *gtm = nullptr, *snap = nullptr;
+    bool snap_on = false, snap_valid = false;
+
+    // reductions
+    double *part = nullptr;        // k_g2p / k_reduce partials
+    int64_t part_cap = 0;
+    double *fpart = nullptr;       // contact-force partials
+    int64_t fpart_cap = 0;
+    double *d_stats = nullptr;     // kRed finalized statistics
+    double *d_cf = nullptr;        // contact force of the last step
+    int32_t *d_modes = nullptr;    // finalize modes for kRed columns, then one "sum"
+    double *h_stats = nullptr;     // pinned copy of d_stats
+    double *h_cf = nullptr;        // pinned copy of d_cf
+    double *d_near = nullptr;      // nearest-particle partials
+    int32_t *d_near_i = nullptr;
+    double *d_near_out = nullptr;
+    int32_t *d_near_out_i = nullptr;
+
+    // tool
+    int32_t tool_kind = MPMW_TOOL_NONE;
+    void *samples = nullptr;       // 3 * nsamples R (SoA)
+    int32_t nsamples = 0;
+    double sample_lo[3] = {0, 0, 0}, sample_hi[3] = {0, 0, 0};
+    mpmw_tool_desc desc{};
+    mpmw_tool_pose pose{};
+    double tip[3] = {0, 0, 0};
+
+    // stepping state
+    double time = 0.0;
+    int64_t step = 0;
+    bool p2g_ready = false;        // accumulators hold this step's P2G
+    bool stats_fresh = false;      // d_stats/h_stats hold the current AABB
+    double max_stress = 0, max_temp = 0, ke = 0, contact_force = 0;
+    bool metrics_pending = false, cf_pending = false;
+    double bounds[6] = {0, 0, 0, 0, 0, 0};
+    bool deterministic = false;
+
+    // sorting
+    SortState sort{};
+
+    // timing
+    bool timing = false;
+    struct Ev { hipEvent_t a, b; int cls; };
+    std::vector<Ev> events;
+    std::vector<hipEvent_t> ev_pool;
+    double t_ms[8] = {};
+    int64_t t_n[8] = {};
+
+    int64_t device_bytes = 0;
+};
+
+template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm, int32_t count, bool host_perm);
+
+namespace {
+
+// ------------------------------------------------------------ allocation --
+
+int dalloc(mpmw_solver *s, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return MPMW_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(MPMW_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    s->device_bytes += (int64_t)bytes;
+    return MPMW_OK;
+}
+
+void dfree(mpmw_solver *s, void *&p, size_t bytes) {
+    if (p) {
+        (void)hipFree(p);
+        s->device_bytes -= (int64_t)bytes;
+    }
+    p = nullptr;
+}
+
+size_t field_bytes(const mpmw_solver *s, int fld, int64_t count) {
+    return (size_t)count * (fld >= kNumReal ? sizeof(int32_t) : s->rsz);
+}
+
+bool field_live(const mpmw_solver *s, int fld) {
+    if (fld >= F_A0 && fld <= F_A5) return s->has_back;
+    if (fld >= F_FX && fld <= F_FZ) return s->has_fext;
+    return true;
+}
+
+void free_particles(mpmw_solver *s) {
+    for (int k = 0; k < NF; ++k) {
+        dfree(s, s->f[k], field_bytes(s, k, s->cap));
+        dfree(s, s->alt[k], field_bytes(s, k, s->alt_cap));
+    }
+    s->cap = s->alt_cap = 0;
+}
+
+// (Re)allocate the live particle fields for `count` particles; contents are
+// not preserved.
+int alloc_particles(mpmw_solver *s, int32_t count) {
+    if (count <= s->cap) {
+        for (int k = 0; k < NF; ++k)
+            if (field_live(s, k) && !s->f[k]) {
+                int rc = dalloc(s, &s->f[k], field_bytes(s, k, s->cap));
+                if (rc) return rc;
+            }
+        return MPMW_OK;
+    }
+    free_particles(s);
+    s->cap = count;
+    for (int k = 0; k < NF; ++k) {
+        if (!field_live(s, k)) continue;
+        int rc = dalloc(s, &s->f[k], field_bytes(s, k, count));
+        if (rc) return rc;
+    }
+    return MPMW_OK;
+}
+
+int ensure_field(mpmw_solver *s, int fld) {
+    if (s->f[fld] || s->cap == 0) return MPMW_OK;
+    int rc = dalloc(s, &s->f[fld], field_bytes(s, fld, s->cap));
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(s->f[fld], 0, field_bytes(s, fld, s->cap), s->stream));
+    return MPMW_OK;
+}
+
+int ensure_alt(mpmw_solver *s) {
+    if (s->alt_cap >= s->n && s->alt_cap == s->cap) {
+        for (int k = 0; k < NF; ++k)
+            if (field_live(s, k) && !s->alt[k]) {
+                int rc = dalloc(s, &s->alt[k], field_bytes(s, k, s->alt_cap));
+                if (rc) return rc;
+            }
+        return MPMW_OK;
+    }
+    for (int k = 0; k < NF; ++k) dfree(s, s->alt[k], field_bytes(s, k, s->alt_cap));
+    s->alt_cap = s->cap;
+    for (int k = 0; k < NF; ++k) {
+        if (!field_live(s, k)) continue;
+        int rc = dalloc(s, &s->alt[k], field_bytes(s, k, s->alt_cap));
+        if (rc) return rc;
+    }
+    return MPMW_OK;
+}
+
+template <typename T> int grow(mpmw_solver *s, T *&p, int64_t &cap, int64_t need) {
+    if (need <= cap) return MPMW_OK;
+    void *q = p;
+    dfree(s, q, sizeof(T) * (size_t)cap);
+    p = nullptr;
+    int64_t c = std::max<int64_t>(need, cap * 3 / 2);
+    int rc = dalloc(s, &q, sizeof(T) * (size_t)c);
+    if (rc) return rc;
+    p = (T *)q;
+    cap = c;
+    return MPMW_OK;
+}
+
+// ---------------------------------------------------------------- views ---
+
+template <typename R> Particles<R> pview(mpmw_solver *s) {
+    Particles<R> P;
+    auto F = [&](int k) { return (R *)s->f[k]; };
+    P.x = F(F_X); P.y = F(F_Y); P.z = F(F_Z);
+    P.vx = F(F_VX); P.vy = F(F_VY); P.vz = F(F_VZ);
+    P.s0 = F(F_S0); P.s1 = F(F_S1); P.s2 = F(F_S2); P.s3 = F(F_S3); P.s4 = F(F_S4); P.s5 = F(F_S5);
+    P.a0 = s->has_back ? F(F_A0) : nullptr; P.a1 = F(F_A1); P.a2 = F(F_A2);
+    P.a3 = F(F_A3); P.a4 = F(F_A4); P.a5 = F(F_A5);
+    P.fx = s->has_fext ? F(F_FX) : nullptr; P.fy = F(F_FY); P.fz = F(F_FZ);
+    P.eps = F(F_EPS); P.rate = F(F_RATE); P.dmg = F(F_DMG); P.sres = F(F_SRES);
+    P.T = F(F_T); P.m = F(F_M); P.V = F(F_V);
+    P.status = (int32_t *)s->f[F_STATUS];
+    P.id = (int32_t *)s->f[F_ID];
+    P.n = s->n;
+    return P;
+}
+
+template <typename R> GridView<R> gview(const mpmw_solver *s) {
+    GridView<R> g;
+    g.minx = (R)s->minx; g.miny = (R)s->miny; g.minz = (R)s->minz;
+    g.h = (R)s->h;
+    g.inv_h = (R)(1.0 / s->h);
+    g.nx = s->nx; g.ny = s->ny; g.nz = s->nz;
+    g.kfix = s->kfix;
+    g.nn = s->nn;
+    return g;
+}
+
+template <typename R> Grid<R> grid_arrays(mpmw_solver *s) {
+    Grid<R> G;
+    G.acc = (R *)s->acc;
+    G.node = (vec4_t<R> *)s->node;
+    G.gm = (R *)s->gm;
+    G.gtm = (R *)s->gtm;
+    G.snap = s->snap_on ? (R *)s->snap : nullptr;
+    return G;
+}
+
+template <typename R> Material<R> material(const mpmw_solver *s) {
+    const mpmw_params &p = s->prm;
+    Material<R> M;
+    M.shear = (R)(p.young / (2.0 * (1.0 + p.poisson)));
+    M.bulk = (R)(p.young / (3.0 * (1.0 - 2.0 * p.poisson)));
+    M.density = (R)p.density; M.cp = (R)p.specific_heat;
+    M.melt = (R)p.melt_temp; M.ref_temp = (R)p.ref_temp;
+    M.jcA = (R)p.jc_A; M.jcB = (R)p.jc_B; M.jcN = (R)p.jc_n; M.jcC = (R)p.jc_C; M.jcM = (R)p.jc_m;
+    M.ref_rate = (R)p.ref_strain_rate;
+    M.failure_strain = (R)p.failure_strain; M.dth = (R)p.damage_threshold;
+    M.eps_crit = (R)p.critical_plastic_strain; M.chip_ratio = (R)p.thermal_chip_ratio;
+    M.rc = (R)p.tool_contact_radius; M.c_damp = (R)p.contact_damping;
+    M.dT_max = (R)p.max_contact_temp_rise; M.H = (R)p.kinematic_hardening;
+    M.clamp = p.thermal_clamp_enabled;
+    return M;
+}
+
+template <typename R> Tool<R> tool_view(const mpmw_solver *s) {
+    Tool<R> t{};
+    t.kind = s->tool_kind;
+    if (t.kind == MPMW_TOOL_POINTS) {
+        t.nsamples = s->nsamples;
+        t.sx = (const R *)s->samples;
+        t.sy = t.sx + s->nsamples;
+        t.sz = t.sy + s->nsamples;
+        for (int a = 0; a < 3; ++a) { t.lo[a] = (R)s->sample_lo[a]; t.hi[a] = (R)s->sample_hi[a]; }
+        t.rc2 = (R)(s->prm.tool_contact_radius * s->prm.tool_contact_radius);
+    } else if (t.kind == MPMW_TOOL_WEDGE || t.kind == MPMW_TOOL_DRILL) {
+        const mpmw_tool_desc &d = s->desc;
+        const mpmw_tool_pose &p = s->pose;
+        t.cx = (R)p.center[0]; t.cy = (R)p.center[1]; t.cz = (R)p.center[2];
+        t.cos_a = (R)std::cos(p.angle); t.sin_a = (R)std::sin(p.angle);
+        t.vx = (R)p.vel[0]; t.vy = (R)p.vel[1]; t.vz = (R)p.vel[2]; t.omega = (R)p.omega;
+        t.band = (R)d.band; t.mu = (R)d.friction; t.eps = (R)d.grad_eps;
+        t.sin_rake = (R)std::sin(d.rake_angle); t.cos_rake = (R)std::cos(d.rake_angle);
+        t.sin_clear = (R)std::sin(d.clearance_angle); t.cos_clear = (R)std::cos(d.clearance_angle);
+        t.radius = (R)d.radius; t.tan_helix = (R)std::tan(d.helix_angle);
+        t.sin_half_point = (R)std::sin(0.5 * d.point_angle); t.cos_half_point = (R)std::cos(0.5 * d.point_angle);
+        t.flute_offset = (R)d.flute_offset; t.flute_radius = (R)d.flute_radius; t.flute_length = (R)d.flute_length;
+    }
+    return t;
+}
+
+// --------------------------------------------------------------- timing ---
+
+hipEvent_t take_event(mpmw_solver *s) {
+    if (!s->ev_pool.empty()) {
+        hipEvent_t e = s->ev_pool.back();
+        s->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Timed {
+    mpmw_solver *s;
+    mpmw_solver::Ev ev{};
+    bool on;
+    Timed(mpmw_solver *s_, int cls) : s(s_), on(s_->timing) {
+        if (!on) return;
+        ev.a = take_event(s);
+        ev.b = take_event(s);
+        ev.cls = cls;
+        if (!ev.a || !ev.b) { on = false; return; }
+        (void)hipEventRecord(ev.a, s->stream);
+    }
+    ~Timed() {
+        if (!on) return;
+        (void)hipEventRecord(ev.b, s->stream);
+        s->events.push_back(ev);
+    }
+};
+
+void drain_events(mpmw_solver *s) {
+    if (s->events.empty()) return;
+    (void)hipStreamSynchronize(s->stream);
+    for (auto &e : s->events) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            s->t_ms[e.cls] += ms;
+            s->t_n[e.cls] += 1;
+        }
+        s->ev_pool.push_back(e.a);
+        s->ev_pool.push_back(e.b);
+    }
+    s->events.clear();
+}
+
+// ----------------------------------------------------------- reductions ---
+
+int ensure_part(mpmw_solver *s, int64_t nblocks) { return grow(s, s->part, s->part_cap, nblocks * kRed); }
+
+// Finalize kRed statistics from `nparts` partials into d_stats and start the
+// copy to the pinned host buffer.
+int finalize_stats(mpmw_solver *s, int nparts) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, s->part, nparts, kRed, s->d_modes,
+                       s->d_stats);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->h_stats, s->d_stats, sizeof(double) * kRed, hipMemcpyDeviceToHost, s->stream));
+    return MPMW_OK;
+}
+
+template <typename R> int reduce_particles(mpmw_solver *s) {
+    int nb = std::min(blocks_for(s->n), 2048);
+    int rc = ensure_part(s, nb);
+    if (rc) return rc;
+    {
+        Timed t(s, MPMW_K_REDUCE);
+        hipLaunchKernelGGL(k_reduce<R>, dim3(nb), dim3(kBlock), 0, s->stream, pview<R>(s), s->prm.ref_temp,
+                           s->part);
+        HIP_TRY(hipGetLastError());
+        rc = finalize_stats(s, nb);
+    }
+    return rc;
+}
+
+// Copy finalized statistics (already synchronised) into the cached metrics.
+void absorb_stats(mpmw_solver *s) {
+    const double *v = s->h_stats;
+    s->bounds[0] = v[0]; s->bounds[1] = v[1]; s->bounds[2] = v[2];
+    s->bounds[3] = -v[3]; s->bounds[4] = -v[4]; s->bounds[5] = -v[5];
+    s->max_stress = v[6];
+    s->max_temp = v[7];
+    s->ke = v[8];
+}
+
+// ------------------------------------------------------------ the grid ----
+
+// rebuildGridFromHostBounds, MPMWorkpieceSolver.cu:570-586, from the AABB in
+// h_stats (min xyz, -max xyz).
+int rebuild_grid_from_stats(mpmw_solver *s) {
+    const double *v = s->h_stats;
+    double mn[3] = {v[0], v[1], v[2]}, mx[3] = {-v[3], -v[4], -v[5]};
+    if (!(mn[0] <= mx[0])) {  // no non-INACTIVE particle: reference leaves +-1e30 bounds
+        mn[0] = mn[1] = mn[2] = 1e30;
+        mx[0] = mx[1] = mx[2] = -1e30;
+    }
+    const double cell = s->h;
+    const double pad = cell * 4.0;
+    s->minx = mn[0] - pad;
+    s->miny = mn[1] - pad;
+    s->minz = mn[2] - pad;
+    // max(4, (int)ceil(extent / h) + 1), with the int conversion kept in range
+    auto dim = [&](double lo, double hi, int32_t &out) {
+        const double c = std::ceil((hi - lo + 2 * pad) / cell) + 1.0;
+        if (!(c < 1e9)) return false;
+        out = c < 4.0 ? 4 : (int32_t)c;
+        return true;
+    };
+    if (!dim(mn[0], mx[0], s->nx) || !dim(mn[1], mx[1], s->ny) || !dim(mn[2], mx[2], s->nz))
+        return fail(MPMW_E_RANGE, "particle bounds give a grid beyond 1e9 nodes per axis");
+    const int64_t nn = (int64_t)s->nx * s->ny * s->nz;
+    if (nn > (int64_t)1 << 33) return fail(MPMW_E_RANGE, "grid of " + std::to_string(nn) + " nodes is too large");
+    // fixture: nodes with pos.z <= minZ + fixture + h are clamped (:153)
+    const double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.002;
+    s->kfix = -1;
+    for (int k = 0; k < s->nz; ++k) {
+        double pz = s->minz + k * cell;
+        if (pz <= s->minz + fixture + cell) s->kfix = k;
+        else break;
+    }
+    if (nn > s->nn_cap) {
+        dfree(s, s->acc, 8 * s->rsz * (size_t)s->nn_cap);
+        dfree(s, s->node, 4 * s->rsz * (size_t)s->nn_cap);
+        dfree(s, s->gm, s->rsz * (size_t)s->nn_cap);
+        dfree(s, s->gtm, s->rsz * (size_t)s->nn_cap);
+        dfree(s, s->snap, 8 * s->rsz * (size_t)s->nn_cap);
+        int64_t c = std::max<int64_t>(nn, s->nn_cap * 5 / 4);
+        int rc;
+        if ((rc = dalloc(s, &s->acc, 8 * s->rsz * (size_t)c))) return rc;
+        if ((rc = dalloc(s, &s->node, 4 * s->rsz * (size_t)c))) return rc;
+        if ((rc = dalloc(s, &s->gm, s->rsz * (size_t)c))) return rc;
+        if ((rc = dalloc(s, &s->gtm, s->rsz * (size_t)c))) return rc;
+        if (s->snap_on && (rc = dalloc(s, &s->snap, 8 * s->rsz * (size_t)c))) return rc;
+        s->nn_cap = c;
+        s->snap_valid = false;
+    }
+    s->nn = nn;
+    int rc = grow(s, s->fpart, s->fpart_cap, blocks_for(nn));
+    if (rc) return rc;
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+template <typename R> int zero_acc(mpmw_solver *s) {
+    hipLaunchKernelGGL(k_fill<R>, dim3(std::min(blocks_for(8 * s->nn), 4096)), dim3(kBlock), 0, s->stream,
+                       (R *)s->acc, 8 * s->nn, R(0));
+    HIP_TRY(hipGetLastError());
+    return MPMW_OK;
+}
+
+template <typename R> int sort_particles(mpmw_solver *) { return MPMW_OK; }
+void free_sort_state(mpmw_solver *) {}
+
+// Recompute the AABB on the device, read it back and rebuild the grid.
+template <typename R> int rebuild(mpmw_solver *s) {
+    int rc;
+    if (!s->stats_fresh) {
+        if ((rc = reduce_particles<R>(s))) return rc;
+        s->stats_fresh = true;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if ((rc = rebuild_grid_from_stats(s))) return rc;
+    if ((rc = sort_particles<R>(s))) return rc;
+    return MPMW_OK;
+}
+
+template <typename R> int compact_impl(mpmw_solver *s);
+
+// ------------------------------------------------------------- the step ---
+
+template <typename R> int step_impl(mpmw_solver *s, double dt) {
+    if (s->n <= 0 || s->nn <= 0) return MPMW_OK;  // :728 silent return
+    int rc;
+    if (s->step % 25 == 0) {
+        if ((rc = rebuild<R>(s))) return rc;
+    }
+    const GridView<R> g = gview<R>(s);
+    const Grid<R> G = grid_arrays<R>(s);
+    Particles<R> P = pview<R>(s);
+    const Material<R> M = material<R>(s);
+    if (!s->p2g_ready) {
+        if ((rc = zero_acc<R>(s))) return rc;
+        Timed t(s, MPMW_K_P2G);
+        hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->stream, P, g, G.acc);
+        HIP_TRY(hipGetLastError());
+    }
+    const bool diffusion = s->prm.specific_heat > 0.0 &&
+                           s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
+    {
+        Timed t(s, MPMW_K_GRID);
+        hipLaunchKernelGGL(k_grid_update<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, M,
+                           tool_view<R>(s), (R)dt, diffusion ? 0 : 1, s->fpart);
+        HIP_TRY(hipGetLastError());
+    }
+    {
+        Timed t(s, MPMW_K_REDUCE);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, s->fpart, blocks_for(s->nn), 1,
+                           s->d_modes + kRed, s->d_cf);
+        HIP_TRY(hipGetLastError());
+    }
+    s->cf_pending = true;
+    s->snap_valid = s->snap_on;
+    if (diffusion) {
+        const double alpha = s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat);
+        Timed t(s, MPMW_K_DIFFUSE);
+        hipLaunchKernelGGL(k_diffuse<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, (R)alpha,
+                           (R)dt);
+        HIP_TRY(hipGetLastError());
+    }
+    const bool next_rebuild = ((s->step + 1) % 25 == 0);
+    G2PFlags fl;
+    fl.chip = (s->step % 2 == 0);
+    fl.p2g = next_rebuild ? 0 : 1;
+    fl.stats = next_rebuild ? 1 : 0;
+    const double zcut = s->tip[2] - s->h * 1.5;
+    const int nb = blocks_for(s->n);
+    if (fl.stats && (rc = ensure_part(s, nb))) return rc;
+    {
+        Timed t(s, MPMW_K_G2P_P2G);
+        hipLaunchKernelGGL(k_g2p<R>, dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, (R)zcut, fl,
+                           s->prm.ref_temp, s->part);
+        HIP_TRY(hipGetLastError());
+    }
+    if (fl.stats) {
+        Timed t(s, MPMW_K_REDUCE);
+        if ((rc = finalize_stats(s, nb))) return rc;
+        s->stats_fresh = true;
+        s->metrics_pending = true;
+    } else {
+        s->stats_fresh = false;
+    }
+    s->p2g_ready = fl.p2g != 0;
+    // kinematic tool
+    for (int a = 0; a < 3; ++a) s->pose.center[a] += s->pose.vel[a] * dt;
+    s->pose.angle += s->pose.omega * dt;
+    s->time += dt;
+    ++s->step;
+    if (s->step % 1000 == 0) return compact_impl<R>(s);  // :788-790
+    return MPMW_OK;
+}
+
+// Order-preserving removal of INACTIVE particles (compactParticles :475-527).
+template <typename R> int compact_impl(mpmw_solver *s) {
+    if (s->n <= 0) return MPMW_OK;
+    std::vector<int32_t> st(s->n);
+    HIP_TRY(hipMemcpyAsync(st.data(), s->f[F_STATUS], sizeof(int32_t) * s->n, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::vector<int32_t> keep;
+    keep.reserve(s->n);
+    for (int32_t i = 0; i < s->n; ++i)
+        if (st[i] != MPMW_INACTIVE) keep.push_back(i);
+    if (keep.empty() || (int32_t)keep.size() == s->n) return MPMW_OK;
+    return permute_particles<R>(s, keep.data(), (int32_t)keep.size(), true);
+}
+
+}  // namespace
+
+// Permute all live particle fields: new[i] = old[perm[i]] for i < count.
+// perm is a host array when host_perm, else a device array.
+template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm, int32_t count, bool host_perm) {
+    int rc = ensure_alt(s);
+    if (rc) return rc;
+    const int32_t *dperm = perm;
+    void *tmp = nullptr;
+    if (host_perm) {
+        if ((rc = dalloc(s, &tmp, sizeof(int32_t) * (size_t)count))) return rc;
+        HIP_TRY(hipMemcpyAsync(tmp, perm, sizeof(int32_t) * count, hipMemcpyHostToDevice, s->stream));
+        dperm = (const int32_t *)tmp;
+    }
+    {
+        Timed t(s, MPMW_K_SORT);
+        for (int k = 0; k < NF; ++k) {
+            if (!field_live(s, k)) continue;
+            if (k >= kNumReal)
+                hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
+                                   (int32_t *)s->alt[k], (const int32_t *)s->f[k], dperm, count);
+            else
+                hipLaunchKernelGGL(k_gather<R>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream, (R *)s->alt[k],
+                                   (const R *)s->f[k], dperm, count);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (host_perm) {
+        (void)hipStreamSynchronize(s->stream);
+        dfree(s, tmp, sizeof(int32_t) * (size_t)count);
+    }
+    if (e != hipSuccess) return fail(MPMW_E_HIP, std::string("permute: ") + hipGetErrorString(e));
+    for (int k = 0; k < NF; ++k) std::swap(s->f[k], s->alt[k]);
+    s->n = count;
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+// ================================================================ C ABI ====
+
+namespace {
+
+int check(mpmw_solver *s) {
+    if (!s) return fail(MPMW_E_INVAL, "null solver handle");
+    hipError_t e = hipSetDevice(s->prm.device);
+    if (e != hipSuccess) return fail(MPMW_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return MPMW_OK;
+}
+
+#define CHECK_S(s)              \
+    do {                        \
+        int rc_ = check(s);     \
+        if (rc_) return rc_;    \
+    } while (0)
+
+#define DISPATCH(s, fn, ...) ((s)->fp64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+bool finite_array(const double *a, int64_t n) {
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+template <typename R> int upload_field(mpmw_solver *s, int fld, const double *src, int64_t n, int comp, int c,
+                                       double fill) {
+    std::vector<R> tmp((size_t)n);
+    for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (R)(src ? src[i * comp + c] : fill);
+    HIP_TRY(hipMemcpy(s->f[fld], tmp.data(), sizeof(R) * (size_t)n, hipMemcpyHostToDevice));
+    return MPMW_OK;
+}
+
+template <typename R> int download_field(mpmw_solver *s, int fld, double *dst, int64_t n, int comp, int c) {
+    std::vector<R> tmp((size_t)n);
+    HIP_TRY(hipMemcpy(tmp.data(), s->f[fld], sizeof(R) * (size_t)n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) dst[i * comp + c] = (double)tmp[(size_t)i];
+    return MPMW_OK;
+}
+
+template <typename R> int set_particles_impl(mpmw_solver *s, const mpmw_particle_arrays *a) {
+    const int32_t n = a->n;
+    bool back = s->prm.kinematic_hardening > 0.0;
+    if (a->backstress)
+        for (int64_t i = 0; i < 6 * (int64_t)n && !back; ++i) back = a->backstress[i] != 0.0;
+    bool fext = false;
+    if (a->fext)
+        for (int64_t i = 0; i < 3 * (int64_t)n && !fext; ++i) fext = a->fext[i] != 0.0;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->metrics_pending) {
+        absorb_stats(s);
+        s->metrics_pending = false;
+    }
+    s->has_back = back;
+    s->has_fext = fext;
+    s->n = 0;
+    int rc = alloc_particles(s, n);
+    if (rc) return rc;
+    s->n = n;
+    if (n == 0) return MPMW_OK;
+    struct { int fld; const double *src; int comp, c; double fill; } map[] = {
+        {F_X, a->pos, 3, 0, 0}, {F_Y, a->pos, 3, 1, 0}, {F_Z, a->pos, 3, 2, 0},
+        {F_VX, a->vel, 3, 0, 0}, {F_VY, a->vel, 3, 1, 0}, {F_VZ, a->vel, 3, 2, 0},
+        {F_S0, a->stress, 6, 0, 0}, {F_S1, a->stress, 6, 1, 0}, {F_S2, a->stress, 6, 2, 0},
+        {F_S3, a->stress, 6, 3, 0}, {F_S4, a->stress, 6, 4, 0}, {F_S5, a->stress, 6, 5, 0},
+        {F_EPS, a->plastic_strain, 1, 0, 0}, {F_RATE, a->strain_rate, 1, 0, 0},
+        {F_DMG, a->damage, 1, 0, 0}, {F_SRES, a->residual_stress, 1, 0, 0},
+        {F_T, a->temperature, 1, 0, 0}, {F_M, a->mass, 1, 0, 0}, {F_V, a->volume, 1, 0, 0},
+        {F_DENS, a->density, 1, 0, s->prm.density},
+        {F_A0, a->backstress, 6, 0, 0}, {F_A1, a->backstress, 6, 1, 0}, {F_A2, a->backstress, 6, 2, 0},
+        {F_A3, a->backstress, 6, 3, 0}, {F_A4, a->backstress, 6, 4, 0}, {F_A5, a->backstress, 6, 5, 0},
+        {F_FX, a->fext, 3, 0, 0}, {F_FY, a->fext, 3, 1, 0}, {F_FZ, a->fext, 3, 2, 0},
+    };
+    for (auto &e : map) {
+        if (!field_live(s, e.fld)) continue;
+        if ((rc = upload_field<R>(s, e.fld, e.src, n, e.comp, e.c, e.fill))) return rc;
+    }
+    HIP_TRY(hipMemcpy(s->f[F_STATUS], a->status, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->f[F_ID], a->id, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    s->p2g_ready = false;
+    s->stats_fresh = false;
+    s->sort.valid = false;
+    // as initializeParticleBox does, size the grid for the new particles now
+    rc = reduce_particles<R>(s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->stats_fresh = true;
+    {
+        const double *v = s->h_stats;
+        s->bounds[0] = v[0]; s->bounds[1] = v[1]; s->bounds[2] = v[2];
+        s->bounds[3] = -v[3]; s->bounds[4] = -v[4]; s->bounds[5] = -v[5];
+    }
+    return rebuild_grid_from_stats(s);
+}
+
+template <typename R> int get_particles_impl(mpmw_solver *s, mpmw_particle_arrays *a) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int32_t n = std::min(a->n, s->n);
+    int rc;
+    struct { int fld; double *dst; int comp, c; } map[] = {
+        {F_X, a->pos, 3, 0}, {F_Y, a->pos, 3, 1}, {F_Z, a->pos, 3, 2},
+        {F_VX, a->vel, 3, 0}, {F_VY, a->vel, 3, 1}, {F_VZ, a->vel, 3, 2},
+        {F_S0, a->stress, 6, 0}, {F_S1, a->stress, 6, 1}, {F_S2, a->stress, 6, 2},
+        {F_S3, a->stress, 6, 3}, {F_S4, a->stress, 6, 4}, {F_S5, a->stress, 6, 5},
+        {F_EPS, a->plastic_strain, 1, 0}, {F_RATE, a->strain_rate, 1, 0},
+        {F_DMG, a->damage, 1, 0}, {F_SRES, a->residual_stress, 1, 0},
+        {F_T, a->temperature, 1, 0}, {F_M, a->mass, 1, 0}, {F_V, a->volume, 1, 0},
+        {F_DENS, a->density, 1, 0},
+        {F_A0, a->backstress, 6, 0}, {F_A1, a->backstress, 6, 1}, {F_A2, a->backstress, 6, 2},
+        {F_A3, a->backstress, 6, 3}, {F_A4, a->backstress, 6, 4}, {F_A5, a->backstress, 6, 5},
+        {F_FX, a->fext, 3, 0}, {F_FY, a->fext, 3, 1}, {F_FZ, a->fext, 3, 2},
+    };
+    for (auto &e : map) {
+        if (!e.dst || n == 0) continue;
+        if (!field_live(s, e.fld)) {
+            for (int64_t i = 0; i < n; ++i) e.dst[i * e.comp + e.c] = 0.0;
+            continue;
+        }
+        if ((rc = download_field<R>(s, e.fld, e.dst, n, e.comp, e.c))) return rc;
+    }
+    if (n > 0 && a->status) HIP_TRY(hipMemcpy(a->status, s->f[F_STATUS], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (n > 0 && a->id) HIP_TRY(hipMemcpy(a->id, s->f[F_ID], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    a->n = n;
+    return MPMW_OK;
+}
+
+// initializeParticleBox / initializeCylindricalWorkpiece lattice, host side.
+struct Lattice {
+    std::vector<double> pos, mass, vol, dens, T;
+    std::vector<int32_t> id, status;
+    void push(const mpmw_params &p, double x, double y, double z, double v, int32_t pid, bool fixed) {
+        pos.push_back(x); pos.push_back(y); pos.push_back(z);
+        mass.push_back(p.density * v);
+        vol.push_back(v);
+        dens.push_back(p.density);
+        T.push_back(p.ref_temp);
+        id.push_back(pid);
+        status.push_back(fixed ? MPMW_FIXED_BOUNDARY : MPMW_ACTIVE);
+    }
+    int upload(mpmw_solver *s) {
+        const size_t n = mass.size();
+        std::vector<double> zero3(3 * n, 0.0), zero1(n, 0.0), zero6(6 * n, 0.0);
+        mpmw_particle_arrays a{};
+        a.n = (int32_t)n;
+        a.pos = pos.data(); a.vel = zero3.data(); a.fext = nullptr;
+        a.mass = mass.data(); a.volume = vol.data(); a.density = dens.data(); a.temperature = T.data();
+        a.plastic_strain = zero1.data(); a.strain_rate = zero1.data(); a.damage = zero1.data();
+        a.residual_stress = zero1.data(); a.stress = zero6.data(); a.backstress = nullptr;
+        a.id = id.data(); a.status = status.data();
+        return mpmw_set_particles(s, &a);
+    }
+};
+
+uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+namespace {
+template <typename R> int nearest_impl(mpmw_solver *s, double x, double y, double z, int apply, double q) {
+    const int nb = std::min(blocks_for(s->n), 4096);
+    hipLaunchKernelGGL(k_nearest<R>, dim3(nb), dim3(kBlock), 0, s->stream, pview<R>(s), x, y, z, s->d_near,
+                       s->d_near_i);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_nearest_finalize<R>, dim3(1), dim3(1), 0, s->stream, pview<R>(s), s->d_near, s->d_near_i,
+                       nb, s->d_near_out, s->d_near_out_i, apply, q, s->prm.specific_heat, s->prm.melt_temp,
+                       s->h * s->h);
+    HIP_TRY(hipGetLastError());
+    return MPMW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char *mpmw_last_error(void) { return g_err.c_str(); }
+int mpmw_abi_version(void) { return MPMW_ABI_VERSION; }
+
+void mpmw_params_default(mpmw_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    // Config.h defaults (MaterialProperties :23-45, SimulationParams :74-80,
+    // MPMParams :128-157, PhysicsSafety :343-346, MachineSetupParams :366-391),
+    // mapped as MPMWorkpieceSolver::initialize does (:428-451); k, c_damp and H
+    // keep MPMMaterialParams' own defaults (MPMWorkpieceSolver.cuh:85-88).
+    p->density = 4430.0;
+    p->young = 113.8e9;
+    p->poisson = 0.34;
+    p->specific_heat = 526.3;
+    p->melt_temp = 1660.0;
+    p->ref_temp = 25.0;
+    p->jc_A = 880e6; p->jc_B = 290e6; p->jc_n = 0.47; p->jc_C = 0.015; p->jc_m = 1.0;
+    p->ref_strain_rate = 1.0;
+    p->failure_strain = 0.3;
+    p->damage_threshold = 1.0;
+    p->critical_plastic_strain = 0.15;
+    p->thermal_chip_ratio = 0.25;
+    p->cell_size = 0.0001;
+    p->tool_contact_radius = 1.5 * p->cell_size;
+    p->contact_damping = 0.15;
+    p->max_contact_temp_rise = 50.0;
+    p->thermal_conductivity = 42.6;
+    p->kinematic_hardening = 0.0;
+    p->thermal_clamp_enabled = 1;
+    p->has_config = 1;
+    p->particle_spacing_factor = 0.8;
+    p->fixture_thickness = 0.002;
+    p->min_time_step = 1e-12;
+    p->max_time_step = 1e-4;
+    p->max_particles = 100000;
+    p->precision = MPMW_FP32;
+    p->deterministic = 0;
+    p->device = 0;
+}
+
+int mpmw_create(const mpmw_params *p, mpmw_solver **out) {
+    if (!p || !out) return fail(MPMW_E_INVAL, "mpmw_create: null argument");
+    *out = nullptr;
+    if (!(p->cell_size > 0.0) || !std::isfinite(p->cell_size))
+        return fail(MPMW_E_INVAL, "cell_size must be > 0 (smoothing_radius_m)");
+    if (p->precision != MPMW_FP32 && p->precision != MPMW_FP64)
+        return fail(MPMW_E_INVAL, "precision must be MPMW_FP32 or MPMW_FP64");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(MPMW_E_HIP, std::string("no HIP device available: ") + hipGetErrorString(e));
+    if (p->device < 0 || p->device >= ndev) return fail(MPMW_E_INVAL, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(p->device));
+    auto *s = new mpmw_solver();
+    s->prm = *p;
+    s->fp64 = p->precision == MPMW_FP64;
+    s->rsz = s->fp64 ? sizeof(double) : sizeof(float);
+    s->h = p->cell_size;
+    s->deterministic = p->deterministic != 0;
+    s->max_temp = p->ref_temp;
+    int rc = MPMW_OK;
+    auto bail = [&](int code) { mpmw_destroy(s); return code; };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(MPMW_E_HIP, "hipStreamCreate failed"));
+    void *q;
+    if ((rc = dalloc(s, &q, sizeof(double) * kRed))) return bail(rc);
+    s->d_stats = (double *)q;
+    if ((rc = dalloc(s, &q, sizeof(double)))) return bail(rc);
+    s->d_cf = (double *)q;
+    if ((rc = dalloc(s, &q, sizeof(int32_t) * (kRed + 1)))) return bail(rc);
+    s->d_modes = (int32_t *)q;
+    int32_t modes[kRed + 1] = {0, 0, 0, 0, 0, 0, 1, 1, 2, 2};
+    if (hipMemcpy(s->d_modes, modes, sizeof(modes), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(MPMW_E_HIP, "hipMemcpy modes failed"));
+    if (hipHostMalloc((void **)&s->h_stats, sizeof(double) * kRed, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_cf, sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(MPMW_E_HIP, "hipHostMalloc failed"));
+    std::memset(s->h_stats, 0, sizeof(double) * kRed);
+    *s->h_cf = 0.0;
+    if ((rc = dalloc(s, &q, sizeof(double) * 4096))) return bail(rc);
+    s->d_near = (double *)q;
+    if ((rc = dalloc(s, &q, sizeof(int32_t) * 4096))) return bail(rc);
+    s->d_near_i = (int32_t *)q;
+    if ((rc = dalloc(s, &q, sizeof(double)))) return bail(rc);
+    s->d_near_out = (double *)q;
+    if ((rc = dalloc(s, &q, sizeof(int32_t)))) return bail(rc);
+    s->d_near_out_i = (int32_t *)q;
+    *out = s;
+    return MPMW_OK;
+}
+
+int mpmw_destroy(mpmw_solver *s) {
+    if (!s) return MPMW_OK;
+    (void)hipSetDevice(s->prm.device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &e : s->events) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+    free_particles(s);
+    dfree(s, s->acc, 0); dfree(s, s->node, 0); dfree(s, s->gm, 0); dfree(s, s->gtm, 0); dfree(s, s->snap, 0);
+    void *q;
+    q = s->part; dfree(s, q, 0);
+    q = s->fpart; dfree(s, q, 0);
+    q = s->d_stats; dfree(s, q, 0);
+    q = s->d_cf; dfree(s, q, 0);
+    q = s->d_modes; dfree(s, q, 0);
+    q = s->d_near; dfree(s, q, 0);
+    q = s->d_near_i; dfree(s, q, 0);
+    q = s->d_near_out; dfree(s, q, 0);
+    q = s->d_near_out_i; dfree(s, q, 0);
+    dfree(s, s->samples, 0);
+    free_sort_state(s);
+    if (s->h_stats) (void)hipHostFree(s->h_stats);
+    if (s->h_cf) (void)hipHostFree(s->h_cf);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return MPMW_OK;
+}
+
+int mpmw_reset(mpmw_solver *s) {
+    CHECK_S(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    // reset(), :830-841: host state only; grid parameters are kept
+    s->n = 0;
+    s->tool_kind = MPMW_TOOL_NONE;
+    s->nsamples = 0;
+    s->time = 0.0;
+    s->step = 0;
+    s->max_stress = 0.0;
+    s->max_temp = s->prm.ref_temp;
+    s->ke = 0.0;
+    s->tip[0] = s->tip[1] = s->tip[2] = 0.0;
+    s->p2g_ready = false;
+    s->stats_fresh = false;
+    s->metrics_pending = false;
+    s->sort.valid = false;
+    return MPMW_OK;
+}
+
+int mpmw_set_particles(mpmw_solver *s, const mpmw_particle_arrays *a) {
+    CHECK_S(s);
+    if (!a || a->n < 0) return fail(MPMW_E_INVAL, "set_particles: bad arrays");
+    const int64_t n = a->n;
+    if (n > 0 && (!a->pos || !a->vel || !a->mass || !a->volume || !a->temperature || !a->plastic_strain ||
+                  !a->strain_rate || !a->damage || !a->residual_stress || !a->stress || !a->id || !a->status))
+        return fail(MPMW_E_INVAL, "set_particles: a required array is NULL");
+    struct { const double *p; int64_t k; const char *name; } chk[] = {
+        {a->pos, 3 * n, "pos"}, {a->vel, 3 * n, "vel"}, {a->mass, n, "mass"}, {a->volume, n, "volume"},
+        {a->temperature, n, "temperature"}, {a->plastic_strain, n, "plastic_strain"},
+        {a->strain_rate, n, "strain_rate"}, {a->damage, n, "damage"},
+        {a->residual_stress, n, "residual_stress"}, {a->stress, 6 * n, "stress"},
+        {a->fext, 3 * n, "fext"}, {a->backstress, 6 * n, "backstress"}, {a->density, n, "density"}};
+    for (auto &c : chk)
+        if (c.p && !finite_array(c.p, c.k)) return fail(MPMW_E_INVAL, std::string("set_particles: non-finite ") + c.name);
+    for (int64_t i = 0; i < n; ++i)
+        if (a->status[i] < MPMW_ACTIVE || a->status[i] > MPMW_FIXED_BOUNDARY)
+            return fail(MPMW_E_INVAL, "set_particles: status out of range");
+    return DISPATCH(s, set_particles_impl, s, a);
+}
+
+int mpmw_get_particles(mpmw_solver *s, mpmw_particle_arrays *a) {
+    CHECK_S(s);
+    if (!a || a->n < 0) return fail(MPMW_E_INVAL, "get_particles: bad arrays");
+    return DISPATCH(s, get_particles_impl, s, a);
+}
+
+int mpmw_particle_count(mpmw_solver *s, int32_t *n) {
+    CHECK_S(s);
+    if (!n) return fail(MPMW_E_INVAL, "null");
+    *n = s->n;
+    return MPMW_OK;
+}
+
+int mpmw_init_particle_box(mpmw_solver *s, const double mn[3], const double mx[3], double spacing) {
+    CHECK_S(s);
+    if (!mn || !mx || !(spacing > 0.0)) return fail(MPMW_E_INVAL, "init_particle_box: bad arguments");
+    const mpmw_params &p = s->prm;
+    const int maxp = p.has_config ? p.max_particles : 100000;
+    const double vol = spacing * spacing * spacing;
+    Lattice L;
+    int32_t id = 0;
+    for (double z = mx[2]; z >= mn[2] - 1e-12 && (int)L.mass.size() < maxp; z -= spacing)
+        for (double y = mn[1]; y <= mx[1] + 1e-12 && (int)L.mass.size() < maxp; y += spacing)
+            for (double x = mn[0]; x <= mx[0] + 1e-12 && (int)L.mass.size() < maxp; x += spacing)
+                L.push(p, x, y, z, vol, id++, p.has_config && z <= mn[2] + p.fixture_thickness);
+    return L.upload(s);
+}
+
+int mpmw_init_cylinder(mpmw_solver *s, const double c[3], double radius, double length, double spacing,
+                       int32_t axis) {
+    CHECK_S(s);
+    (void)axis;  // the reference ignores it too (:618)
+    if (!c || !(spacing > 0.0)) return fail(MPMW_E_INVAL, "init_cylinder: bad arguments");
+    const mpmw_params &p = s->prm;
+    const int maxp = p.has_config ? p.max_particles : 100000;
+    const double vol = spacing * spacing * spacing;
+    const double minZ = c[2] - length * 0.5, maxZ = c[2] + length * 0.5;
+    Lattice L;
+    int32_t id = 0;
+    for (double z = maxZ; z >= minZ - 1e-12 && (int)L.mass.size() < maxp; z -= spacing)
+        for (double y = c[1] - radius; y <= c[1] + radius + 1e-12 && (int)L.mass.size() < maxp; y += spacing)
+            for (double x = c[0] - radius; x <= c[0] + radius + 1e-12 && (int)L.mass.size() < maxp; x += spacing) {
+                double dx = x - c[0], dy = y - c[1];
+                if (dx * dx + dy * dy > radius * radius) continue;
+                L.push(p, x, y, z, vol, id++, p.has_config && z <= minZ + p.fixture_thickness);
+            }
+    return L.upload(s);
+}
+
+int mpmw_gen_jittered_slab(uint64_t seed, int32_t cx, int32_t cy, int32_t cz, double h, const double origin[3],
+                           double density, double fixture, double *pos, double *mass, double *volume, int32_t *id,
+                           int32_t *status) {
+    if (cx <= 0 || cy <= 0 || cz <= 0 || !(h > 0) || !origin || !pos || !mass || !volume || !id || !status)
+        return fail(MPMW_E_INVAL, "gen_jittered_slab: bad arguments");
+    const int64_t ncell = (int64_t)cx * cy * cz;
+    if (ncell * 8 > INT32_MAX) return fail(MPMW_E_RANGE, "gen_jittered_slab: too many particles");
+    const double vol = h * h * h / 8.0;
+    for (int64_t c = 0; c < ncell; ++c) {
+        const int64_t cell[3] = {c % cx, (c / cx) % cy, c / ((int64_t)cx * cy)};
+        for (int k = 0; k < 8; ++k) {
+            const int64_t pi = c * 8 + k;
+            const int sub[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
+            for (int ax = 0; ax < 3; ++ax) {
+                const uint64_t ctr = (uint64_t)(pi * 3 + ax);
+                const double u = (double)(splitmix64(seed ^ (ctr * 0xD1B54A32D192ED03ull)) >> 11) * 0x1.0p-53;
+                const double x = origin[ax] + ((double)cell[ax] + 0.5 * sub[ax] + 0.5 * u) * h;
+                pos[3 * pi + ax] = (double)(float)x;
+            }
+            mass[pi] = density * vol;
+            volume[pi] = vol;
+            id[pi] = (int32_t)pi;
+            status[pi] = (pos[3 * pi + 2] <= origin[2] + fixture) ? MPMW_FIXED_BOUNDARY : MPMW_ACTIVE;
+        }
+    }
+    return MPMW_OK;
+}
+
+int mpmw_init_jittered_slab(mpmw_solver *s, uint64_t seed, int32_t cx, int32_t cy, int32_t cz,
+                            const double origin[3], const double vel[3], double temperature) {
+    CHECK_S(s);
+    if (!origin || !vel) return fail(MPMW_E_INVAL, "init_jittered_slab: null argument");
+    const int64_t n = (int64_t)cx * cy * cz * 8;
+    if (cx <= 0 || cy <= 0 || cz <= 0 || n > INT32_MAX) return fail(MPMW_E_RANGE, "init_jittered_slab: bad size");
+    std::vector<double> pos(3 * n), mass(n), vol(n), velv(3 * n), T(n, temperature), z1(n, 0.0), z6(6 * n, 0.0);
+    std::vector<int32_t> id(n), st(n);
+    const double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.0;
+    int rc = mpmw_gen_jittered_slab(seed, cx, cy, cz, s->prm.cell_size, origin, s->prm.density, fixture,
+                                    pos.data(), mass.data(), vol.data(), id.data(), st.data());
+    if (rc) return rc;
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) velv[3 * i + a] = (st[i] == MPMW_FIXED_BOUNDARY) ? 0.0 : vel[a];
+    mpmw_particle_arrays a{};
+    a.n = (int32_t)n;
+    a.pos = pos.data(); a.vel = velv.data(); a.mass = mass.data(); a.volume = vol.data();
+    a.temperature = T.data(); a.plastic_strain = z1.data(); a.strain_rate = z1.data(); a.damage = z1.data();
+    a.residual_stress = z1.data(); a.stress = z6.data(); a.id = id.data(); a.status = st.data();
+    return mpmw_set_particles(s, &a);
+}
+
+int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double contact_radius) {
+    CHECK_S(s);
+    if (n < 0 || (n > 0 && !xyz)) return fail(MPMW_E_INVAL, "set_tool_samples: bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    // setToolMesh, :793-810
+    s->prm.tool_contact_radius = std::max(contact_radius, s->h);
+    const int stride = std::max(1, n / 4000);
+    std::vector<double> pts;
+    for (int64_t i = 0; i < n; i += stride)
+        for (int a = 0; a < 3; ++a) pts.push_back(xyz[3 * i + a]);
+    const int32_t m = (int32_t)(pts.size() / 3);
+    if (!finite_array(pts.data(), (int64_t)pts.size())) return fail(MPMW_E_INVAL, "set_tool_samples: non-finite");
+    dfree(s, s->samples, 0);
+    s->nsamples = 0;
+    s->tool_kind = MPMW_TOOL_NONE;
+    if (m == 0) return MPMW_OK;
+    int rc = dalloc(s, &s->samples, 3 * s->rsz * (size_t)m);
+    if (rc) return rc;
+    const double rcr = s->prm.tool_contact_radius;
+    for (int a = 0; a < 3; ++a) { s->sample_lo[a] = 1e300; s->sample_hi[a] = -1e300; }
+    for (int32_t i = 0; i < m; ++i)
+        for (int a = 0; a < 3; ++a) {
+            s->sample_lo[a] = std::min(s->sample_lo[a], pts[3 * i + a]);
+            s->sample_hi[a] = std::max(s->sample_hi[a], pts[3 * i + a]);
+        }
+    for (int a = 0; a < 3; ++a) {  // grown by r_c plus a margin for FP32 node positions
+        const double marg = rcr * 1.001 + 1e-6 * (std::fabs(s->sample_lo[a]) + std::fabs(s->sample_hi[a])) + s->h * 1e-3;
+        s->sample_lo[a] -= marg;
+        s->sample_hi[a] += marg;
+    }
+    if (s->fp64) {
+        std::vector<double> soa(3 * (size_t)m);
+        for (int32_t i = 0; i < m; ++i)
+            for (int a = 0; a < 3; ++a) soa[(size_t)a * m + i] = pts[3 * i + a];
+        HIP_TRY(hipMemcpy(s->samples, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> soa(3 * (size_t)m);
+        for (int32_t i = 0; i < m; ++i)
+            for (int a = 0; a < 3; ++a) soa[(size_t)a * m + i] = (float)pts[3 * i + a];
+        HIP_TRY(hipMemcpy(s->samples, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice));
+    }
+    s->nsamples = m;
+    s->tool_kind = MPMW_TOOL_POINTS;
+    return MPMW_OK;
+}
+
+int mpmw_set_tool_analytic(mpmw_solver *s, const mpmw_tool_desc *d) {
+    CHECK_S(s);
+    if (!d) return fail(MPMW_E_INVAL, "null tool description");
+    if (d->kind == MPMW_TOOL_NONE) {
+        s->tool_kind = MPMW_TOOL_NONE;
+        return MPMW_OK;
+    }
+    if (d->kind != MPMW_TOOL_WEDGE && d->kind != MPMW_TOOL_DRILL)
+        return fail(MPMW_E_INVAL, "tool kind must be NONE, WEDGE or DRILL here");
+    if (!(d->grad_eps > 0.0)) return fail(MPMW_E_INVAL, "tool grad_eps must be > 0");
+    if (d->kind == MPMW_TOOL_DRILL && !(d->radius > 0.0)) return fail(MPMW_E_INVAL, "drill radius must be > 0");
+    if (d->friction < 0.0) return fail(MPMW_E_INVAL, "friction must be >= 0");
+    s->desc = *d;
+    s->tool_kind = d->kind;
+    return MPMW_OK;
+}
+
+int mpmw_set_tool_pose(mpmw_solver *s, const mpmw_tool_pose *p) {
+    CHECK_S(s);
+    if (!p) return fail(MPMW_E_INVAL, "null pose");
+    s->pose = *p;
+    return MPMW_OK;
+}
+
+int mpmw_set_tool_tip(mpmw_solver *s, double x, double y, double z) {
+    CHECK_S(s);
+    s->tip[0] = x; s->tip[1] = y; s->tip[2] = z;
+    return MPMW_OK;
+}
+
+int mpmw_apply_external_force(mpmw_solver *s, const double c[3], double radius, const double f[3]) {
+    CHECK_S(s);
+    if (!c || !f) return fail(MPMW_E_INVAL, "null argument");
+    if (s->n <= 0) return MPMW_OK;
+    if (!s->has_fext) {
+        s->has_fext = true;
+        for (int k = F_FX; k <= F_FZ; ++k) {
+            int rc = ensure_field(s, k);
+            if (rc) return rc;
+        }
+        if (s->alt_cap) {
+            for (int k = F_FX; k <= F_FZ; ++k) dfree(s, s->alt[k], field_bytes(s, k, s->alt_cap));
+            s->alt_cap = 0;
+        }
+    }
+    if (s->fp64)
+        hipLaunchKernelGGL(k_external_force<double>, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->stream,
+                           pview<double>(s), c[0], c[1], c[2], radius * radius, f[0], f[1], f[2]);
+    else
+        hipLaunchKernelGGL(k_external_force<float>, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->stream,
+                           pview<float>(s), (float)c[0], (float)c[1], (float)c[2], (float)(radius * radius),
+                           (float)f[0], (float)f[1], (float)f[2]);
+    HIP_TRY(hipGetLastError());
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+int mpmw_step(mpmw_solver *s, double dt) {
+    CHECK_S(s);
+    if (!std::isfinite(dt)) return fail(MPMW_E_INVAL, "step: dt must be finite");
+    return DISPATCH(s, step_impl, s, dt);
+}
+
+int mpmw_run(mpmw_solver *s, int32_t nsteps, double dt) {
+    CHECK_S(s);
+    for (int32_t i = 0; i < nsteps; ++i) {
+        int rc = mpmw_step(s, dt);
+        if (rc) return rc;
+    }
+    return MPMW_OK;
+}
+
+int mpmw_synchronize(mpmw_solver *s) {
+    CHECK_S(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPMW_OK;
+}
+
+int mpmw_stable_dt(mpmw_solver *s, double *dt) {
+    CHECK_S(s);
+    if (!dt) return fail(MPMW_E_INVAL, "null");
+    // getStableTimeStep, :820-828
+    const mpmw_params &p = s->prm;
+    double wave = std::sqrt(std::max(p.young / std::max(p.density, 1.0), 1.0));
+    double v = 0.35 * s->h / wave;
+    if (p.has_config) v = std::max(p.min_time_step, std::min(p.max_time_step, v));
+    *dt = v;
+    return MPMW_OK;
+}
+
+int mpmw_get_metrics(mpmw_solver *s, mpmw_metrics *m) {
+    CHECK_S(s);
+    if (!m) return fail(MPMW_E_INVAL, "null");
+    if (s->metrics_pending || s->cf_pending) {
+        if (s->cf_pending)
+            HIP_TRY(hipMemcpyAsync(s->h_cf, s->d_cf, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        if (s->metrics_pending) absorb_stats(s);
+        if (s->cf_pending) s->contact_force = *s->h_cf;
+        s->metrics_pending = s->cf_pending = false;
+    }
+    m->max_stress = s->max_stress;
+    m->max_temperature = s->max_temp;
+    m->kinetic_energy = s->ke;
+    m->contact_force = s->contact_force;
+    m->time = s->time;
+    m->step = s->step;
+    return MPMW_OK;
+}
+
+int mpmw_sync_metrics(mpmw_solver *s) {
+    CHECK_S(s);
+    if (s->n <= 0) return MPMW_OK;
+    int rc = DISPATCH(s, reduce_particles, s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    absorb_stats(s);
+    s->metrics_pending = false;
+    s->stats_fresh = true;
+    return MPMW_OK;
+}
+
+int mpmw_compact(mpmw_solver *s) {
+    CHECK_S(s);
+    return DISPATCH(s, compact_impl, s);
+}
+
+int mpmw_get_bounds(mpmw_solver *s, double b[6]) {
+    CHECK_S(s);
+    if (!b) return fail(MPMW_E_INVAL, "null");
+    if (s->metrics_pending) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        absorb_stats(s);
+        s->metrics_pending = false;
+    }
+    if (s->n == 0) {
+        for (int k = 0; k < 6; ++k) b[k] = 0.0;
+        return MPMW_OK;
+    }
+    for (int k = 0; k < 6; ++k) b[k] = s->bounds[k];
+    return MPMW_OK;
+}
+
+
+int mpmw_temperature_at(mpmw_solver *s, double x, double y, double z, double *t) {
+    CHECK_S(s);
+    if (!t) return fail(MPMW_E_INVAL, "null");
+    *t = s->prm.ref_temp;
+    if (s->n <= 0) return MPMW_OK;
+    int rc = DISPATCH(s, nearest_impl, s, x, y, z, 0, 0.0);
+    if (rc) return rc;
+    int32_t bi = -1;
+    HIP_TRY(hipMemcpyAsync(&bi, s->d_near_out_i, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (bi < 0) return MPMW_OK;
+    if (s->fp64) {
+        double v;
+        HIP_TRY(hipMemcpy(&v, (double *)s->f[F_T] + bi, sizeof(double), hipMemcpyDeviceToHost));
+        *t = v;
+    } else {
+        float v;
+        HIP_TRY(hipMemcpy(&v, (float *)s->f[F_T] + bi, sizeof(float), hipMemcpyDeviceToHost));
+        *t = v;
+    }
+    return MPMW_OK;
+}
+
+int mpmw_apply_heat_flux(mpmw_solver *s, double x, double y, double z, double q) {
+    CHECK_S(s);
+    if (s->n <= 0) return MPMW_OK;
+    int rc = DISPATCH(s, nearest_impl, s, x, y, z, 1, q);
+    if (rc) return rc;
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t count) {
+    CHECK_S(s);
+    if (!t || count <= 0 || count > s->n) return MPMW_OK;  // :907 ignores bad counts
+    void *tmp = nullptr;
+    int rc = dalloc(s, &tmp, sizeof(double) * (size_t)count);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(tmp, t, sizeof(double) * count, hipMemcpyHostToDevice, s->stream));
+    if (s->fp64)
+        hipLaunchKernelGGL(k_set_temperature<double>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
+                           (double *)s->f[F_T], (const double *)tmp, count);
+    else
+        hipLaunchKernelGGL(k_set_temperature<float>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
+                           (float *)s->f[F_T], (const double *)tmp, count);
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(s->stream);
+    dfree(s, tmp, sizeof(double) * (size_t)count);
+    if (e != hipSuccess) return fail(MPMW_E_HIP, hipGetErrorString(e));
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+int mpmw_set_deterministic(mpmw_solver *s, int32_t on) {
+    CHECK_S(s);
+    s->deterministic = on != 0;
+    s->p2g_ready = false;
+    return MPMW_OK;
+}
+
+int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g) {
+    CHECK_S(s);
+    if (!g) return fail(MPMW_E_INVAL, "null");
+    g->cell = s->h;
+    g->minx = s->minx; g->miny = s->miny; g->minz = s->minz;
+    g->nx = s->nx; g->ny = s->ny; g->nz = s->nz;
+    g->pad_ = 0;
+    return MPMW_OK;
+}
+
+int mpmw_set_grid_snapshot(mpmw_solver *s, int32_t on) {
+    CHECK_S(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->snap_on = on != 0;
+    s->snap_valid = false;
+    if (s->snap_on && !s->snap && s->nn_cap > 0) {
+        int rc = dalloc(s, &s->snap, 8 * s->rsz * (size_t)s->nn_cap);
+        if (rc) { s->snap_on = false; return rc; }
+    }
+    return MPMW_OK;
+}
+
+int mpmw_get_grid_snapshot(mpmw_solver *s, double *mass, double *mom, double *force, double *tm) {
+    CHECK_S(s);
+    if (!s->snap_valid || !s->snap) return fail(MPMW_E_STATE, "no grid snapshot: enable it and step first");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int64_t nn = s->nn;
+    std::vector<double> buf((size_t)(8 * nn));
+    if (s->fp64) {
+        HIP_TRY(hipMemcpy(buf.data(), s->snap, sizeof(double) * 8 * nn, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f((size_t)(8 * nn));
+        HIP_TRY(hipMemcpy(f.data(), s->snap, sizeof(float) * 8 * nn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); ++i) buf[i] = f[i];
+    }
+    for (int64_t i = 0; i < nn; ++i) {
+        if (mass) mass[i] = buf[i];
+        if (mom) for (int a = 0; a < 3; ++a) mom[3 * i + a] = buf[(1 + a) * nn + i];
+        if (force) for (int a = 0; a < 3; ++a) force[3 * i + a] = buf[(4 + a) * nn + i];
+        if (tm) tm[i] = buf[7 * nn + i];
+    }
+    return MPMW_OK;
+}
+
+int mpmw_set_kernel_timing(mpmw_solver *s, int32_t on) {
+    CHECK_S(s);
+    drain_events(s);
+    s->timing = on != 0;
+    for (int k = 0; k < 8; ++k) { s->t_ms[k] = 0; s->t_n[k] = 0; }
+    return MPMW_OK;
+}
+
+int mpmw_get_kernel_times(mpmw_solver *s, mpmw_kernel_times *t) {
+    CHECK_S(s);
+    if (!t) return fail(MPMW_E_INVAL, "null");
+    drain_events(s);
+    for (int k = 0; k < 8; ++k) { t->ms[k] = s->t_ms[k]; t->launches[k] = s->t_n[k]; }
+    return MPMW_OK;
+}
+
+int mpmw_get_stream(mpmw_solver *s, void **stream) {
+    CHECK_S(s);
+    if (!stream) return fail(MPMW_E_INVAL, "null");
+    *stream = (void *)s->stream;
+    return MPMW_OK;
+}
+
+int mpmw_device_bytes(mpmw_solver *s, int64_t *bytes) {
+    CHECK_S(s);
+    if (!bytes) return fail(MPMW_E_INVAL, "null");
+    *bytes = s->device_bytes;
+    return MPMW_OK;
+}
+
+}  // extern "C"

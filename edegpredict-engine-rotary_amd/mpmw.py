@@ -1,0 +1,358 @@
+"""ctypes binding of libmpmw_hip.so (the C ABI in include/mpmw.h).
+
+Python-side access to the MI355X MPM workpiece solver for tests and the
+benchmark.  The product path is the HIP library; there is no CPU fallback:
+loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmpmw_hip.so")
+
+ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
+FP32, FP64 = 0, 1
+TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL = 0, 1, 2, 3
+K_G2P_P2G, K_P2G, K_GRID, K_DIFFUSE, K_SORT, K_REDUCE, K_OTHER = range(7)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+# Every symbol the C ABI exports (include/mpmw.h).
+EXPORTS = (
+    "mpmw_last_error", "mpmw_abi_version", "mpmw_params_default", "mpmw_create", "mpmw_destroy",
+    "mpmw_reset", "mpmw_set_particles", "mpmw_get_particles", "mpmw_particle_count",
+    "mpmw_init_particle_box", "mpmw_init_cylinder", "mpmw_gen_jittered_slab", "mpmw_init_jittered_slab",
+    "mpmw_set_tool_samples", "mpmw_set_tool_analytic", "mpmw_set_tool_pose", "mpmw_set_tool_tip",
+    "mpmw_apply_external_force", "mpmw_step", "mpmw_run", "mpmw_synchronize", "mpmw_stable_dt",
+    "mpmw_get_metrics", "mpmw_sync_metrics", "mpmw_compact", "mpmw_get_bounds", "mpmw_temperature_at",
+    "mpmw_apply_heat_flux", "mpmw_set_particle_temperatures", "mpmw_set_deterministic",
+    "mpmw_get_grid_info", "mpmw_set_grid_snapshot", "mpmw_get_grid_snapshot", "mpmw_set_kernel_timing",
+    "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "density", "young", "poisson", "specific_heat", "melt_temp", "ref_temp",
+        "jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "ref_strain_rate",
+        "failure_strain", "damage_threshold", "critical_plastic_strain", "thermal_chip_ratio",
+        "tool_contact_radius", "contact_damping", "max_contact_temp_rise",
+        "thermal_conductivity", "kinematic_hardening")] + [
+        ("thermal_clamp_enabled", C.c_int32), ("has_config", C.c_int32),
+        ("cell_size", C.c_double), ("particle_spacing_factor", C.c_double),
+        ("fixture_thickness", C.c_double), ("min_time_step", C.c_double),
+        ("max_time_step", C.c_double), ("max_particles", C.c_int32), ("precision", C.c_int32),
+        ("deterministic", C.c_int32), ("device", C.c_int32)]
+
+
+class ParticleArrays(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pad_", C.c_int32)] + [(n, _dp) for n in (
+        "pos", "vel", "fext", "mass", "volume", "density", "temperature",
+        "plastic_strain", "strain_rate", "damage", "residual_stress", "stress",
+        "backstress")] + [("id", _ip), ("status", _ip)]
+
+
+class ToolDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32)] + [(n, C.c_double) for n in (
+        "band", "friction", "grad_eps", "rake_angle", "clearance_angle", "radius",
+        "helix_angle", "point_angle", "flute_offset", "flute_radius", "flute_length")]
+
+
+class ToolPose(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("angle", C.c_double),
+                ("vel", C.c_double * 3), ("omega", C.c_double)]
+
+
+class Metrics(C.Structure):
+    _fields_ = [("max_stress", C.c_double), ("max_temperature", C.c_double),
+                ("kinetic_energy", C.c_double), ("contact_force", C.c_double),
+                ("time", C.c_double), ("step", C.c_int64)]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("cell", C.c_double), ("minx", C.c_double), ("miny", C.c_double),
+                ("minz", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
+
+
+class MPMError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libmpmw_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MPMError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "mpmw_last_error": (C.c_char_p, []),
+        "mpmw_abi_version": (C.c_int, []),
+        "mpmw_params_default": (None, [C.POINTER(Params)]),
+        "mpmw_create": (C.c_int, [C.POINTER(Params), C.POINTER(vp)]),
+        "mpmw_destroy": (C.c_int, [vp]),
+        "mpmw_reset": (C.c_int, [vp]),
+        "mpmw_set_particles": (C.c_int, [vp, C.POINTER(ParticleArrays)]),
+        "mpmw_get_particles": (C.c_int, [vp, C.POINTER(ParticleArrays)]),
+        "mpmw_particle_count": (C.c_int, [vp, _ip]),
+        "mpmw_init_particle_box": (C.c_int, [vp, _dp, _dp, C.c_double]),
+        "mpmw_init_cylinder": (C.c_int, [vp, _dp, C.c_double, C.c_double, C.c_double, C.c_int32]),
+        "mpmw_gen_jittered_slab": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp,
+                                             C.c_double, C.c_double, _dp, _dp, _dp, _ip, _ip]),
+        "mpmw_init_jittered_slab": (C.c_int, [vp, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _dp, _dp,
+                                              C.c_double]),
+        "mpmw_set_tool_samples": (C.c_int, [vp, _dp, C.c_int32, C.c_double]),
+        "mpmw_set_tool_analytic": (C.c_int, [vp, C.POINTER(ToolDesc)]),
+        "mpmw_set_tool_pose": (C.c_int, [vp, C.POINTER(ToolPose)]),
+        "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
+        "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
+        "mpmw_step": (C.c_int, [vp, C.c_double]),
+        "mpmw_run": (C.c_int, [vp, C.c_int32, C.c_double]),
+        "mpmw_synchronize": (C.c_int, [vp]),
+        "mpmw_stable_dt": (C.c_int, [vp, _dp]),
+        "mpmw_get_metrics": (C.c_int, [vp, C.POINTER(Metrics)]),
+        "mpmw_sync_metrics": (C.c_int, [vp]),
+        "mpmw_compact": (C.c_int, [vp]),
+        "mpmw_get_bounds": (C.c_int, [vp, _dp]),
+        "mpmw_temperature_at": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, _dp]),
+        "mpmw_apply_heat_flux": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "mpmw_set_particle_temperatures": (C.c_int, [vp, _dp, C.c_int32]),
+        "mpmw_set_deterministic": (C.c_int, [vp, C.c_int32]),
+        "mpmw_get_grid_info": (C.c_int, [vp, C.POINTER(GridInfo)]),
+        "mpmw_set_grid_snapshot": (C.c_int, [vp, C.c_int32]),
+        "mpmw_get_grid_snapshot": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "mpmw_set_kernel_timing": (C.c_int, [vp, C.c_int32]),
+        "mpmw_get_kernel_times": (C.c_int, [vp, C.POINTER(KernelTimes)]),
+        "mpmw_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "mpmw_device_bytes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().mpmw_last_error().decode(errors="replace")
+        raise MPMError(f"{what} failed ({rc}): {msg}")
+
+
+def _p(a, ct=_dp):
+    return None if a is None else a.ctypes.data_as(ct)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().mpmw_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def make_arrays(n):
+    return {
+        "pos": np.zeros((n, 3)), "vel": np.zeros((n, 3)), "fext": np.zeros((n, 3)),
+        "mass": np.zeros(n), "volume": np.zeros(n), "density": np.zeros(n),
+        "temperature": np.zeros(n), "plastic_strain": np.zeros(n), "strain_rate": np.zeros(n),
+        "damage": np.zeros(n), "residual_stress": np.zeros(n),
+        "stress": np.zeros((n, 6)), "backstress": np.zeros((n, 6)),
+        "id": np.zeros(n, dtype=np.int32), "status": np.zeros(n, dtype=np.int32),
+    }
+
+
+def _arrays_struct(d):
+    a = ParticleArrays()
+    a.n = len(d["mass"])
+    keep = {}
+    for k in ("pos", "vel", "fext", "mass", "volume", "density", "temperature", "plastic_strain",
+              "strain_rate", "damage", "residual_stress", "stress", "backstress"):
+        v = d.get(k)
+        if v is not None:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            keep[k] = v
+        setattr(a, k, _p(v))
+    for k in ("id", "status"):
+        v = np.ascontiguousarray(d[k], dtype=np.int32)
+        keep[k] = v
+        setattr(a, k, _p(v, _ip))
+    return a, keep
+
+
+def gen_jittered_slab(seed, cx, cy, cz, h, origin=(0.0, 0.0, 0.0), density=4430.0, fixture=0.0):
+    n = cx * cy * cz * 8
+    d = make_arrays(n)
+    d["density"][:] = density
+    org = np.asarray(origin, dtype=np.float64)
+    _check(lib().mpmw_gen_jittered_slab(seed, cx, cy, cz, h, _p(org), density, fixture, _p(d["pos"]),
+                                        _p(d["mass"]), _p(d["volume"]), _p(d["id"], _ip),
+                                        _p(d["status"], _ip)), "gen_jittered_slab")
+    return d
+
+
+class Solver:
+    """One MPM workpiece on one MI355X (a handle on the C ABI)."""
+
+    def __init__(self, params=None, **kw):
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        _check(lib().mpmw_create(C.byref(self.params), C.byref(h)), "mpmw_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpmw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_particles(self, d):
+        a, keep = _arrays_struct(d)
+        _check(lib().mpmw_set_particles(self.h, C.byref(a)), "set_particles")
+
+    def count(self):
+        n = C.c_int32()
+        _check(lib().mpmw_particle_count(self.h, C.byref(n)), "particle_count")
+        return n.value
+
+    def get_particles(self):
+        d = make_arrays(self.count())
+        a, keep = _arrays_struct(d)
+        _check(lib().mpmw_get_particles(self.h, C.byref(a)), "get_particles")
+        for k, v in keep.items():
+            d[k] = v
+        return d
+
+    def init_box(self, mn, mx, spacing):
+        _check(lib().mpmw_init_particle_box(self.h, _p(np.asarray(mn, float)), _p(np.asarray(mx, float)),
+                                            spacing), "init_particle_box")
+
+    def init_cylinder(self, c, radius, length, spacing, axis=2):
+        _check(lib().mpmw_init_cylinder(self.h, _p(np.asarray(c, float)), radius, length, spacing, axis),
+               "init_cylinder")
+
+    def init_jittered_slab(self, seed, cx, cy, cz, origin=(0, 0, 0), vel=(0, 0, 0), temperature=293.0):
+        _check(lib().mpmw_init_jittered_slab(self.h, seed, cx, cy, cz, _p(np.asarray(origin, float)),
+                                             _p(np.asarray(vel, float)), temperature), "init_jittered_slab")
+
+    def set_tool_samples(self, xyz, contact_radius):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        _check(lib().mpmw_set_tool_samples(self.h, _p(xyz), len(xyz), contact_radius), "set_tool_samples")
+
+    def set_tool_analytic(self, desc):
+        _check(lib().mpmw_set_tool_analytic(self.h, C.byref(desc)), "set_tool_analytic")
+
+    def set_tool_pose(self, pose):
+        _check(lib().mpmw_set_tool_pose(self.h, C.byref(pose)), "set_tool_pose")
+
+    def set_tool_tip(self, x, y, z):
+        _check(lib().mpmw_set_tool_tip(self.h, x, y, z), "set_tool_tip")
+
+    def apply_external_force(self, c, radius, f):
+        _check(lib().mpmw_apply_external_force(self.h, _p(np.asarray(c, float)), radius,
+                                               _p(np.asarray(f, float))), "apply_external_force")
+
+    def step(self, dt):
+        _check(lib().mpmw_step(self.h, dt), "step")
+
+    def run(self, nsteps, dt):
+        _check(lib().mpmw_run(self.h, nsteps, dt), "run")
+
+    def synchronize(self):
+        _check(lib().mpmw_synchronize(self.h), "synchronize")
+
+    def stable_dt(self):
+        v = C.c_double()
+        _check(lib().mpmw_stable_dt(self.h, C.byref(v)), "stable_dt")
+        return v.value
+
+    def metrics(self):
+        m = Metrics()
+        _check(lib().mpmw_get_metrics(self.h, C.byref(m)), "get_metrics")
+        return m
+
+    def sync_metrics(self):
+        _check(lib().mpmw_sync_metrics(self.h), "sync_metrics")
+
+    def compact(self):
+        _check(lib().mpmw_compact(self.h), "compact")
+
+    def reset(self):
+        _check(lib().mpmw_reset(self.h), "reset")
+
+    def bounds(self):
+        b = np.zeros(6)
+        _check(lib().mpmw_get_bounds(self.h, _p(b)), "get_bounds")
+        return b
+
+    def temperature_at(self, x, y, z):
+        v = C.c_double()
+        _check(lib().mpmw_temperature_at(self.h, x, y, z, C.byref(v)), "temperature_at")
+        return v.value
+
+    def apply_heat_flux(self, x, y, z, q):
+        _check(lib().mpmw_apply_heat_flux(self.h, x, y, z, q), "apply_heat_flux")
+
+    def set_particle_temperatures(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        _check(lib().mpmw_set_particle_temperatures(self.h, _p(t), len(t)), "set_particle_temperatures")
+
+    def set_deterministic(self, on):
+        _check(lib().mpmw_set_deterministic(self.h, 1 if on else 0), "set_deterministic")
+
+    def grid_info(self):
+        g = GridInfo()
+        _check(lib().mpmw_get_grid_info(self.h, C.byref(g)), "get_grid_info")
+        return g
+
+    def set_grid_snapshot(self, on):
+        _check(lib().mpmw_set_grid_snapshot(self.h, 1 if on else 0), "set_grid_snapshot")
+
+    def grid_snapshot(self):
+        g = self.grid_info()
+        n = g.nx * g.ny * g.nz
+        m, p, f, tm = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+        _check(lib().mpmw_get_grid_snapshot(self.h, _p(m), _p(p), _p(f), _p(tm)), "get_grid_snapshot")
+        return {"mass": m, "mom": p, "force": f, "tm": tm}
+
+    def set_kernel_timing(self, on):
+        _check(lib().mpmw_set_kernel_timing(self.h, 1 if on else 0), "set_kernel_timing")
+
+    def kernel_times(self):
+        t = KernelTimes()
+        _check(lib().mpmw_get_kernel_times(self.h, C.byref(t)), "get_kernel_times")
+        return list(t.ms), list(t.launches)
+
+    def stream(self):
+        v = C.c_void_p()
+        _check(lib().mpmw_get_stream(self.h, C.byref(v)), "get_stream")
+        return v.value
+
+    def device_bytes(self):
+        v = C.c_int64()
+        _check(lib().mpmw_device_bytes(self.h, C.byref(v)), "device_bytes")
+        return v.value

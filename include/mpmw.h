@@ -1,0 +1,222 @@
+/*
+ * mpmw.h -- C ABI of the MI355X MPM workpiece solver (libmpmw_hip.so).
+ *
+ * Drop-in replacement for the device side of the reference's
+ * MPMWorkpieceSolver (include/MPMWorkpieceSolver.cuh:92-179,
+ * src/MPMWorkpieceSolver.cu:418-946 of Up25-pathan/edegpredict-engine-rotary).
+ * Plain C: opaque handle, host pointers and sizes, int status codes; no HIP
+ * or torch types appear here.  The C++ class edgepredict::MPMWorkpieceSolver
+ * (include/edgepredict/MPMWorkpieceSolver.h) maps these onto the reference's
+ * IPhysicsSolver / IMetricsProvider / IThermalCoupling signatures.
+ *
+ * Every entry point returns MPMW_OK (0) or a negative error code; the text of
+ * the last error on the calling thread is returned by mpmw_last_error().
+ * A solver handle is not thread-safe: calls on one handle must not overlap.
+ */
+#ifndef MPMW_H
+#define MPMW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPMW_ABI_VERSION 1
+
+enum mpmw_status_code {
+    MPMW_OK = 0,
+    MPMW_E_INVAL = -1,     /* bad argument (null pointer, size, non-finite value) */
+    MPMW_E_HIP = -2,       /* HIP runtime error */
+    MPMW_E_STATE = -3,     /* call not valid in the solver's current state */
+    MPMW_E_NOMEM = -4,     /* device allocation failed */
+    MPMW_E_RANGE = -5      /* size beyond a documented limit */
+};
+
+/* ParticleStatus, include/Types.h:37-43 (same values) */
+enum mpmw_particle_status {
+    MPMW_ACTIVE = 0,
+    MPMW_INACTIVE = 1,
+    MPMW_CHIP = 2,
+    MPMW_BOUNDARY = 3,
+    MPMW_FIXED_BOUNDARY = 4
+};
+
+enum mpmw_precision { MPMW_FP32 = 0, MPMW_FP64 = 1 };
+
+enum mpmw_tool_kind {
+    MPMW_TOOL_NONE = 0,
+    MPMW_TOOL_POINTS = 1,  /* reference point samples (setToolMesh) */
+    MPMW_TOOL_WEDGE = 2,   /* analytic rake/flank wedge */
+    MPMW_TOOL_DRILL = 3    /* analytic two-flute helical drill */
+};
+
+/* Solver parameters.  Material fields are MPMMaterialParams
+ * (MPMWorkpieceSolver.cuh:65-90) as MPMWorkpieceSolver::initialize fills them
+ * (MPMWorkpieceSolver.cu:428-451); the setup fields are the Config values the
+ * solver reads through its stored Config pointer (maxParticles :591,
+ * fixtureLayerThickness :604,742, min/max time step :823-825).
+ * mpmw_params_default() fills the reference Config defaults. */
+typedef struct mpmw_params {
+    double density, young, poisson, specific_heat, melt_temp, ref_temp;
+    double jc_A, jc_B, jc_n, jc_C, jc_m, ref_strain_rate;
+    double failure_strain, damage_threshold, critical_plastic_strain, thermal_chip_ratio;
+    double tool_contact_radius, contact_damping, max_contact_temp_rise;
+    double thermal_conductivity, kinematic_hardening;
+    int32_t thermal_clamp_enabled;
+    int32_t has_config;            /* 0 reproduces a solver built without a Config */
+    double cell_size;              /* smoothingRadius: grid cell h */
+    double particle_spacing_factor;
+    double fixture_thickness;
+    double min_time_step, max_time_step;
+    int32_t max_particles;
+    int32_t precision;             /* mpmw_precision */
+    int32_t deterministic;         /* 1: sorted fixed-order reductions, bit-exact reruns */
+    int32_t device;                /* HIP device ordinal */
+} mpmw_params;
+
+/* Host-side particle arrays (FP64 and int32, caller-owned).  Layout of the
+ * fields follows MPMParticleGPU (MPMWorkpieceSolver.cuh:18-45); vectors are
+ * interleaved xyz, tensors are xx yy zz xy xz yz.  fext, backstress and
+ * density may be NULL (meaning zero / zero / material density on input). */
+typedef struct mpmw_particle_arrays {
+    int32_t n;
+    int32_t pad_;
+    double *pos, *vel, *fext;
+    double *mass, *volume, *density, *temperature;
+    double *plastic_strain, *strain_rate, *damage, *residual_stress;
+    double *stress, *backstress;
+    int32_t *id, *status;
+} mpmw_particle_arrays;
+
+/* Analytic tool geometry + contact law (grid-node SDF collision; the
+ * reference's particle form is ContactSolver.cu:797-923). */
+typedef struct mpmw_tool_desc {
+    int32_t kind;
+    int32_t pad_;
+    double band;               /* contact when phi < band (m) */
+    double friction;           /* Coulomb coefficient */
+    double grad_eps;           /* central-difference step for the normal (m) */
+    double rake_angle, clearance_angle;                  /* wedge (rad) */
+    double radius, helix_angle, point_angle;             /* drill (m, rad, rad) */
+    double flute_offset, flute_radius, flute_length;     /* drill (m) */
+} mpmw_tool_desc;
+
+/* Tool pose; the pose advances by vel*dt and omega*dt after every step. */
+typedef struct mpmw_tool_pose {
+    double center[3];
+    double angle;
+    double vel[3];
+    double omega;
+} mpmw_tool_pose;
+
+typedef struct mpmw_metrics {
+    double max_stress, max_temperature, kinetic_energy, contact_force;
+    double time;
+    int64_t step;
+} mpmw_metrics;
+
+typedef struct mpmw_grid_info {
+    double cell, minx, miny, minz;
+    int32_t nx, ny, nz, pad_;
+} mpmw_grid_info;
+
+/* Per-kernel timing collected with HIP events when enabled. */
+typedef struct mpmw_kernel_times {
+    double ms[8];              /* accumulated milliseconds per kernel class */
+    int64_t launches[8];       /* launches per kernel class */
+} mpmw_kernel_times;
+
+enum mpmw_kernel_class {
+    MPMW_K_G2P_P2G = 0,        /* fused G2P + constitutive + chip + next P2G */
+    MPMW_K_P2G = 1,            /* standalone P2G */
+    MPMW_K_GRID = 2,           /* grid update + tool contact */
+    MPMW_K_DIFFUSE = 3,        /* grid heat diffusion */
+    MPMW_K_SORT = 4,           /* binning / sort / permute */
+    MPMW_K_REDUCE = 5,         /* AABB / metrics / contact-force reductions */
+    MPMW_K_OTHER = 6
+};
+
+typedef struct mpmw_solver mpmw_solver;
+
+const char *mpmw_last_error(void);
+int mpmw_abi_version(void);
+
+/* --- lifecycle: MPMWorkpieceSolver ctor/initialize/~ (MPMWorkpieceSolver.cu:418-458) */
+void mpmw_params_default(mpmw_params *p);
+int mpmw_create(const mpmw_params *p, mpmw_solver **out);
+int mpmw_destroy(mpmw_solver *s);
+/* reset(), MPMWorkpieceSolver.cu:830-841 */
+int mpmw_reset(mpmw_solver *s);
+
+/* --- particles: uploadParticles/downloadParticles (:546-560) and
+ *     getParticlesForExport (:870-894) in SoA form.  get_particles writes
+ *     min(a->n, count) particles in device order and sets a->n. */
+int mpmw_set_particles(mpmw_solver *s, const mpmw_particle_arrays *a);
+int mpmw_get_particles(mpmw_solver *s, mpmw_particle_arrays *a);
+int mpmw_particle_count(mpmw_solver *s, int32_t *n);
+/* initializeParticleBox (:588-614) and initializeCylindricalWorkpiece (:616-648) */
+int mpmw_init_particle_box(mpmw_solver *s, const double mn[3], const double mx[3], double spacing);
+int mpmw_init_cylinder(mpmw_solver *s, const double c[3], double radius, double length,
+                       double spacing, int32_t axis);
+/* Seeded jittered slab (8 particles per cell, float-exact positions); fills
+ * caller arrays of cx*cy*cz*8 entries.  No counterpart in the reference: it
+ * generates the benchmark workpieces. */
+int mpmw_gen_jittered_slab(uint64_t seed, int32_t cx, int32_t cy, int32_t cz, double h,
+                           const double origin[3], double density, double fixture, double *pos,
+                           double *mass, double *volume, int32_t *id, int32_t *status);
+/* Generate the same slab straight into the solver (no host round trip). */
+int mpmw_init_jittered_slab(mpmw_solver *s, uint64_t seed, int32_t cx, int32_t cy, int32_t cz,
+                            const double origin[3], const double vel[3], double temperature);
+
+/* --- tool: setToolMesh (:793-810) takes every stride-th of n mesh nodes
+ *     (stride = max(1, n/4000)) and r_c = max(contact_radius, h). */
+int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double contact_radius);
+int mpmw_set_tool_analytic(mpmw_solver *s, const mpmw_tool_desc *d);
+int mpmw_set_tool_pose(mpmw_solver *s, const mpmw_tool_pose *p);
+/* setToolTipPosition, MPMWorkpieceSolver.cuh:133-135 */
+int mpmw_set_tool_tip(mpmw_solver *s, double x, double y, double z);
+/* applyExternalForce, MPMWorkpieceSolver.cu:812-818 */
+int mpmw_apply_external_force(mpmw_solver *s, const double c[3], double radius, const double f[3]);
+
+/* --- the step: MPMWorkpieceSolver::step(double), MPMWorkpieceSolver.cu:727-791.
+ *     Enqueues on the solver's stream; returns before the GPU finishes except
+ *     on grid-rebuild steps (every 25th), which read back the particle AABB. */
+int mpmw_step(mpmw_solver *s, double dt);
+int mpmw_run(mpmw_solver *s, int32_t nsteps, double dt);
+int mpmw_synchronize(mpmw_solver *s);
+/* getStableTimeStep, :820-828 */
+int mpmw_stable_dt(mpmw_solver *s, double *dt);
+/* getMaxStress/getMaxTemperature/getTotalKineticEnergy (refreshed every 25
+ * steps, :781), getTotalContactForce (last step), time, step count. */
+int mpmw_get_metrics(mpmw_solver *s, mpmw_metrics *m);
+/* syncMetrics -> updateMetrics now, :855-868 */
+int mpmw_sync_metrics(mpmw_solver *s);
+/* compactParticles, :475-527 (order-preserving here) */
+int mpmw_compact(mpmw_solver *s);
+/* getBounds, :843-853: AABB of the particles at the last metrics refresh */
+int mpmw_get_bounds(mpmw_solver *s, double b[6]);
+/* getTemperatureAt :936-946, applyHeatFlux :918-934, setParticleTemperatures :906-916 */
+int mpmw_temperature_at(mpmw_solver *s, double x, double y, double z, double *t);
+int mpmw_apply_heat_flux(mpmw_solver *s, double x, double y, double z, double q);
+int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t count);
+
+/* --- inspection (parity / measurement) */
+int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
+int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);
+/* Keep a copy of each step's post-P2G grid (mass, momentum, force, T*m). */
+int mpmw_set_grid_snapshot(mpmw_solver *s, int32_t on);
+/* Copy the last snapshot to host arrays of grid-node length (NULL skips). */
+int mpmw_get_grid_snapshot(mpmw_solver *s, double *mass, double *mom, double *force, double *tm);
+int mpmw_set_kernel_timing(mpmw_solver *s, int32_t on);
+int mpmw_get_kernel_times(mpmw_solver *s, mpmw_kernel_times *t);
+/* The hipStream_t the solver launches on, as an opaque pointer. */
+int mpmw_get_stream(mpmw_solver *s, void **stream);
+/* Device bytes currently held by the solver. */
+int mpmw_device_bytes(mpmw_solver *s, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol the
+header declares, and its host-only functions agree with the oracle.  No
+compute on a device happens here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mpmw
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "mpmw.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpmw_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = mpmw.lib()
+    names = _declared()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(mpmw.EXPORTS)
+    assert L.mpmw_abi_version() == 1
+
+
+def test_params_default_match_oracle_defaults(orc):
+    p = mpmw.default_params()
+    o = orc.default_params()
+    for name, _ in orc.Params._fields_:
+        if name == "pad_":
+            continue
+        assert getattr(p, name) == getattr(o, name), name
+    assert p.precision == mpmw.FP32 and p.deterministic == 0
+
+
+def test_generator_matches_oracle_bit_exactly(orc):
+    a = mpmw.gen_jittered_slab(0xED6E0001, 5, 4, 3, 2.5e-4, (1e-3, -2e-3, 0.5e-3), 4430.0, 7e-4)
+    b = orc.gen_jittered_slab(0xED6E0001, 5, 4, 3, 2.5e-4, (1e-3, -2e-3, 0.5e-3), 4430.0, 7e-4)
+    for k in ("pos", "mass", "volume", "id", "status"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (a["status"] == mpmw.FIXED_BOUNDARY).any()
+
+
+def test_generator_rejects_bad_sizes():
+    with pytest.raises(mpmw.MPMError):
+        mpmw.gen_jittered_slab(1, 0, 4, 3, 2.5e-4)
+
+
+def test_create_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(mpmw.MPMError, match="HIP|device"):
+        mpmw.Solver(**mpmw_params())
+
+
+def mpmw_params():
+    return {"cell_size": 2.5e-4}
+
+
+def test_workload_tool_geometry():
+    import workloads
+    pts = workloads.rake_face_samples(0.0, 8e-3, 3.5e-3)
+    assert pts.shape == (65 * 13, 3)
+    # every sample lies on the rake plane through the edge, 5 degrees back
+    n = np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))])
+    assert np.abs((pts - [0.0, 0.0, 3.5e-3]) @ n).max() < 1e-15
+    c1 = workloads.drill_config1()
+    assert np.prod(c1["cells"]) * 8 == 16_777_216
