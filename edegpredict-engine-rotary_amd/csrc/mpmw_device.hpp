@@ -34,16 +34,23 @@ template <typename R> struct Material {
 // MPMGridParams (MPMWorkpieceSolver.cuh:59-63) plus the integer fixture
 // threshold: nodes with k <= kfix satisfy the reference's FP64 test
 // pos.z <= minZ + fixture + h (MPMWorkpieceSolver.cu:153), evaluated on the host.
+// Origin and h are kept in FP64 as well: particle positions are FP64 in every
+// mode, so the cell index and the weight fractions are formed in FP64.
 template <typename R> struct GridView {
-    R minx, miny, minz, h, inv_h;
+    double ox, oy, oz, hd;
+    R inv_h;
     int32_t nx, ny, nz, kfix;
     int64_t nn;
 };
 
 // Particle state, structure of arrays.  a0 == nullptr means all backstress is
-// zero; fx == nullptr means no external force.
+// zero; fx == nullptr means no external force.  Positions are FP64 in every
+// mode: a step moves a particle by ~1e-8 m, a few FP32 ulps of a cm-scale
+// coordinate, and the reference's near-empty free-surface nodes turn weight
+// errors of that size into large velocity errors.
 template <typename R> struct Particles {
-    R *x, *y, *z, *vx, *vy, *vz;
+    double *x, *y, *z;
+    R *vx, *vy, *vz;
     R *s0, *s1, *s2, *s3, *s4, *s5;
     R *a0, *a1, *a2, *a3, *a4, *a5;
     R *fx, *fy, *fz;
@@ -63,18 +70,19 @@ template <typename R> struct Grid {
     R *snap;  // optional copy of acc (8 channels) for parity inspection
 };
 
-// Tool state for the grid update.
-template <typename R> struct Tool {
+// Tool state for the grid update.  Tool geometry (node positions, sample
+// distances, SDF and its gradient) is evaluated in FP64 in every mode.
+struct Tool {
     int32_t kind, nsamples;
-    const R *sx, *sy, *sz;            // point samples (SoA)
-    R lo[3], hi[3];                   // sample AABB grown by r_c (conservative)
-    R rc2;
+    const double *sx, *sy, *sz;       // point samples (SoA)
+    double lo[3], hi[3];              // sample AABB grown by r_c
+    double rc2;
     // analytic
-    R cx, cy, cz, cos_a, sin_a;       // pose
-    R vx, vy, vz, omega;              // tool velocity
-    R band, mu, eps;
-    R sin_rake, cos_rake, sin_clear, cos_clear;
-    R radius, tan_helix, sin_half_point, cos_half_point, flute_offset, flute_radius, flute_length;
+    double cx, cy, cz, cos_a, sin_a;  // pose
+    double vx, vy, vz, omega;         // tool velocity
+    double band, mu, eps;
+    double sin_rake, cos_rake, sin_clear, cos_clear;
+    double radius, tan_helix, sin_half_point, cos_half_point, flute_offset, flute_radius, flute_length;
 };
 
 // --------------------------------------------------------------- helpers --
@@ -100,26 +108,39 @@ __device__ __forceinline__ R jc_flow(const Material<R> &M, R eps, R rate, R T) {
     return strainTerm * rateTerm * thermalTerm;
 }
 
-// Base cell and fractions of computeWeights, MPMWorkpieceSolver.cu:54-62.
+// Base cell and the 1-D weights of computeWeights, MPMWorkpieceSolver.cu:54-62
+// and :77-79: w1 = r - floor(r), w0 = 1 - w1, both formed in FP64 and then
+// rounded once to R.
+template <typename R> struct Frac {
+    int b[3];
+    R w0[3], w1[3];
+};
+
 template <typename R>
-__device__ __forceinline__ void base_and_frac(const GridView<R> &g, R x, R y, R z, int &bx, int &by,
-                                              int &bz, R &fx, R &fy, R &fz) {
-    R rx = (x - g.minx) / g.h, ry = (y - g.miny) / g.h, rz = (z - g.minz) / g.h;
-    R flx = floor(rx), fly = floor(ry), flz = floor(rz);
-    bx = (int)flx; by = (int)fly; bz = (int)flz;
-    fx = rx - (R)bx; fy = ry - (R)by; fz = rz - (R)bz;
+__device__ __forceinline__ Frac<R> base_and_frac(const GridView<R> &g, double x, double y, double z) {
+    Frac<R> f;
+    const double r[3] = {(x - g.ox) / g.hd, (y - g.oy) / g.hd, (z - g.oz) / g.hd};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double fl = floor(r[a]);
+        f.b[a] = (int)fl;
+        const double t = r[a] - (double)f.b[a];
+        f.w1[a] = (R)t;
+        f.w0[a] = (R)(1.0 - t);
+    }
+    return f;
 }
 
 // One of the 8 trilinear nodes (dz outer, dy, dx inner), :64-89.
 template <typename R>
-__device__ __forceinline__ bool corner(const GridView<R> &g, int bx, int by, int bz, R fx, R fy, R fz,
-                                       int n, int64_t &idx, R &w, R &gx, R &gy, R &gz) {
+__device__ __forceinline__ bool corner(const GridView<R> &g, const Frac<R> &f, int n, int64_t &idx, R &w, R &gx,
+                                       R &gy, R &gz) {
     int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
-    int i = bx + dx, j = by + dy, k = bz + dz;
+    int i = f.b[0] + dx, j = f.b[1] + dy, k = f.b[2] + dz;
     if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return false;
-    R wx = dx ? fx : R(1) - fx;
-    R wy = dy ? fy : R(1) - fy;
-    R wz = dz ? fz : R(1) - fz;
+    R wx = dx ? f.w1[0] : f.w0[0];
+    R wy = dy ? f.w1[1] : f.w0[1];
+    R wz = dz ? f.w1[2] : f.w0[2];
     R dwx = dx ? g.inv_h : -g.inv_h;
     R dwy = dy ? g.inv_h : -g.inv_h;
     R dwz = dz ? g.inv_h : -g.inv_h;
@@ -133,38 +154,38 @@ __device__ __forceinline__ bool corner(const GridView<R> &g, int bx, int by, int
 
 // --------------------------------------------------------------- tools ----
 
-template <typename R> __device__ __forceinline__ void tool_frame(const Tool<R> &t, R x, R y, R z, R q[3]) {
-    R dx = x - t.cx, dy = y - t.cy;
+__device__ __forceinline__ void tool_frame(const Tool &t, double x, double y, double z, double q[3]) {
+    double dx = x - t.cx, dy = y - t.cy;
     q[0] = dx * t.cos_a + dy * t.sin_a;
     q[1] = -dx * t.sin_a + dy * t.cos_a;
     q[2] = z - t.cz;
 }
 
-template <typename R> __device__ __forceinline__ R phi_local(const Tool<R> &t, const R q[3]) {
+__device__ __forceinline__ double phi_local(const Tool &t, const double q[3]) {
     if (t.kind == kToolWedge) {
-        R pr = t.cos_rake * q[0] + t.sin_rake * q[2];
-        R pf = -t.sin_clear * q[0] - t.cos_clear * q[2];
-        return rmax(pr, pf);
+        double pr = t.cos_rake * q[0] + t.sin_rake * q[2];
+        double pf = -t.sin_clear * q[0] - t.cos_clear * q[2];
+        return fmax(pr, pf);
     }
     // two-flute helical drill
-    R r = sqrt(q[0] * q[0] + q[1] * q[1]);
-    R tt = r * t.sin_half_point + q[2] * t.cos_half_point;
-    R cone = (tt < R(0)) ? sqrt(r * r + q[2] * q[2]) : (r * t.cos_half_point - q[2] * t.sin_half_point);
-    R body = rmax(rmax(cone, r - t.radius), q[2] - t.flute_length);
-    R psi = q[2] * t.tan_helix / t.radius;
-    R gmin = R(1e30);
+    double r = sqrt(q[0] * q[0] + q[1] * q[1]);
+    double tt = r * t.sin_half_point + q[2] * t.cos_half_point;
+    double cone = (tt < 0.0) ? sqrt(r * r + q[2] * q[2]) : (r * t.cos_half_point - q[2] * t.sin_half_point);
+    double body = fmax(fmax(cone, r - t.radius), q[2] - t.flute_length);
+    double psi = q[2] * t.tan_helix / t.radius;
+    double gmin = 1e300;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        R ang = psi + R(k) * R(3.14159265358979323846);
-        R fcx = t.flute_offset * cos(ang), fcy = t.flute_offset * sin(ang);
-        R ex = q[0] - fcx, ey = q[1] - fcy;
-        gmin = rmin(gmin, sqrt(ex * ex + ey * ey) - t.flute_radius);
+        double ang = psi + k * 3.14159265358979323846;
+        double fcx = t.flute_offset * cos(ang), fcy = t.flute_offset * sin(ang);
+        double ex = q[0] - fcx, ey = q[1] - fcy;
+        gmin = fmin(gmin, sqrt(ex * ex + ey * ey) - t.flute_radius);
     }
-    return rmax(body, -gmin);
+    return fmax(body, -gmin);
 }
 
-template <typename R> __device__ __forceinline__ R tool_phi(const Tool<R> &t, R x, R y, R z) {
-    R q[3];
+__device__ __forceinline__ double tool_phi(const Tool &t, double x, double y, double z) {
+    double q[3];
     tool_frame(t, x, y, z, q);
     return phi_local(t, q);
 }
@@ -173,7 +194,8 @@ template <typename R> __device__ __forceinline__ R tool_phi(const Tool<R> &t, R 
 
 // Register copy of one particle's mutable state.
 template <typename R> struct PState {
-    R x, y, z, vx, vy, vz;
+    double x, y, z;
+    R vx, vy, vz;
     R s[6], a[6];
     R eps, rate, dmg, sres, T;
     int32_t status;
@@ -242,7 +264,7 @@ template <typename R> __device__ __forceinline__ void chip_transition(const Mate
 
 // chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725.
 template <typename R>
-__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, int32_t id, R zcut) {
+__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, int32_t id, double zcut) {
     if (p.status == kActive) {
         if (p.z < zcut && p.eps > R(1e-6)) {
             p.status = kChip;

@@ -119,19 +119,17 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 
 // Scatter of one particle, mpmP2GKernel :110-133, with device-wide atomics.
 template <typename R>
-__device__ __forceinline__ void p2g_particle(const GridView<R> &g, R *acc, R x, R y, R z, R vx, R vy, R vz,
-                                             R mass, R vol, R T, const R s[6], R fex, R fey, R fez,
+__device__ __forceinline__ void p2g_particle(const GridView<R> &g, R *acc, double x, double y, double z, R vx,
+                                             R vy, R vz, R mass, R vol, R T, const R s[6], R fex, R fey, R fez,
                                              int32_t status) {
-    int bx, by, bz;
-    R fx, fy, fz;
-    base_and_frac(g, x, y, z, bx, by, bz, fx, fy, fz);
+    const Frac<R> f = base_and_frac(g, x, y, z);
     const bool stress_force = (status == kActive || status == kFixed);
     const bool ext_only = (status == kChip);
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         int64_t idx;
         R w, gx, gy, gz;
-        if (!corner(g, bx, by, bz, fx, fy, fz, n, idx, w, gx, gy, gz)) continue;
+        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
         R m = mass * w;
         unsafeAtomicAdd(acc + idx, m);
         unsafeAtomicAdd(acc + g.nn + idx, vx * m);
@@ -172,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, GridView<R> g, R
 // the accumulators; writes node velocity, mass and T*m.  When `direct_T` is
 // set (no diffusion pass follows) it also writes the gather temperature.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, Material<R> M, Tool<R> tool,
+__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, Material<R> M, Tool tool,
                                                         R dt, int direct_T, double *force_part) {
     const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double cf = 0.0;
@@ -199,26 +197,27 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
                 vx += (fxn / mass) * dt;
                 vy += (fyn / mass) * dt;
                 vz += (fzn / mass) * dt;
-                const R px = g.minx + (R)i * g.h, py = g.miny + (R)j * g.h, pz = g.minz + (R)k * g.h;
+                // node position nodePosition (:37-41), in FP64
+                const double px = g.ox + i * g.hd, py = g.oy + j * g.hd, pz = g.oz + k * g.hd;
                 if (tool.kind == kToolPoints) {
                     if (px >= tool.lo[0] && px <= tool.hi[0] && py >= tool.lo[1] && py <= tool.hi[1] &&
                         pz >= tool.lo[2] && pz <= tool.hi[2]) {
-                        R r2 = tool.rc2;
-                        R bdx = 0, bdy = 0, bdz = 0;
+                        double r2 = tool.rc2;
+                        double bdx = 0, bdy = 0, bdz = 0;
                         bool hit = false;
                         for (int s = 0; s < tool.nsamples; ++s) {
-                            R dx = px - tool.sx[s], dy = py - tool.sy[s], dz = pz - tool.sz[s];
-                            R d2 = dx * dx + dy * dy + dz * dz;
-                            if (d2 < r2 && d2 > R(1e-20)) {
+                            double dx = px - tool.sx[s], dy = py - tool.sy[s], dz = pz - tool.sz[s];
+                            double d2 = dx * dx + dy * dy + dz * dz;
+                            if (d2 < r2 && d2 > 1e-20) {
                                 r2 = d2;
                                 bdx = dx; bdy = dy; bdz = dz;
                                 hit = true;
                             }
                         }
                         if (hit) {
-                            R len = sqrt(bdx * bdx + bdy * bdy + bdz * bdz);
+                            double len = sqrt(bdx * bdx + bdy * bdy + bdz * bdz);
                             R nx = R(0), ny = R(0), nz = R(0);
-                            if (len > R(1e-12)) { nx = bdx / len; ny = bdy / len; nz = bdz / len; }
+                            if (len > 1e-12) { nx = (R)(bdx / len); ny = (R)(bdy / len); nz = (R)(bdz / len); }
                             R vn = vx * nx + vy * ny + vz * nz;
                             if (vn < R(0)) {
                                 cf += (double)fabs(mass * vn / rmax(dt, R(1e-15)));
@@ -231,18 +230,18 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
                         }
                     }
                 } else if (tool.kind == kToolWedge || tool.kind == kToolDrill) {
-                    R phi = tool_phi(tool, px, py, pz);
+                    const double phi = tool_phi(tool, px, py, pz);
                     if (phi < tool.band) {
-                        const R e = tool.eps;
-                        R gx = (tool_phi(tool, px + e, py, pz) - tool_phi(tool, px - e, py, pz)) / (R(2) * e);
-                        R gy = (tool_phi(tool, px, py + e, pz) - tool_phi(tool, px, py - e, pz)) / (R(2) * e);
-                        R gz = (tool_phi(tool, px, py, pz + e) - tool_phi(tool, px, py, pz - e)) / (R(2) * e);
-                        R len = sqrt(gx * gx + gy * gy + gz * gz);
-                        if (len > R(1e-12)) {
-                            R nx = gx / len, ny = gy / len, nz = gz / len;
-                            R tvx = tool.vx - tool.omega * (py - tool.cy);
-                            R tvy = tool.vy + tool.omega * (px - tool.cx);
-                            R tvz = tool.vz;
+                        const double e = tool.eps;
+                        double gx = (tool_phi(tool, px + e, py, pz) - tool_phi(tool, px - e, py, pz)) / (2.0 * e);
+                        double gy = (tool_phi(tool, px, py + e, pz) - tool_phi(tool, px, py - e, pz)) / (2.0 * e);
+                        double gz = (tool_phi(tool, px, py, pz + e) - tool_phi(tool, px, py, pz - e)) / (2.0 * e);
+                        double len = sqrt(gx * gx + gy * gy + gz * gz);
+                        if (len > 1e-12) {
+                            R nx = (R)(gx / len), ny = (R)(gy / len), nz = (R)(gz / len);
+                            R tvx = (R)(tool.vx - tool.omega * (py - tool.cy));
+                            R tvy = (R)(tool.vy + tool.omega * (px - tool.cx));
+                            R tvz = (R)tool.vz;
                             R rx = vx - tvx, ry = vy - tvy, rz = vz - tvz;
                             R vn = rx * nx + ry * ny + rz * nz;
                             if (vn < R(0)) {
@@ -251,7 +250,8 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
                                 R ut2 = ux * ux + uy * uy + uz * uz;
                                 R ut = sqrt(ut2);
                                 R sc = R(1);
-                                if (tool.mu > R(0) && ut > R(0)) sc = rmax(R(0), R(1) + tool.mu * vn / ut);
+                                const R mu = (R)tool.mu;
+                                if (mu > R(0) && ut > R(0)) sc = rmax(R(0), R(1) + mu * vn / ut);
                                 vx = tvx + sc * ux;
                                 vy = tvy + sc * uy;
                                 vz = tvz + sc * uz;
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R 
     R tm = G.gtm[idx];
     if (mass >= R(1e-15)) {
         const R T = tm / mass;
-        const R inv_h2 = R(1) / (g.h * g.h);
+        const R inv_h2 = (R)(1.0 / (g.hd * g.hd));
         const int i = (int)(idx % g.nx);
         const int64_t t = idx / g.nx;
         const int j = (int)(t % g.ny), k = (int)(t / g.ny);
@@ -326,7 +326,8 @@ struct G2PFlags {
 // and, when flags.p2g, with the next step's mpmP2GKernel :98-134.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
-                                                R zcut, G2PFlags flags, double ref_temp, double *part) {
+                                                double dtd, double zcut, G2PFlags flags, double ref_temp,
+                                                double *part) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double stats[kRed];
     if (flags.stats) stats_identity(stats, ref_temp);
@@ -335,16 +336,14 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, G
         p.status = P.status[i];
         if (p.status != kInactive) {
             p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
-            int bx, by, bz;
-            R fx, fy, fz;
-            base_and_frac(g, p.x, p.y, p.z, bx, by, bz, fx, fy, fz);
+            const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
             R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
             R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int n = 0; n < 8; ++n) {
                 int64_t idx;
                 R w, gx, gy, gz;
-                if (!corner(g, bx, by, bz, fx, fy, fz, n, idx, w, gx, gy, gz)) continue;
+                if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
                 const vec4_t<R> nd = G.node[idx];
                 if (nd.w != nd.w) continue;  // mass <= 1e-18
                 v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
@@ -373,9 +372,9 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, G
                 p.vx = p.vy = p.vz = R(0);
             } else {
                 p.vx = v0; p.vy = v1; p.vz = v2;
-                p.x += p.vx * dt;
-                p.y += p.vy * dt;
-                p.z += p.vz * dt;
+                p.x += (double)p.vx * dtd;
+                p.y += (double)p.vy * dtd;
+                p.z += (double)p.vz * dtd;
                 if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
                 if (p.status == kActive) constitutive_active(M, p, L, dt, has_back);
                 chip_transition(M, p);
@@ -416,10 +415,10 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, G
 
 // applyMPMExternalForceKernel, MPMWorkpieceSolver.cu:402-414
 template <typename R>
-__global__ void k_external_force(Particles<R> P, R cx, R cy, R cz, R r2, R fx, R fy, R fz) {
+__global__ void k_external_force(Particles<R> P, double cx, double cy, double cz, double r2, R fx, R fy, R fz) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n || P.status[i] != kActive) return;
-    R dx = P.x[i] - cx, dy = P.y[i] - cy, dz = P.z[i] - cz;
+    double dx = P.x[i] - cx, dy = P.y[i] - cy, dz = P.z[i] - cz;
     if (dx * dx + dy * dy + dz * dz <= r2) {
         P.fx[i] += fx;
         P.fy[i] += fy;

@@ -153,8 +153,12 @@ void dfree(mpmw_solver *s, void *&p, size_t bytes) {
     p = nullptr;
 }
 
+// Positions are FP64 in every precision mode; ids and status are int32.
+inline bool is_pos(int fld) { return fld == F_X || fld == F_Y || fld == F_Z; }
+
 size_t field_bytes(const mpmw_solver *s, int fld, int64_t count) {
-    return (size_t)count * (fld >= kNumReal ? sizeof(int32_t) : s->rsz);
+    if (fld >= kNumReal) return (size_t)count * sizeof(int32_t);
+    return (size_t)count * (is_pos(fld) ? sizeof(double) : s->rsz);
 }
 
 bool field_live(const mpmw_solver *s, int fld) {
@@ -237,7 +241,7 @@ template <typename T> int grow(mpmw_solver *s, T *&p, int64_t &cap, int64_t need
 template <typename R> Particles<R> pview(mpmw_solver *s) {
     Particles<R> P;
     auto F = [&](int k) { return (R *)s->f[k]; };
-    P.x = F(F_X); P.y = F(F_Y); P.z = F(F_Z);
+    P.x = (double *)s->f[F_X]; P.y = (double *)s->f[F_Y]; P.z = (double *)s->f[F_Z];
     P.vx = F(F_VX); P.vy = F(F_VY); P.vz = F(F_VZ);
     P.s0 = F(F_S0); P.s1 = F(F_S1); P.s2 = F(F_S2); P.s3 = F(F_S3); P.s4 = F(F_S4); P.s5 = F(F_S5);
     P.a0 = s->has_back ? F(F_A0) : nullptr; P.a1 = F(F_A1); P.a2 = F(F_A2);
@@ -253,8 +257,8 @@ template <typename R> Particles<R> pview(mpmw_solver *s) {
 
 template <typename R> GridView<R> gview(const mpmw_solver *s) {
     GridView<R> g;
-    g.minx = (R)s->minx; g.miny = (R)s->miny; g.minz = (R)s->minz;
-    g.h = (R)s->h;
+    g.ox = s->minx; g.oy = s->miny; g.oz = s->minz;
+    g.hd = s->h;
     g.inv_h = (R)(1.0 / s->h);
     g.nx = s->nx; g.ny = s->ny; g.nz = s->nz;
     g.kfix = s->kfix;
@@ -289,28 +293,28 @@ template <typename R> Material<R> material(const mpmw_solver *s) {
     return M;
 }
 
-template <typename R> Tool<R> tool_view(const mpmw_solver *s) {
-    Tool<R> t{};
+Tool tool_view(const mpmw_solver *s) {
+    Tool t{};
     t.kind = s->tool_kind;
     if (t.kind == MPMW_TOOL_POINTS) {
         t.nsamples = s->nsamples;
-        t.sx = (const R *)s->samples;
+        t.sx = (const double *)s->samples;
         t.sy = t.sx + s->nsamples;
         t.sz = t.sy + s->nsamples;
-        for (int a = 0; a < 3; ++a) { t.lo[a] = (R)s->sample_lo[a]; t.hi[a] = (R)s->sample_hi[a]; }
-        t.rc2 = (R)(s->prm.tool_contact_radius * s->prm.tool_contact_radius);
+        for (int a = 0; a < 3; ++a) { t.lo[a] = s->sample_lo[a]; t.hi[a] = s->sample_hi[a]; }
+        t.rc2 = s->prm.tool_contact_radius * s->prm.tool_contact_radius;
     } else if (t.kind == MPMW_TOOL_WEDGE || t.kind == MPMW_TOOL_DRILL) {
         const mpmw_tool_desc &d = s->desc;
         const mpmw_tool_pose &p = s->pose;
-        t.cx = (R)p.center[0]; t.cy = (R)p.center[1]; t.cz = (R)p.center[2];
-        t.cos_a = (R)std::cos(p.angle); t.sin_a = (R)std::sin(p.angle);
-        t.vx = (R)p.vel[0]; t.vy = (R)p.vel[1]; t.vz = (R)p.vel[2]; t.omega = (R)p.omega;
-        t.band = (R)d.band; t.mu = (R)d.friction; t.eps = (R)d.grad_eps;
-        t.sin_rake = (R)std::sin(d.rake_angle); t.cos_rake = (R)std::cos(d.rake_angle);
-        t.sin_clear = (R)std::sin(d.clearance_angle); t.cos_clear = (R)std::cos(d.clearance_angle);
-        t.radius = (R)d.radius; t.tan_helix = (R)std::tan(d.helix_angle);
-        t.sin_half_point = (R)std::sin(0.5 * d.point_angle); t.cos_half_point = (R)std::cos(0.5 * d.point_angle);
-        t.flute_offset = (R)d.flute_offset; t.flute_radius = (R)d.flute_radius; t.flute_length = (R)d.flute_length;
+        t.cx = p.center[0]; t.cy = p.center[1]; t.cz = p.center[2];
+        t.cos_a = std::cos(p.angle); t.sin_a = std::sin(p.angle);
+        t.vx = p.vel[0]; t.vy = p.vel[1]; t.vz = p.vel[2]; t.omega = p.omega;
+        t.band = d.band; t.mu = d.friction; t.eps = d.grad_eps;
+        t.sin_rake = std::sin(d.rake_angle); t.cos_rake = std::cos(d.rake_angle);
+        t.sin_clear = std::sin(d.clearance_angle); t.cos_clear = std::cos(d.clearance_angle);
+        t.radius = d.radius; t.tan_helix = std::tan(d.helix_angle);
+        t.sin_half_point = std::sin(0.5 * d.point_angle); t.cos_half_point = std::cos(0.5 * d.point_angle);
+        t.flute_offset = d.flute_offset; t.flute_radius = d.flute_radius; t.flute_length = d.flute_length;
     }
     return t;
 }
@@ -506,7 +510,7 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     {
         Timed t(s, MPMW_K_GRID);
         hipLaunchKernelGGL(k_grid_update<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, M,
-                           tool_view<R>(s), (R)dt, diffusion ? 0 : 1, s->fpart);
+                           tool_view(s), (R)dt, diffusion ? 0 : 1, s->fpart);
         HIP_TRY(hipGetLastError());
     }
     {
@@ -534,7 +538,7 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     if (fl.stats && (rc = ensure_part(s, nb))) return rc;
     {
         Timed t(s, MPMW_K_G2P_P2G);
-        hipLaunchKernelGGL(k_g2p<R>, dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, (R)zcut, fl,
+        hipLaunchKernelGGL(k_g2p<R>, dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, dt, zcut, fl,
                            s->prm.ref_temp, s->part);
         HIP_TRY(hipGetLastError());
     }
@@ -591,6 +595,9 @@ template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm,
             if (k >= kNumReal)
                 hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
                                    (int32_t *)s->alt[k], (const int32_t *)s->f[k], dperm, count);
+            else if (is_pos(k))
+                hipLaunchKernelGGL(k_gather<double>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
+                                   (double *)s->alt[k], (const double *)s->f[k], dperm, count);
             else
                 hipLaunchKernelGGL(k_gather<R>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream, (R *)s->alt[k],
                                    (const R *)s->f[k], dperm, count);
@@ -683,7 +690,9 @@ template <typename R> int set_particles_impl(mpmw_solver *s, const mpmw_particle
     };
     for (auto &e : map) {
         if (!field_live(s, e.fld)) continue;
-        if ((rc = upload_field<R>(s, e.fld, e.src, n, e.comp, e.c, e.fill))) return rc;
+        rc = is_pos(e.fld) ? upload_field<double>(s, e.fld, e.src, n, e.comp, e.c, e.fill)
+                           : upload_field<R>(s, e.fld, e.src, n, e.comp, e.c, e.fill);
+        if (rc) return rc;
     }
     HIP_TRY(hipMemcpy(s->f[F_STATUS], a->status, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->f[F_ID], a->id, sizeof(int32_t) * n, hipMemcpyHostToDevice));
@@ -726,7 +735,9 @@ template <typename R> int get_particles_impl(mpmw_solver *s, mpmw_particle_array
             for (int64_t i = 0; i < n; ++i) e.dst[i * e.comp + e.c] = 0.0;
             continue;
         }
-        if ((rc = download_field<R>(s, e.fld, e.dst, n, e.comp, e.c))) return rc;
+        rc = is_pos(e.fld) ? download_field<double>(s, e.fld, e.dst, n, e.comp, e.c)
+                           : download_field<R>(s, e.fld, e.dst, n, e.comp, e.c);
+        if (rc) return rc;
     }
     if (n > 0 && a->status) HIP_TRY(hipMemcpy(a->status, s->f[F_STATUS], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     if (n > 0 && a->id) HIP_TRY(hipMemcpy(a->id, s->f[F_ID], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
@@ -1060,7 +1071,7 @@ int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double c
     s->nsamples = 0;
     s->tool_kind = MPMW_TOOL_NONE;
     if (m == 0) return MPMW_OK;
-    int rc = dalloc(s, &s->samples, 3 * s->rsz * (size_t)m);
+    int rc = dalloc(s, &s->samples, 3 * sizeof(double) * (size_t)m);
     if (rc) return rc;
     const double rcr = s->prm.tool_contact_radius;
     for (int a = 0; a < 3; ++a) { s->sample_lo[a] = 1e300; s->sample_hi[a] = -1e300; }
@@ -1069,21 +1080,16 @@ int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double c
             s->sample_lo[a] = std::min(s->sample_lo[a], pts[3 * i + a]);
             s->sample_hi[a] = std::max(s->sample_hi[a], pts[3 * i + a]);
         }
-    for (int a = 0; a < 3; ++a) {  // grown by r_c plus a margin for FP32 node positions
-        const double marg = rcr * 1.001 + 1e-6 * (std::fabs(s->sample_lo[a]) + std::fabs(s->sample_hi[a])) + s->h * 1e-3;
+    for (int a = 0; a < 3; ++a) {  // grown by r_c (plus a margin) for the node culling test
+        const double marg = rcr * 1.001 + s->h * 1e-3;
         s->sample_lo[a] -= marg;
         s->sample_hi[a] += marg;
     }
-    if (s->fp64) {
+    {
         std::vector<double> soa(3 * (size_t)m);
         for (int32_t i = 0; i < m; ++i)
             for (int a = 0; a < 3; ++a) soa[(size_t)a * m + i] = pts[3 * i + a];
         HIP_TRY(hipMemcpy(s->samples, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> soa(3 * (size_t)m);
-        for (int32_t i = 0; i < m; ++i)
-            for (int a = 0; a < 3; ++a) soa[(size_t)a * m + i] = (float)pts[3 * i + a];
-        HIP_TRY(hipMemcpy(s->samples, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice));
     }
     s->nsamples = m;
     s->tool_kind = MPMW_TOOL_POINTS;
@@ -1140,8 +1146,8 @@ int mpmw_apply_external_force(mpmw_solver *s, const double c[3], double radius, 
                            pview<double>(s), c[0], c[1], c[2], radius * radius, f[0], f[1], f[2]);
     else
         hipLaunchKernelGGL(k_external_force<float>, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->stream,
-                           pview<float>(s), (float)c[0], (float)c[1], (float)c[2], (float)(radius * radius),
-                           (float)f[0], (float)f[1], (float)f[2]);
+                           pview<float>(s), c[0], c[1], c[2], radius * radius, (float)f[0], (float)f[1],
+                           (float)f[2]);
     HIP_TRY(hipGetLastError());
     s->p2g_ready = false;
     return MPMW_OK;
