@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X MPM workpiece step (one GPU).
+
+Workload (one "step" = one MPMWorkpieceSolver::step over the whole
+workpiece): 256x128x64-cell Ti-6Al-4V slab, 8 particles per cell with seeded
+jitter (16,777,216 particles), analytic two-flute helical drill (D=10 mm,
+30 deg helix, 118 deg point) at 3000 rpm and 0.1 mm/rev, dt = 1e-8 s, FP32
+bulk arithmetic.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: particle-steps/s, the dominant kernel's HBM roofline
+fraction (HIP events on the solver's stream), and the CPU oracle on the
+host cores as a reported baseline.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "edegpredict-engine-rotary_amd")
+sys.path.insert(0, PKG)
+
+import mpmw  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle-step of the plain FP32 step (H_kin = 0, no
+# external force), G2P of step n fused with P2G of step n+1:
+#   particles: read 64 B (x 12, sigma 24, eps_p, D, sigma_res, T, status, m, V)
+#              write 72 B (x 12, v 12, sigma 24, eps_p, rate, D, sigma_res, T, status)
+#   grid: 96 B per occupied node (P2G flush 32, update read 32 + write 16, G2P read 16)
+B_PARTICLE = 136.0
+B_NODE_STEP = 96.0
+# of which the fused G2P+P2G kernel moves: all particle bytes, the G2P node
+# read (16 B) and the P2G flush (32 B)
+B_NODE_G2P = 48.0
+
+
+class HipEvents:
+    """hipEvent pairs on an arbitrary stream, through libamdhip64."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def occupied_nodes(cells):
+    cx, cy, cz = cells
+    return (cx + 1) * (cy + 1) * (cz + 1)
+
+
+def cpu_baseline(cfg, budget_s, threads):
+    """The CPU FP64 oracle (restatement of the reference step) on the host
+    cores, over a bounded number of steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    import parity
+    oracle.set_threads(threads)
+    t0 = time.time()
+    o, _ = parity.build_oracle(cfg)
+    setup = time.time() - t0
+    n = o.count()
+    t0 = time.time()
+    o.step(cfg["dt"])                      # warm-up step (includes the first grid rebuild)
+    first = time.time() - t0
+    steps = max(1, min(50, int(budget_s / max(first, 1e-3))))
+    t0 = time.time()
+    for _ in range(steps):
+        o.step(cfg["dt"])
+    el = time.time() - t0
+    return {
+        "value": n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"{cfg['name']}: {n} particles, 1 warm-up + {steps} timed steps of the FP64 oracle "
+                  f"({el:.1f} s; setup {setup:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=25)
+    ap.add_argument("--mode", choices=["atomic", "deterministic"], default="atomic")
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--cells", type=str, default="256,128,64")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    cells = tuple(int(x) for x in args.cells.split(","))
+    cfg = workloads.drill_config1()
+    if cells != cfg["cells"]:
+        cfg = workloads.scaled(cfg, cells)
+    P = dict(cfg["params"], precision=mpmw.FP64 if args.precision == "fp64" else mpmw.FP32,
+             deterministic=1 if args.mode == "deterministic" else 0)
+    s = mpmw.Solver(**P)
+    t0 = time.time()
+    workloads.apply_to_solver(mpmw, s, cfg)
+    setup_s = time.time() - t0
+    n = s.count()
+    dt = cfg["dt"]
+    for _ in range(args.warmup):
+        s.step(dt)
+    s.synchronize()
+
+    ev = HipEvents()
+    stream = s.stream()
+    a, b = ev.create(), ev.create()
+    s.set_kernel_timing(True)
+    s.synchronize()
+    t_wall = time.perf_counter()
+    ev.record(a, stream)
+    for _ in range(args.steps):
+        s.step(dt)
+    ev.record(b, stream)
+    s.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_total = ev.elapsed_ms(a, b)
+    kms, kn = s.kernel_times()
+    s.set_kernel_timing(False)
+    diag = s.diagnostics()
+
+    ms_step = ms_total / args.steps
+    pps = n * args.steps / (ms_total / 1e3)
+    nodes = occupied_nodes(cells)
+    g = s.grid_info()
+
+    # dominant kernel: fused G2P + constitutive + chip + next-step P2G
+    k = mpmw.K_G2P_P2G
+    k_avg_ms = kms[k] / max(kn[k], 1)
+    bytes_launch = B_PARTICLE * n + B_NODE_G2P * nodes
+    achieved = bytes_launch / (k_avg_ms / 1e3) / 1e9
+    b_alg = B_PARTICLE + B_NODE_STEP * nodes / n
+    names = ["g2p_p2g", "p2g", "grid_update", "diffuse", "sort", "reduce", "other"]
+    kernels = {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]}
+
+    out = {
+        "metric": "MPM particle-steps/s at 1 GPU",
+        "value": pps,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision + " (positions fp64)",
+        "data": "synthetic",
+        "config": {
+            "workload": cfg["name"], "particles": n, "cells": list(cells),
+            "grid_nodes": g.nx * g.ny * g.nz, "occupied_nodes": nodes, "tool": "analytic two-flute drill",
+            "dt": dt, "mode": args.mode, "precision": args.precision,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_g2p (fused G2P + J-C return + chip pass + next-step P2G)",
+            "algorithmic_bytes_per_launch": bytes_launch,
+            "avg_launch_ms": k_avg_ms,
+            "step_level": {"B_alg_per_particle_step": b_alg,
+                           "achieved_GBs": pps * b_alg / 1e9,
+                           "frac": pps * b_alg / 1e9 / HBM_PEAK_GBS},
+        },
+        "kernels": kernels,
+        "binning": diag,
+        "wall_s": wall,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, threads)
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -2,11 +2,11 @@
 //
 // Kernel map (reference kernel -> here):
 //   clearMPMGridKernel          -> the grid update zeroes the accumulators it has read
-//   mpmP2GKernel                -> k_p2g (standalone) and the P2G tail of k_g2p
+//   mpmP2GKernel                -> k_tile<.., P2G> (mpmw_tile.hpp)
 //   mpmGridUpdateKernel         -> k_grid_update
 //   mpmDiffuseTemperatureKernel -> k_diffuse (race-free Jacobi form)
-//   mpmG2PKernel + chipFractureAndSeparationKernel -> k_g2p (one particle pass)
-//   updateMetrics / rebuild AABB host loops -> block partials in k_g2p / k_reduce
+//   mpmG2PKernel + chipFractureAndSeparationKernel -> k_tile<.., G2P, ..>
+//   updateMetrics / rebuild AABB host loops -> block partials in k_tile / k_reduce
 #pragma once
 
 #include "mpmw_device.hpp"
@@ -113,54 +113,6 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
         v[8] += 0.5 * (double)P.m[i] * (vx * vx + vy * vy + vz * vz);
     }
     block_reduce_stats(v, part);
-}
-
-// ------------------------------------------------------------------ P2G ---
-
-// Scatter of one particle, mpmP2GKernel :110-133, with device-wide atomics.
-template <typename R>
-__device__ __forceinline__ void p2g_particle(const GridView<R> &g, R *acc, double x, double y, double z, R vx,
-                                             R vy, R vz, R mass, R vol, R T, const R s[6], R fex, R fey, R fez,
-                                             int32_t status) {
-    const Frac<R> f = base_and_frac(g, x, y, z);
-    const bool stress_force = (status == kActive || status == kFixed);
-    const bool ext_only = (status == kChip);
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        int64_t idx;
-        R w, gx, gy, gz;
-        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
-        R m = mass * w;
-        unsafeAtomicAdd(acc + idx, m);
-        unsafeAtomicAdd(acc + g.nn + idx, vx * m);
-        unsafeAtomicAdd(acc + 2 * g.nn + idx, vy * m);
-        unsafeAtomicAdd(acc + 3 * g.nn + idx, vz * m);
-        unsafeAtomicAdd(acc + 7 * g.nn + idx, T * m);
-        if (stress_force) {
-            R fX = -(s[0] * gx + s[3] * gy + s[4] * gz) * vol;
-            R fY = -(s[3] * gx + s[1] * gy + s[5] * gz) * vol;
-            R fZ = -(s[4] * gx + s[5] * gy + s[2] * gz) * vol;
-            unsafeAtomicAdd(acc + 4 * g.nn + idx, fX + fex * w);
-            unsafeAtomicAdd(acc + 5 * g.nn + idx, fY + fey * w);
-            unsafeAtomicAdd(acc + 6 * g.nn + idx, fZ + fez * w);
-        } else if (ext_only) {
-            unsafeAtomicAdd(acc + 4 * g.nn + idx, fex * w);
-            unsafeAtomicAdd(acc + 5 * g.nn + idx, fey * w);
-            unsafeAtomicAdd(acc + 6 * g.nn + idx, fez * w);
-        }
-    }
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, GridView<R> g, R *acc) {
-    int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    int32_t st = P.status[i];
-    if (st == kInactive) return;
-    R s[6] = {P.s0[i], P.s1[i], P.s2[i], P.s3[i], P.s4[i], P.s5[i]};
-    R fex = P.fx ? P.fx[i] : R(0), fey = P.fx ? P.fy[i] : R(0), fez = P.fx ? P.fz[i] : R(0);
-    p2g_particle(g, acc, P.x[i], P.y[i], P.z[i], P.vx[i], P.vy[i], P.vz[i], P.m[i], P.V[i], P.T[i], s, fex,
-                 fey, fez, st);
 }
 
 // ---------------------------------------------------------- grid update ---
@@ -312,103 +264,6 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R 
         tm = rmax(T_new, R(0)) * mass;
     }
     G.node[idx].w = tm / mass;
-}
-
-// ------------------------------------------------------------------ G2P ---
-
-struct G2PFlags {
-    int chip;      // run the chip separation / fracture pass (even step)
-    int p2g;       // scatter into the accumulators for the next step
-    int stats;     // write AABB + metrics block partials
-};
-
-// mpmG2PKernel :264-400 fused with chipFractureAndSeparationKernel :667-725
-// and, when flags.p2g, with the next step's mpmP2GKernel :98-134.
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
-                                                double dtd, double zcut, G2PFlags flags, double ref_temp,
-                                                double *part) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double stats[kRed];
-    if (flags.stats) stats_identity(stats, ref_temp);
-    if (i < P.n) {
-        PState<R> p;
-        p.status = P.status[i];
-        if (p.status != kInactive) {
-            p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
-            const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
-            R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
-            R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int n = 0; n < 8; ++n) {
-                int64_t idx;
-                R w, gx, gy, gz;
-                if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
-                const vec4_t<R> nd = G.node[idx];
-                if (nd.w != nd.w) continue;  // mass <= 1e-18
-                v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
-                temp += nd.w * w;
-                wsum += w;
-                L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
-                L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
-                L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
-            }
-            const bool has_back = P.a0 != nullptr;
-            p.T = P.T[i];
-            p.eps = P.eps[i];
-            p.dmg = P.dmg[i];
-            p.sres = P.sres[i];
-            p.rate = P.rate[i];
-            p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
-            p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
-            if (has_back) {
-                p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
-                p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) p.a[c] = R(0);
-            }
-            if (p.status == kFixed) {
-                p.vx = p.vy = p.vz = R(0);
-            } else {
-                p.vx = v0; p.vy = v1; p.vz = v2;
-                p.x += (double)p.vx * dtd;
-                p.y += (double)p.vy * dtd;
-                p.z += (double)p.vz * dtd;
-                if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
-                if (p.status == kActive) constitutive_active(M, p, L, dt, has_back);
-                chip_transition(M, p);
-            }
-            if (flags.chip) chip_pass(M, p, P.id[i], zcut);
-            // write back
-            P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
-            P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
-            P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
-            P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
-            if (has_back) {
-                P.a0[i] = p.a[0]; P.a1[i] = p.a[1]; P.a2[i] = p.a[2];
-                P.a3[i] = p.a[3]; P.a4[i] = p.a[4]; P.a5[i] = p.a[5];
-            }
-            P.eps[i] = p.eps; P.rate[i] = p.rate; P.dmg[i] = p.dmg; P.sres[i] = p.sres; P.T[i] = p.T;
-            P.status[i] = p.status;
-            if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }
-            if (flags.stats) {
-                const double x = p.x, y = p.y, z = p.z;
-                stats[0] = x; stats[1] = y; stats[2] = z;
-                stats[3] = -x; stats[4] = -y; stats[5] = -z;
-                stats[6] = fmax(0.0, (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]));
-                stats[7] = fmax(ref_temp, (double)p.T);
-                const double vx = p.vx, vy = p.vy, vz = p.vz;
-                stats[8] = 0.5 * (double)P.m[i] * (vx * vx + vy * vy + vz * vz);
-            }
-            if (flags.p2g) {
-                // f_ext was zeroed above, so the next step's P2G sees none
-                p2g_particle(g, G.acc, p.x, p.y, p.z, p.vx, p.vy, p.vz, P.m[i], P.V[i], p.T, p.s, R(0), R(0),
-                             R(0), p.status);
-            }
-        }
-    }
-    if (flags.stats) block_reduce_stats(stats, part);
 }
 
 // ------------------------------------------------------------ small ops ---

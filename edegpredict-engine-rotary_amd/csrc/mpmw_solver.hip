@@ -7,8 +7,10 @@
 //     precision (FP32 bulk or FP64) and never round-trips to the host during
 //     stepping; the grid rebuild every 25 steps reads back only the AABB
 //     (the reference downloads and re-uploads every particle, :729-733);
-//   * G2P, the chip pass and the next step's P2G run as one particle pass
-//     (k_g2p), so each particle is read and written once per step;
+//   * particles are binned into cell tiles; G2P, the chip pass and the next
+//     step's P2G run as one particle pass per tile (k_tile) with the node
+//     window staged in LDS, so each particle is read and written once per
+//     step and the grid sees one atomic flush per tile;
 //   * all reductions (AABB, metrics, contact force) are fixed-order block
 //     partials, so their results do not depend on scheduling.
 #include <hip/hip_runtime.h>
@@ -20,9 +22,8 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mpmw.h"
-#include "mpmw_kernels.hpp"
-#include "mpmw_sort.hpp"
+#include "mpmw.h"
+#include "mpmw_tile.hpp"
 
 using namespace mpmw;
 
@@ -79,7 +80,7 @@ struct mpmw_solver {
     bool snap_on = false, snap_valid = false;
 
     // reductions
-    double *part = nullptr;        // k_g2p / k_reduce partials
+    double *part = nullptr;        // k_tile / k_reduce partials
     int64_t part_cap = 0;
     double *fpart = nullptr;       // contact-force partials
     int64_t fpart_cap = 0;
@@ -469,19 +470,120 @@ template <typename R> int zero_acc(mpmw_solver *s) {
     return MPMW_OK;
 }
 
-template <typename R> int sort_particles(mpmw_solver *) { return MPMW_OK; }
-void free_sort_state(mpmw_solver *) {}
+// ------------------------------------------------------------ binning -----
 
-// Recompute the AABB on the device, read it back and rebuild the grid.
+TileGeom tile_geom(const mpmw_solver *s) {
+    auto tiles = [](int n, int T) { return std::max(1, (n - 1 - kCellOffset + T - 1) / T); };
+    TileGeom t;
+    t.ntx = tiles(s->nx, TX);
+    t.nty = tiles(s->ny, TY);
+    t.ntz = tiles(s->nz, TZ);
+    t.ntiles = t.ntx * t.nty * t.ntz;
+    t.nkeys = t.ntiles * kTileCells;
+    return t;
+}
+
+template <typename T> int grow32(mpmw_solver *s, T *&p, int32_t &cap, int64_t need) {
+    int64_t c = cap;
+    int rc = grow(s, p, c, need);
+    cap = (int32_t)c;
+    return rc;
+}
+
+void free_sort_state(mpmw_solver *s) {
+    SortState &S = s->sort;
+    void *q;
+    q = S.keys; dfree(s, q, 0);
+    q = S.keys_sorted; dfree(s, q, 0);
+    q = S.idx; dfree(s, q, 0);
+    q = S.idx_sorted; dfree(s, q, 0);
+    q = S.perm; dfree(s, q, 0);
+    q = S.counts; dfree(s, q, 0);
+    q = S.cell_start; dfree(s, q, 0);
+    q = S.tile_start; dfree(s, q, 0);
+    q = S.oow; dfree(s, q, 0);
+    if (S.h_oow) (void)hipHostFree(S.h_oow);
+    free_scratch(S.temp);
+    free_scratch(S.temp2);
+    S = SortState();
+}
+
+// Bin the particles into cell tiles of the current grid: stable radix sort
+// by tile-major cell key, slot-major order inside each tile, then one gather
+// of every particle field.  Only the order changes, never a value.
+template <typename R> int sort_particles(mpmw_solver *s) {
+    SortState &S = s->sort;
+    const TileGeom tg = tile_geom(s);
+    if ((int64_t)tg.ntiles * kTileCells >= INT32_MAX) return fail(MPMW_E_RANGE, "too many tiles for the binning");
+    const int n = s->n;
+    int rc;
+    int32_t ncap = S.n_cap, ncap2 = S.n_cap, ncap3 = S.n_cap, ncap4 = S.n_cap, ncap5 = S.n_cap;
+    if ((rc = grow32(s, S.keys, ncap, n)) || (rc = grow32(s, S.keys_sorted, ncap2, n)) ||
+        (rc = grow32(s, S.idx, ncap3, n)) || (rc = grow32(s, S.idx_sorted, ncap4, n)) ||
+        (rc = grow32(s, S.perm, ncap5, n)))
+        return rc;
+    S.n_cap = std::min(std::min(ncap, ncap2), std::min(std::min(ncap3, ncap4), ncap5));
+    int32_t kc = S.key_cap, kc2 = S.key_cap;
+    if ((rc = grow32(s, S.counts, kc, tg.nkeys + 1)) || (rc = grow32(s, S.cell_start, kc2, tg.nkeys + 1))) return rc;
+    S.key_cap = std::min(kc, kc2);
+    if ((rc = grow32(s, S.tile_start, S.tile_cap, tg.ntiles + 1))) return rc;
+    if (!S.oow) {
+        void *q;
+        if ((rc = dalloc(s, &q, sizeof(int32_t)))) return rc;
+        S.oow = (int32_t *)q;
+        HIP_TRY(hipHostMalloc((void **)&S.h_oow, sizeof(int32_t), hipHostMallocDefault));
+        *S.h_oow = 0;
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)1, 0);
+    }
+    int nbits = 1;
+    while (((int64_t)1 << nbits) <= tg.nkeys) ++nbits;
+    {
+        Timed t(s, MPMW_K_SORT);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(std::min(blocks_for(tg.nkeys + 1), 4096)), dim3(kBlock), 0,
+                           s->stream, S.counts, (int64_t)tg.nkeys + 1, 0);
+        hipLaunchKernelGGL(k_bin_keys<R>, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, pview<R>(s),
+                           gview<R>(s), tg, S.keys, S.idx, S.counts);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(radix_sort_pairs(S.temp, S.keys, S.keys_sorted, S.idx, S.idx_sorted, n, nbits, s->stream));
+        HIP_TRY(exclusive_scan(S.temp2, S.counts, S.cell_start, tg.nkeys + 1, s->stream));
+        hipLaunchKernelGGL(k_slot_major, dim3(tg.ntiles), dim3(kBlock), 0, s->stream, S.cell_start, S.idx_sorted,
+                           S.perm, S.tile_start);
+        hipLaunchKernelGGL(k_tail, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, S.cell_start, tg.nkeys,
+                           tg.ntiles, S.idx_sorted, S.perm, S.tile_start, n);
+        HIP_TRY(hipGetLastError());
+    }
+    if ((rc = permute_particles<R>(s, S.perm, n, false))) return rc;
+    S.geom = tg;
+    S.valid = true;
+    S.origin[0] = s->minx; S.origin[1] = s->miny; S.origin[2] = s->minz;
+    S.sorts++;
+    return MPMW_OK;
+}
+
+// Recompute the AABB on the device, read it back, rebuild the grid and
+// re-bin the particles when the tiles no longer match it.
 template <typename R> int rebuild(mpmw_solver *s) {
     int rc;
     if (!s->stats_fresh) {
         if ((rc = reduce_particles<R>(s))) return rc;
         s->stats_fresh = true;
     }
+    SortState &S = s->sort;
+    if (S.oow) HIP_TRY(hipMemcpyAsync(S.h_oow, S.oow, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     if ((rc = rebuild_grid_from_stats(s))) return rc;
-    if ((rc = sort_particles<R>(s))) return rc;
+    bool need = !S.valid;
+    if (S.valid) {
+        const TileGeom tg = tile_geom(s);
+        need = tg.ntx != S.geom.ntx || tg.nty != S.geom.nty || tg.ntz != S.geom.ntz;
+        const double o[3] = {s->minx, s->miny, s->minz};
+        for (int a = 0; a < 3; ++a) need = need || std::fabs(o[a] - S.origin[a]) >= 0.5 * s->h;
+        // particles that left their tile window since the last binning
+        need = need || (double)*S.h_oow > 0.01 * (double)s->n * (double)std::max<int64_t>(1, S.steps_since);
+    }
+    if (S.oow) hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)1, 0);
+    S.steps_since = 0;
+    if (need && (rc = sort_particles<R>(s))) return rc;
     return MPMW_OK;
 }
 
@@ -489,20 +591,30 @@ template <typename R> int compact_impl(mpmw_solver *s);
 
 // ------------------------------------------------------------- the step ---
 
+// MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791.
 template <typename R> int step_impl(mpmw_solver *s, double dt) {
     if (s->n <= 0 || s->nn <= 0) return MPMW_OK;  // :728 silent return
     int rc;
     if (s->step % 25 == 0) {
         if ((rc = rebuild<R>(s))) return rc;
     }
+    if (!s->sort.valid && (rc = sort_particles<R>(s))) return rc;
     const GridView<R> g = gview<R>(s);
     const Grid<R> G = grid_arrays<R>(s);
     Particles<R> P = pview<R>(s);
     const Material<R> M = material<R>(s);
+    const TileGeom tg = s->sort.geom;
+    const int nb = tg.ntiles;
+    if ((rc = ensure_part(s, nb))) return rc;
+    const double zcut = s->tip[2] - s->h * 1.5;
+    G2PFlags fl;
+    fl.chip = (s->step % 2 == 0);
+    fl.stats = 0;
     if (!s->p2g_ready) {
         if ((rc = zero_acc<R>(s))) return rc;
         Timed t(s, MPMW_K_P2G);
-        hipLaunchKernelGGL(k_p2g<R>, dim3(blocks_for(s->n)), dim3(kBlock), 0, s->stream, P, g, G.acc);
+        hipLaunchKernelGGL((k_tile<R, false, true>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, dt,
+                           zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
         HIP_TRY(hipGetLastError());
     }
     const bool diffusion = s->prm.specific_heat > 0.0 &&
@@ -528,18 +640,18 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
                            (R)dt);
         HIP_TRY(hipGetLastError());
     }
+    // G2P + chip pass, fused with the next step's P2G unless the next step
+    // rebuilds the grid (then the AABB and metrics are reduced instead).
     const bool next_rebuild = ((s->step + 1) % 25 == 0);
-    G2PFlags fl;
-    fl.chip = (s->step % 2 == 0);
-    fl.p2g = next_rebuild ? 0 : 1;
     fl.stats = next_rebuild ? 1 : 0;
-    const double zcut = s->tip[2] - s->h * 1.5;
-    const int nb = blocks_for(s->n);
-    if (fl.stats && (rc = ensure_part(s, nb))) return rc;
     {
         Timed t(s, MPMW_K_G2P_P2G);
-        hipLaunchKernelGGL(k_g2p<R>, dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, dt, zcut, fl,
-                           s->prm.ref_temp, s->part);
+        if (next_rebuild)
+            hipLaunchKernelGGL((k_tile<R, true, false>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt,
+                               dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+        else
+            hipLaunchKernelGGL((k_tile<R, true, true>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt,
+                               dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
         HIP_TRY(hipGetLastError());
     }
     if (fl.stats) {
@@ -550,7 +662,8 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     } else {
         s->stats_fresh = false;
     }
-    s->p2g_ready = fl.p2g != 0;
+    s->p2g_ready = !next_rebuild;
+    s->sort.steps_since++;
     // kinematic tool
     for (int a = 0; a < 3; ++a) s->pose.center[a] += s->pose.vel[a] * dt;
     s->pose.angle += s->pose.omega * dt;
@@ -1375,3 +1488,26 @@ int mpmw_device_bytes(mpmw_solver *s, int64_t *bytes) {
 }
 
 }  // extern "C"
+
+extern "C" int mpmw_get_diagnostics(mpmw_solver *s, mpmw_diagnostics *d) {
+    CHECK_S(s);
+    if (!d) return fail(MPMW_E_INVAL, "null");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    std::memset(d, 0, sizeof(*d));
+    const SortState &S = s->sort;
+    d->sorts = S.sorts;
+    d->steps_since_rebuild = S.steps_since;
+    d->tiles = S.geom.ntiles;
+    d->tile_cells = kTileCells;
+    if (S.oow) {
+        int32_t v = 0;
+        HIP_TRY(hipMemcpy(&v, S.oow, sizeof(int32_t), hipMemcpyDeviceToHost));
+        d->out_of_window = v;
+    }
+    if (S.valid && S.tile_start && S.geom.ntiles > 0) {
+        std::vector<int32_t> ts((size_t)S.geom.ntiles + 1);
+        HIP_TRY(hipMemcpy(ts.data(), S.tile_start, sizeof(int32_t) * ts.size(), hipMemcpyDeviceToHost));
+        for (int t = 0; t < S.geom.ntiles; ++t) d->max_tile_particles = std::max(d->max_tile_particles, ts[t + 1] - ts[t]);
+    }
+    return MPMW_OK;
+}

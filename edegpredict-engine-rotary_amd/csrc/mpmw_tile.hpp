@@ -1,0 +1,512 @@
+// mpmw_tile.hpp -- the particle pass of the MPM step over cell tiles.
+//
+// One workgroup per tile of TX x TY x TZ cells; thread t owns cell t.  The
+// tile's node window (WX x WY x WZ nodes) is staged in LDS: the gather record
+// (v, T) for G2P and eight accumulator channels for P2G.  A thread keeps the
+// 8 gather nodes of its cell and the 8 x 8 P2G corner sums of its cell's
+// particles in registers; after the particle loop the sums are combined
+// across the wave with lane shuffles into node values and added to the LDS
+// window layer by layer, then the window is flushed once to HBM.  No LDS
+// float atomics are used on this path (they serialise on gfx950).  A
+// particle that has left its cell since the last binning is scattered with
+// atomics (LDS inside the window, device memory outside it).
+//
+// Reference kernels: mpmP2GKernel (MPMWorkpieceSolver.cu:98-134),
+// mpmG2PKernel (:264-400), chipFractureAndSeparationKernel (:667-725).
+#pragma once
+
+#include "mpmw_kernels.hpp"
+#include "mpmw_sort.hpp"
+
+namespace mpmw {
+
+struct G2PFlags {
+    int chip;      // run the chip separation / fracture pass (even step)
+    int stats;     // write AABB + metrics block partials
+};
+
+// ------------------------------------------------------- particle state ---
+
+template <typename R> __device__ __forceinline__ void load_state(const Particles<R> &P, int i, PState<R> &p) {
+    p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+    p.T = P.T[i];
+    p.eps = P.eps[i];
+    p.dmg = P.dmg[i];
+    p.sres = P.sres[i];
+    p.rate = P.rate[i];
+    p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+    p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+    if (P.a0) {
+        p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
+        p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p.a[c] = R(0);
+    }
+}
+
+template <typename R> __device__ __forceinline__ void store_state(const Particles<R> &P, int i, const PState<R> &p) {
+    P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
+    P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
+    P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
+    P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
+    if (P.a0) {
+        P.a0[i] = p.a[0]; P.a1[i] = p.a[1]; P.a2[i] = p.a[2];
+        P.a3[i] = p.a[3]; P.a4[i] = p.a[4]; P.a5[i] = p.a[5];
+    }
+    P.eps[i] = p.eps; P.rate[i] = p.rate; P.dmg[i] = p.dmg; P.sres[i] = p.sres; P.T[i] = p.T;
+    P.status[i] = p.status;
+    if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }   // :399
+}
+
+// mpmG2PKernel body, MPMWorkpieceSolver.cu:275-397.  node_at(n, idx) returns
+// the gather record of corner n (global index idx).
+template <typename R, typename NodeAt>
+__device__ __forceinline__ void g2p_core(PState<R> &p, const Frac<R> &f, const GridView<R> &g, const Material<R> &M,
+                                         R dt, double dtd, bool has_back, NodeAt node_at) {
+    R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
+    R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        int64_t idx;
+        R w, gx, gy, gz;
+        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
+        const vec4_t<R> nd = node_at(n, idx);
+        if (nd.w != nd.w) continue;  // node mass <= 1e-18 (:286)
+        v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
+        temp += nd.w * w;
+        wsum += w;
+        L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
+        L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
+        L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
+    }
+    if (p.status == kFixed) {  // :296-300
+        p.vx = p.vy = p.vz = R(0);
+        return;
+    }
+    p.vx = v0; p.vy = v1; p.vz = v2;
+    p.x += (double)p.vx * dtd;
+    p.y += (double)p.vy * dtd;
+    p.z += (double)p.vz * dtd;
+    if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
+    if (p.status == kActive) constitutive_active(M, p, L, dt, has_back);
+    chip_transition(M, p);
+}
+
+__device__ __forceinline__ void accumulate_stats(double st[kRed], double x, double y, double z, double vm, double T,
+                                                 double ke) {
+    st[0] = fmin(st[0], x); st[1] = fmin(st[1], y); st[2] = fmin(st[2], z);
+    st[3] = fmin(st[3], -x); st[4] = fmin(st[4], -y); st[5] = fmin(st[5], -z);
+    st[6] = fmax(st[6], vm);
+    st[7] = fmax(st[7], T);
+    st[8] += ke;
+}
+
+// The 8 channel contributions of one particle to one corner node,
+// mpmP2GKernel :113-132.  Returns the number of channels written to c[]
+// (5: mass, momentum, T*m; 8: plus force).
+template <typename R>
+__device__ __forceinline__ int p2g_corner(R mass, R vol, R vx, R vy, R vz, R T, const R s[6], R fex, R fey, R fez,
+                                          bool stress_force, bool ext_only, R w, R gx, R gy, R gz, R c[8]) {
+    const R m = mass * w;
+    c[0] = m;
+    c[1] = vx * m;
+    c[2] = vy * m;
+    c[3] = vz * m;
+    c[7] = T * m;
+    if (stress_force) {
+        const R fX = -(s[0] * gx + s[3] * gy + s[4] * gz) * vol;
+        const R fY = -(s[3] * gx + s[1] * gy + s[5] * gz) * vol;
+        const R fZ = -(s[4] * gx + s[5] * gy + s[2] * gz) * vol;
+        c[4] = fX + fex * w;
+        c[5] = fY + fey * w;
+        c[6] = fZ + fez * w;
+        return 8;
+    }
+    if (ext_only) {
+        c[4] = fex * w;
+        c[5] = fey * w;
+        c[6] = fez * w;
+        return 8;
+    }
+    return 5;
+}
+
+// ----------------------------------------------------------- tile kernel --
+
+__device__ __forceinline__ int wnode(int x, int y, int z) { return (z * WY + y) * WX + x; }
+
+// The 8 corner contributions of the particles of one cell, 8 channels each
+// (mass, momentum xyz, force xyz, T*m), held in registers.  Corner c has
+// offsets dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1 (the node order
+// of computeWeights, :64-66).
+template <typename R> struct CellAcc {
+    R v[8][8];  // [channel][corner]
+};
+
+// Add one particle's P2G contributions (mpmP2GKernel :110-133) to the
+// accumulators of its base cell.  Corners with w <= 0 are skipped as the
+// reference skips them (:112).
+template <typename R>
+__device__ __forceinline__ void cell_accumulate(CellAcc<R> &A, const Frac<R> &f, R inv_h, R mass, R vol, R vx, R vy,
+                                                R vz, R T, const R s[6], R fex, R fey, R fez, bool stress_force,
+                                                bool ext_only) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const R wx = dx ? f.w1[0] : f.w0[0];
+        const R wy = dy ? f.w1[1] : f.w0[1];
+        const R wz = dz ? f.w1[2] : f.w0[2];
+        const R w = wx * wy * wz;
+        if (!(w > R(0))) continue;
+        const R gx = (dx ? inv_h : -inv_h) * wy * wz;
+        const R gy = wx * (dy ? inv_h : -inv_h) * wz;
+        const R gz = wx * wy * (dz ? inv_h : -inv_h);
+        R ch[8];
+        const int nch = p2g_corner(mass, vol, vx, vy, vz, T, s, fex, fey, fez, stress_force, ext_only, w, gx, gy,
+                                   gz, ch);
+        A.v[0][c] += ch[0];
+        A.v[1][c] += ch[1];
+        A.v[2][c] += ch[2];
+        A.v[3][c] += ch[3];
+        A.v[7][c] += ch[7];
+        if (nch == 8) {
+            A.v[4][c] += ch[4];
+            A.v[5][c] += ch[5];
+            A.v[6][c] += ch[6];
+        }
+    }
+}
+
+// Scatter of one particle whose stencil is not its home cell: through LDS
+// atomics when the stencil lies in the tile window, else device atomics.
+template <typename R>
+__device__ __forceinline__ void scatter_exception(const GridView<R> &g, R *acc, R *s_acc, const Frac<R> &f,
+                                                  int wx0, int wy0, int wz0, R mass, R vol, R vx, R vy, R vz, R T,
+                                                  const R s[6], R fex, R fey, R fez, bool stress_force,
+                                                  bool ext_only) {
+    const int lx = f.b[0] - wx0, ly = f.b[1] - wy0, lz = f.b[2] - wz0;
+    const bool in = s_acc && (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY && (unsigned)lz < (unsigned)TZ;
+    const int wb = wnode(lx, ly, lz);
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        int64_t idx;
+        R w, gx, gy, gz;
+        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
+        R c[8];
+        const int nch = p2g_corner(mass, vol, vx, vy, vz, T, s, fex, fey, fez, stress_force, ext_only, w, gx, gy,
+                                   gz, c);
+        if (in) {
+            R *a = s_acc + wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1);
+            atomicAdd(a, c[0]);
+            atomicAdd(a + WN, c[1]);
+            atomicAdd(a + 2 * WN, c[2]);
+            atomicAdd(a + 3 * WN, c[3]);
+            atomicAdd(a + 7 * WN, c[7]);
+            if (nch == 8) {
+                atomicAdd(a + 4 * WN, c[4]);
+                atomicAdd(a + 5 * WN, c[5]);
+                atomicAdd(a + 6 * WN, c[6]);
+            }
+        } else {
+            R *a = acc + idx;
+            unsafeAtomicAdd(a, c[0]);
+            unsafeAtomicAdd(a + g.nn, c[1]);
+            unsafeAtomicAdd(a + 2 * g.nn, c[2]);
+            unsafeAtomicAdd(a + 3 * g.nn, c[3]);
+            unsafeAtomicAdd(a + 7 * g.nn, c[7]);
+            if (nch == 8) {
+                unsafeAtomicAdd(a + 4 * g.nn, c[4]);
+                unsafeAtomicAdd(a + 5 * g.nn, c[5]);
+                unsafeAtomicAdd(a + 6 * g.nn, c[6]);
+            }
+        }
+    }
+}
+
+template <typename R> __device__ __forceinline__ R shfl_from(R v, int src) { return __shfl(v, src, kWave); }
+
+// Combine the cell accumulators of a wave (an 8 x 8 layer of cells) into
+// node values of node layers z and z+1 and add them to the LDS window.
+// Node (x, y) of a layer = corner (0,0) of cell (x,y) + (1,0) of (x-1,y) +
+// (0,1) of (x,y-1) + (1,1) of (x-1,y-1); the far edge nodes x = 8 / y = 8
+// are formed by the lanes of the last column / row.  Fixed summation order.
+template <typename R>
+__device__ __forceinline__ void combine_layer(const CellAcc<R> &A, R *s_acc, int hx, int hy, int hz, int dz) {
+    const int lane = threadIdx.x % kWave;
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) {
+        const R c00 = A.v[ch][4 * dz + 0], c10 = A.v[ch][4 * dz + 1];
+        const R c01 = A.v[ch][4 * dz + 2], c11 = A.v[ch][4 * dz + 3];
+        const R l1 = shfl_from(c10, lane - 1);   // (x-1, y) corner (1,0)
+        const R l8 = shfl_from(c01, lane - 8);   // (x, y-1) corner (0,1)
+        const R l9 = shfl_from(c11, lane - 9);   // (x-1, y-1) corner (1,1)
+        const R e8 = shfl_from(c11, lane - 8);   // for node (8, y): (7, y-1) corner (1,1)
+        const R e1 = shfl_from(c11, lane - 1);   // for node (x, 8): (x-1, 7) corner (1,1)
+        R v = c00;
+        if (hx >= 1) v += l1;
+        if (hy >= 1) v += l8;
+        if (hx >= 1 && hy >= 1) v += l9;
+        R *a = s_acc + ch * WN;
+        a[wnode(hx, hy, hz + dz)] += v;
+        if (hx == TX - 1) {
+            R ve = c10;
+            if (hy >= 1) ve += e8;
+            a[wnode(TX, hy, hz + dz)] += ve;
+        }
+        if (hy == TY - 1) {
+            R ve = c01;
+            if (hx >= 1) ve += e1;
+            a[wnode(hx, TY, hz + dz)] += ve;
+        }
+        if (hx == TX - 1 && hy == TY - 1) a[wnode(TX, TY, hz + dz)] += c11;
+    }
+}
+
+// One workgroup per tile; thread t owns cell t of the tile (x fastest) and
+// walks that cell's particles slot by slot (the memory order is slot-major,
+// so a full tile reads contiguously).  DO_G2P: gather, constitutive update
+// and chip pass.  DO_P2G: scatter into the next step's accumulators.
+template <typename R, bool DO_G2P, bool DO_P2G>
+__global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
+                                                 double dtd, double zcut, G2PFlags flags, double ref_temp,
+                                                 double *part, const int32_t *cell_start, TileGeom tg,
+                                                 int32_t *oow) {
+    static_assert(TX == 8 && TY == 8 && TZ * TX * TY == kBlock, "one 8x8 cell layer per wave");
+    constexpr int kNodeBytes = DO_G2P ? WN * (int)sizeof(vec4_t<R>) : 0;
+    constexpr int kAccBytes = DO_P2G ? 8 * WN * (int)sizeof(R) : 0;
+    __shared__ __align__(32) unsigned char smem[kNodeBytes + kAccBytes + 32];
+    __shared__ int32_t s_cnt[2][kBlock / kWave];
+    __shared__ int32_t s_red[2][kBlock / kWave];
+    vec4_t<R> *s_node = reinterpret_cast<vec4_t<R> *>(smem);
+    R *s_acc = DO_P2G ? reinterpret_cast<R *>(smem + kNodeBytes) : nullptr;
+
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid % kWave, wv = tid / kWave;
+    const int hx = tid % TX, hy = (tid / TX) % TY, hz = tid / (TX * TY);
+    const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
+    const int wx0 = kCellOffset + tx * TX, wy0 = kCellOffset + ty * TY, wz0 = kCellOffset + tz * TZ;
+    const int key = t * kTileCells + tid;
+    const int c_begin = cell_start[key];
+    const int cnt = cell_start[key + 1] - c_begin;
+    const int b0 = cell_start[t * kTileCells];
+    const int b1 = cell_start[(t + 1) * kTileCells];
+    double st[kRed];
+    stats_identity(st, ref_temp);
+
+    if (b0 < b1) {  // uniform over the workgroup
+        if (DO_G2P) {
+            for (int w = tid; w < WN; w += kBlock) {
+                const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
+                vec4_t<R> nd;
+                if (i < g.nx && j < g.ny && k < g.nz) {
+                    nd = G.node[((int64_t)k * g.ny + j) * g.nx + i];
+                } else {
+                    nd.x = nd.y = nd.z = R(0);
+                    nd.w = R(__builtin_nan(""));
+                }
+                s_node[w] = nd;
+            }
+        }
+        if (DO_P2G) {
+            for (int w = tid; w < 8 * WN; w += kBlock) s_acc[w] = R(0);
+        }
+        // largest and smallest particle count of the tile's cells
+        int mx = cnt, mn = cnt;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            mx = max(mx, __shfl_xor(mx, o, kWave));
+            mn = min(mn, __shfl_xor(mn, o, kWave));
+        }
+        if (lane == 0) { s_red[0][wv] = mx; s_red[1][wv] = mn; }
+        __syncthreads();
+        int maxc = s_red[0][0], minc = s_red[1][0];
+        for (int w = 1; w < kBlock / kWave; ++w) { maxc = max(maxc, s_red[0][w]); minc = min(minc, s_red[1][w]); }
+        const bool uniform = (maxc == minc);
+
+        // home cell: its global cell index
+        const int hgx = wx0 + hx, hgy = wy0 + hy, hgz = wz0 + hz;
+        const bool home_ok = hgx + 1 < g.nx && hgy + 1 < g.ny && hgz + 1 < g.nz;
+        CellAcc<R> A;
+        if (DO_P2G) {
+#pragma unroll
+            for (int ch = 0; ch < 8; ++ch)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) A.v[ch][c] = R(0);
+        }
+        int my_oow = 0;
+        const bool has_back = P.a0 != nullptr;
+        const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+        int base = 0;
+        for (int s = 0; s < maxc; ++s) {
+            const bool active = cnt > s;
+            int i;
+            if (uniform) {
+                i = b0 + s * kBlock + tid;
+            } else {
+                // slot-major position: cells with more than s particles, in key order
+                const unsigned long long mask = __ballot(active);
+                if (lane == 0) s_cnt[s & 1][wv] = __popcll(mask);
+                __syncthreads();
+                int before = 0, total = 0;
+                for (int w = 0; w < kBlock / kWave; ++w) {
+                    const int c = s_cnt[s & 1][w];
+                    before += (w < wv) ? c : 0;
+                    total += c;
+                }
+                i = b0 + base + before + __popcll(mask & lt);
+                base += total;
+            }
+            if (!active) continue;
+            PState<R> p;
+            p.status = P.status[i];
+            if (p.status == kInactive) continue;
+            const R mass = P.m[i], vol = P.V[i];
+            R fex = R(0), fey = R(0), fez = R(0);
+            if (DO_G2P) {
+                load_state(P, i, p);
+                const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                const int lx = f.b[0] - wx0, ly = f.b[1] - wy0, lz = f.b[2] - wz0;
+                const bool in = (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY &&
+                                (unsigned)lz < (unsigned)TZ;
+                const int wb = wnode(lx, ly, lz);
+                my_oow += in ? 0 : 1;
+                g2p_core(p, f, g, M, dt, dtd, has_back, [&](int n, int64_t idx) -> vec4_t<R> {
+                    if (in) return s_node[wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1)];
+                    return G.node[idx];
+                });
+                if (flags.chip) chip_pass(M, p, P.id[i], zcut);
+                store_state(P, i, p);
+                if (flags.stats) {
+                    const double vx = p.vx, vy = p.vy, vz = p.vz;
+                    accumulate_stats(st, p.x, p.y, p.z,
+                                     (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]),
+                                     (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
+                }
+            } else {
+                p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+                p.vx = P.vx[i]; p.vy = P.vy[i]; p.vz = P.vz[i];
+                p.T = P.T[i];
+                p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+                p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+                if (P.fx) { fex = P.fx[i]; fey = P.fy[i]; fez = P.fz[i]; }
+            }
+            if (DO_P2G) {
+                // after G2P the external force has been consumed (:399)
+                const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                const bool stress_force = (p.status == kActive || p.status == kFixed);
+                const bool ext_only = (p.status == kChip);
+                if (home_ok && f.b[0] == hgx && f.b[1] == hgy && f.b[2] == hgz) {
+                    cell_accumulate(A, f, g.inv_h, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex, fey, fez,
+                                    stress_force, ext_only);
+                } else {
+                    ++my_oow;
+                    scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex,
+                                      fey, fez, stress_force, ext_only);
+                }
+            }
+        }
+        if (DO_P2G) {
+            __syncthreads();  // exception atomics are in
+            combine_layer(A, s_acc, hx, hy, hz, 0);
+            __syncthreads();  // layer z written by wave z only
+            combine_layer(A, s_acc, hx, hy, hz, 1);
+            __syncthreads();  // layer z+1 written by wave z only
+            for (int w = tid; w < WN; w += kBlock) {
+                const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
+                if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
+                R *a = G.acc + ((int64_t)k * g.ny + j) * g.nx + i;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const R v = s_acc[c * WN + w];
+                    if (v != R(0)) unsafeAtomicAdd(a + c * g.nn, v);
+                }
+            }
+        }
+        int cntv = my_oow;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) cntv += __shfl_xor(cntv, o, kWave);
+        if (lane == 0 && cntv) atomicAdd(oow, cntv);
+    }
+    if (DO_G2P && flags.stats) block_reduce_stats(st, part);
+}
+
+// ------------------------------------------------------------- binning ----
+
+// Tile-major cell key of every particle on the current grid; INACTIVE
+// particles get key nkeys and sort to the end.
+template <typename R>
+__global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, uint32_t *keys, int32_t *idx,
+                           int32_t *counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    idx[i] = i;
+    uint32_t key = (uint32_t)tg.nkeys;
+    if (P.status[i] != kInactive) {
+        const double r[3] = {(P.x[i] - g.ox) / g.hd, (P.y[i] - g.oy) / g.hd, (P.z[i] - g.oz) / g.hd};
+        const int nt[3] = {tg.ntx * TX, tg.nty * TY, tg.ntz * TZ};
+        int c[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double fl = floor(r[a]) - kCellOffset;
+            fl = fmin(fmax(fl, 0.0), (double)(nt[a] - 1));
+            c[a] = (int)fl;
+        }
+        const uint32_t tile = (uint32_t)(((c[2] / TZ) * tg.nty + c[1] / TY) * tg.ntx + c[0] / TX);
+        const uint32_t local = (uint32_t)(((c[2] % TZ) * TY + c[1] % TY) * TX + c[0] % TX);
+        key = tile * kTileCells + local;
+    }
+    keys[i] = key;
+    atomicAdd(&counts[key], 1);
+}
+
+// Slot-major order inside each tile: for slot s = 0, 1, ... the s-th particle
+// of every cell holding more than s particles, cells in key order.
+__global__ void __launch_bounds__(kBlock) k_slot_major(const int32_t *cell_start, const int32_t *idx_sorted,
+                                                       int32_t *perm, int32_t *tile_start) {
+    static_assert(kTileCells == kBlock, "one cell per thread");
+    __shared__ int32_t s_wave[kBlock / kWave];
+    __shared__ int32_t s_max[kBlock / kWave];
+    const int t = blockIdx.x, c = threadIdx.x;
+    const int lane = c % kWave, wid = c / kWave;
+    const int key = t * kTileCells + c;
+    const int cs = cell_start[key];
+    const int cnt = cell_start[key + 1] - cs;
+    const int tstart = cell_start[t * kTileCells];
+    if (c == 0) tile_start[t] = tstart;
+    int m = cnt;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if (lane == 0) s_max[wid] = m;
+    __syncthreads();
+    int maxc = s_max[0];
+    for (int w = 1; w < kBlock / kWave; ++w) maxc = max(maxc, s_max[w]);
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int base = tstart;
+    for (int s = 0; s < maxc; ++s) {
+        const bool flag = cnt > s;
+        const unsigned long long mask = __ballot(flag);
+        const int pos = __popcll(mask & lt);
+        __syncthreads();
+        if (lane == 0) s_wave[wid] = __popcll(mask);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            if (w < wid) before += s_wave[w];
+            total += s_wave[w];
+        }
+        if (flag) perm[base + before + pos] = idx_sorted[cs + s];
+        base += total;
+    }
+}
+
+// Inactive particles (key nkeys) keep their sorted order after the tiles.
+__global__ void k_tail(const int32_t *cell_start, int32_t nkeys, int32_t ntiles, const int32_t *idx_sorted,
+                       int32_t *perm, int32_t *tile_start, int n) {
+    const int first = cell_start[nkeys];
+    const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_start[ntiles] = first;
+    if (i < n) perm[i] = idx_sorted[i];
+}
+
+}  // namespace mpmw

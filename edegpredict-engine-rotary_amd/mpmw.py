@@ -30,7 +30,7 @@ EXPORTS = (
     "mpmw_get_metrics", "mpmw_sync_metrics", "mpmw_compact", "mpmw_get_bounds", "mpmw_temperature_at",
     "mpmw_apply_heat_flux", "mpmw_set_particle_temperatures", "mpmw_set_deterministic",
     "mpmw_get_grid_info", "mpmw_set_grid_snapshot", "mpmw_get_grid_snapshot", "mpmw_set_kernel_timing",
-    "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes",
+    "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes", "mpmw_get_diagnostics",
 )
 
 
@@ -80,6 +80,12 @@ class GridInfo(C.Structure):
 
 class KernelTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [("sorts", C.c_int64), ("out_of_window", C.c_int64), ("steps_since_rebuild", C.c_int64),
+                ("tiles", C.c_int32), ("tile_cells", C.c_int32), ("max_tile_particles", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class MPMError(RuntimeError):
@@ -138,6 +144,7 @@ def lib():
         "mpmw_get_kernel_times": (C.c_int, [vp, C.POINTER(KernelTimes)]),
         "mpmw_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "mpmw_device_bytes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mpmw_get_diagnostics": (C.c_int, [vp, C.POINTER(Diagnostics)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -351,6 +358,11 @@ class Solver:
         v = C.c_void_p()
         _check(lib().mpmw_get_stream(self.h, C.byref(v)), "get_stream")
         return v.value
+
+    def diagnostics(self):
+        d = Diagnostics()
+        _check(lib().mpmw_get_diagnostics(self.h, C.byref(d)), "get_diagnostics")
+        return {k: getattr(d, k) for k, _ in Diagnostics._fields_ if k != "pad_"}
 
     def device_bytes(self):
         v = C.c_int64()

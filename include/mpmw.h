@@ -216,6 +216,17 @@ int mpmw_get_stream(mpmw_solver *s, void **stream);
 /* Device bytes currently held by the solver. */
 int mpmw_device_bytes(mpmw_solver *s, int64_t *bytes);
 
+/* Layout diagnostics of the particle binning. */
+typedef struct mpmw_diagnostics {
+    int64_t sorts;             /* times the particles were re-binned into tiles */
+    int64_t out_of_window;     /* particle visits that took the device-memory path since the last grid rebuild */
+    int64_t steps_since_rebuild;
+    int32_t tiles, tile_cells; /* tiles of the current binning, cells per tile */
+    int32_t max_tile_particles;
+    int32_t pad_;
+} mpmw_diagnostics;
+int mpmw_get_diagnostics(mpmw_solver *s, mpmw_diagnostics *d);
+
 #ifdef __cplusplus
 }
 #endif
