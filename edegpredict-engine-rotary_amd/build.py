@@ -47,6 +47,12 @@ def build(verbose=True):
     srcs = [s for s in srcs if os.path.exists(s)]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
+    # the C++ drop-in class driven through the reference interfaces
+    test_src = os.path.join(ROOT, "tests", "cpp", "solver_api_test.cpp")
+    if os.path.exists(test_src):
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o",
+              os.path.join(LIB_DIR, "solver_api_test"), test_src, "-L", LIB_DIR, "-lmpmw_hip",
+              f"-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64"])
     return out
 
 

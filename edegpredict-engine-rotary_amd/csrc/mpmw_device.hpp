@@ -55,6 +55,7 @@ template <typename R> struct Particles {
     R *a0, *a1, *a2, *a3, *a4, *a5;
     R *fx, *fy, *fz;
     R *eps, *rate, *dmg, *sres, *T, *m, *V;
+    R m0, V0;                  // mass and volume of every particle when m == nullptr
     int32_t *status, *id;
     int32_t n;
 };
@@ -192,7 +193,10 @@ __device__ __forceinline__ double tool_phi(const Tool &t, double x, double y, do
 
 // ---------------------------------------------------------- constitutive --
 
-// Register copy of one particle's mutable state.
+// Register copy of one particle's mutable state.  `sres` holds the largest
+// candidate value for the residual stress met during the step (-inf if
+// none); the stored value becomes max(stored, sres), which is the
+// reference's chain of fmax(residualStress, x) updates.
 template <typename R> struct PState {
     double x, y, z;
     R vx, vy, vz;
@@ -202,9 +206,8 @@ template <typename R> struct PState {
 };
 
 // ACTIVE-particle block of mpmG2PKernel, MPMWorkpieceSolver.cu:308-382.
-template <typename R>
-__device__ __forceinline__ void constitutive_active(const Material<R> &M, PState<R> &p, const R L[9], R dt,
-                                                    bool has_back) {
+template <typename R, bool HAS_BACK>
+__device__ __forceinline__ void constitutive_active(const Material<R> &M, PState<R> &p, const R L[9], R dt) {
     R Dxx = L[0], Dyy = L[4], Dzz = L[8];
     R Dxy = R(0.5) * (L[1] + L[3]);
     R Dxz = R(0.5) * (L[2] + L[6]);
@@ -223,15 +226,15 @@ __device__ __forceinline__ void constitutive_active(const Material<R> &M, PState
     p.s[5] += R(2) * M.shear * Dyz * dt;
     R sh[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sh[c] = p.s[c] - p.a[c];
+    for (int c = 0; c < 6; ++c) sh[c] = HAS_BACK ? p.s[c] - p.a[c] : p.s[c];
     R seq = von_mises(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]);
     R flow = jc_flow(M, p.eps, p.rate, p.T);
     if (seq > flow && seq > R(1e-9)) {
         R scale = flow / seq;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) p.s[c] = p.a[c] + scale * sh[c];
+        for (int c = 0; c < 6; ++c) p.s[c] = HAS_BACK ? p.a[c] + scale * sh[c] : scale * sh[c];
         R dp = (seq - flow) / rmax(R(3) * M.shear, R(1));
-        if (has_back && M.H > R(0) && dp > R(0)) {
+        if (HAS_BACK && M.H > R(0) && dp > R(0)) {
             R ns = M.H * dp / seq;
 #pragma unroll
             for (int c = 0; c < 6; ++c) p.a[c] += ns * sh[c];
@@ -246,7 +249,10 @@ __device__ __forceinline__ void constitutive_active(const Material<R> &M, PState
         R degrade = (M.dth - p.dmg) / (M.dth * R(0.3));
         degrade = rmax(R(0), degrade);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) { p.s[c] *= degrade; p.a[c] *= degrade; }
+        for (int c = 0; c < 6; ++c) {
+            p.s[c] *= degrade;
+            if (HAS_BACK) p.a[c] *= degrade;
+        }
         p.sres = rmax(p.sres, von_mises(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]));
     }
 }
@@ -262,9 +268,11 @@ template <typename R> __device__ __forceinline__ void chip_transition(const Mate
     }
 }
 
-// chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725.
+// chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725.  The
+// particle id (seed of the fracture kick) is read only when a kick happens.
 template <typename R>
-__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, int32_t id, double zcut) {
+__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, const int32_t *ids, int i,
+                                          double zcut) {
     if (p.status == kActive) {
         if (p.z < zcut && p.eps > R(1e-6)) {
             p.status = kChip;
@@ -277,7 +285,7 @@ __device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, in
     }
     if (p.status == kChip) {
         if (p.dmg < M.dth * R(1.5)) return;
-        uint32_t seed = (uint32_t)id * 2654435761u;
+        uint32_t seed = (uint32_t)ids[i] * 2654435761u;
         R rx = (R)((seed >> 16) & 0xFFu) / R(255) - R(0.5);
         R ry = (R)((seed >> 8) & 0xFFu) / R(255) - R(0.5);
         R rz = (R)(seed & 0xFFu) / R(255) - R(0.5);

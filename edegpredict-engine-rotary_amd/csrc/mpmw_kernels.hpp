@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
         v[6] = fmax(v[6], vm);
         v[7] = fmax(v[7], (double)P.T[i]);
         double vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
-        v[8] += 0.5 * (double)P.m[i] * (vx * vx + vy * vy + vz * vz);
+        v[8] += 0.5 * (double)(P.m ? P.m[i] : P.m0) * (vx * vx + vy * vy + vz * vz);
     }
     block_reduce_stats(v, part);
 }
@@ -339,7 +339,7 @@ __global__ void k_nearest_finalize(Particles<R> P, const double *best_d2, const 
     *out_i = bi;
     if (apply && bi >= 0 && bd <= h2) {
         double T = (double)P.T[bi];
-        double m = (double)P.m[bi];
+        double m = (double)(P.m ? P.m[bi] : P.m0);
         P.T[bi] = (R)fmin(T + q / fmax(m * cp, 1e-12), melt);
     }
 }

@@ -112,6 +112,8 @@ struct mpmw_solver {
     bool metrics_pending = false, cf_pending = false;
     double bounds[6] = {0, 0, 0, 0, 0, 0};
     bool deterministic = false;
+    bool uniform_mv = false;       // every particle has mass m0 and volume V0
+    double m0 = 0.0, V0 = 0.0;
 
     // sorting
     SortState sort{};
@@ -249,7 +251,12 @@ template <typename R> Particles<R> pview(mpmw_solver *s) {
     P.a3 = F(F_A3); P.a4 = F(F_A4); P.a5 = F(F_A5);
     P.fx = s->has_fext ? F(F_FX) : nullptr; P.fy = F(F_FY); P.fz = F(F_FZ);
     P.eps = F(F_EPS); P.rate = F(F_RATE); P.dmg = F(F_DMG); P.sres = F(F_SRES);
-    P.T = F(F_T); P.m = F(F_M); P.V = F(F_V);
+    P.T = F(F_T);
+    // uniform mass and volume are passed as scalars (the arrays stay for export)
+    P.m = s->uniform_mv ? nullptr : F(F_M);
+    P.V = F(F_V);
+    P.m0 = (R)s->m0;
+    P.V0 = (R)s->V0;
     P.status = (int32_t *)s->f[F_STATUS];
     P.id = (int32_t *)s->f[F_ID];
     P.n = s->n;
@@ -591,6 +598,19 @@ template <typename R> int compact_impl(mpmw_solver *s);
 
 // ------------------------------------------------------------- the step ---
 
+// Launch one tile pass; the backstress arrays select the kernel variant.
+template <typename R, bool DO_G2P, bool DO_P2G>
+void launch_tile(mpmw_solver *s, const Particles<R> &P, const GridView<R> &g, const Grid<R> &G,
+                 const Material<R> &M, double dt, double zcut, G2PFlags fl) {
+    const TileGeom tg = s->sort.geom;
+    if (P.a0)
+        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, true>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P, g, G,
+                           M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+    else
+        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, false>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P, g,
+                           G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+}
+
 // MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791.
 template <typename R> int step_impl(mpmw_solver *s, double dt) {
     if (s->n <= 0 || s->nn <= 0) return MPMW_OK;  // :728 silent return
@@ -613,8 +633,7 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     if (!s->p2g_ready) {
         if ((rc = zero_acc<R>(s))) return rc;
         Timed t(s, MPMW_K_P2G);
-        hipLaunchKernelGGL((k_tile<R, false, true>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt, dt,
-                           zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+        launch_tile<R, false, true>(s, P, g, G, M, dt, zcut, fl);
         HIP_TRY(hipGetLastError());
     }
     const bool diffusion = s->prm.specific_heat > 0.0 &&
@@ -647,11 +666,9 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     {
         Timed t(s, MPMW_K_G2P_P2G);
         if (next_rebuild)
-            hipLaunchKernelGGL((k_tile<R, true, false>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt,
-                               dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+            launch_tile<R, true, false>(s, P, g, G, M, dt, zcut, fl);
         else
-            hipLaunchKernelGGL((k_tile<R, true, true>), dim3(nb), dim3(kBlock), 0, s->stream, P, g, G, M, (R)dt,
-                               dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+            launch_tile<R, true, true>(s, P, g, G, M, dt, zcut, fl);
         HIP_TRY(hipGetLastError());
     }
     if (fl.stats) {
@@ -807,6 +824,11 @@ template <typename R> int set_particles_impl(mpmw_solver *s, const mpmw_particle
                            : upload_field<R>(s, e.fld, e.src, n, e.comp, e.c, e.fill);
         if (rc) return rc;
     }
+    s->uniform_mv = true;
+    s->m0 = a->mass[0];
+    s->V0 = a->volume[0];
+    for (int32_t i = 1; i < n && s->uniform_mv; ++i)
+        s->uniform_mv = a->mass[i] == s->m0 && a->volume[i] == s->V0;
     HIP_TRY(hipMemcpy(s->f[F_STATUS], a->status, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->f[F_ID], a->id, sizeof(int32_t) * n, hipMemcpyHostToDevice));
     s->p2g_ready = false;

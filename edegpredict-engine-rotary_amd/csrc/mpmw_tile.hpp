@@ -3,8 +3,8 @@
 // One workgroup per tile of TX x TY x TZ cells; thread t owns cell t.  The
 // tile's node window (WX x WY x WZ nodes) is staged in LDS: the gather record
 // (v, T) for G2P and eight accumulator channels for P2G.  A thread keeps the
-// 8 gather nodes of its cell and the 8 x 8 P2G corner sums of its cell's
-// particles in registers; after the particle loop the sums are combined
+// 8 x 8 P2G corner sums of its cell's particles in registers; after the
+// particle loop the sums are combined
 // across the wave with lane shuffles into node values and added to the LDS
 // window layer by layer, then the window is flushed once to HBM.  No LDS
 // float atomics are used on this path (they serialise on gfx950).  A
@@ -27,70 +27,85 @@ struct G2PFlags {
 
 // ------------------------------------------------------- particle state ---
 
-template <typename R> __device__ __forceinline__ void load_state(const Particles<R> &P, int i, PState<R> &p) {
-    p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+// One particle through mpmG2PKernel (MPMWorkpieceSolver.cu:264-400) and, on
+// chip steps, chipFractureAndSeparationKernel (:667-725).  The caller has
+// loaded status and position.  Only the fields the particle's status makes
+// the reference read are loaded, and only fields that changed are stored.
+// node_at(n, idx) returns the gather record of corner n (global index idx).
+template <typename R, bool HAS_BACK, typename NodeAt>
+__device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, PState<R> &p, const Frac<R> &f,
+                                             const GridView<R> &g, const Material<R> &M, R dt, double dtd,
+                                             bool chip_step, bool need_stress, double zcut, NodeAt node_at) {
+    const int32_t st0 = p.status;
+    const bool act = st0 == kActive, fixed = st0 == kFixed;
     p.T = P.T[i];
-    p.eps = P.eps[i];
-    p.dmg = P.dmg[i];
-    p.sres = P.sres[i];
-    p.rate = P.rate[i];
-    p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
-    p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
-    if (P.a0) {
-        p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
-        p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
-    } else {
+    p.sres = -__builtin_huge_valf();
+    p.rate = R(0);
+    p.eps = R(0);
+    p.dmg = R(0);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) p.a[c] = R(0);
+    for (int c = 0; c < 6; ++c) { p.s[c] = R(0); p.a[c] = R(0); }
+    if (act || fixed || need_stress) {
+        p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+        p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
     }
-}
-
-template <typename R> __device__ __forceinline__ void store_state(const Particles<R> &P, int i, const PState<R> &p) {
-    P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z;
-    P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
-    P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
-    P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
-    if (P.a0) {
-        P.a0[i] = p.a[0]; P.a1[i] = p.a[1]; P.a2[i] = p.a[2];
-        P.a3[i] = p.a[3]; P.a4[i] = p.a[4]; P.a5[i] = p.a[5];
+    if (act) {
+        p.eps = P.eps[i];
+        if (HAS_BACK) {
+            p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
+            p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
+        }
     }
-    P.eps[i] = p.eps; P.rate[i] = p.rate; P.dmg[i] = p.dmg; P.sres[i] = p.sres; P.T[i] = p.T;
-    P.status[i] = p.status;
-    if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }   // :399
-}
-
-// mpmG2PKernel body, MPMWorkpieceSolver.cu:275-397.  node_at(n, idx) returns
-// the gather record of corner n (global index idx).
-template <typename R, typename NodeAt>
-__device__ __forceinline__ void g2p_core(PState<R> &p, const Frac<R> &f, const GridView<R> &g, const Material<R> &M,
-                                         R dt, double dtd, bool has_back, NodeAt node_at) {
-    R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
-    R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        int64_t idx;
-        R w, gx, gy, gz;
-        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
-        const vec4_t<R> nd = node_at(n, idx);
-        if (nd.w != nd.w) continue;  // node mass <= 1e-18 (:286)
-        v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
-        temp += nd.w * w;
-        wsum += w;
-        L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
-        L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
-        L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
-    }
-    if (p.status == kFixed) {  // :296-300
+    if (act || st0 == kChip) p.dmg = P.dmg[i];
+    const R eps0 = p.eps, dmg0 = p.dmg, T0 = p.T;
+    if (fixed) {  // :296-300
         p.vx = p.vy = p.vz = R(0);
-        return;
+    } else {
+        R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
+        R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            int64_t idx;
+            R w, gx, gy, gz;
+            if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
+            const vec4_t<R> nd = node_at(n, idx);
+            if (nd.w != nd.w) continue;  // node mass <= 1e-18 (:286)
+            v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
+            temp += nd.w * w;
+            wsum += w;
+            L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
+            L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
+            L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
+        }
+        p.vx = v0; p.vy = v1; p.vz = v2;
+        p.x += (double)p.vx * dtd;
+        p.y += (double)p.vy * dtd;
+        p.z += (double)p.vz * dtd;
+        if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
+        if (act) constitutive_active<R, HAS_BACK>(M, p, L, dt);
+        chip_transition(M, p);
     }
-    p.vx = v0; p.vy = v1; p.vz = v2;
-    p.x += (double)p.vx * dtd;
-    p.y += (double)p.vy * dtd;
-    p.z += (double)p.vz * dtd;
-    if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
-    if (p.status == kActive) constitutive_active(M, p, L, dt, has_back);
-    chip_transition(M, p);
+    if (chip_step) chip_pass(M, p, P.id, i, zcut);
+    if (!fixed) { P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z; }
+    P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
+    if (act) {
+        P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
+        P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
+        if (HAS_BACK) {
+            P.a0[i] = p.a[0]; P.a1[i] = p.a[1]; P.a2[i] = p.a[2];
+            P.a3[i] = p.a[3]; P.a4[i] = p.a[4]; P.a5[i] = p.a[5];
+        }
+        P.rate[i] = p.rate;
+    }
+    if (p.eps != eps0) P.eps[i] = p.eps;
+    if (p.dmg != dmg0) P.dmg[i] = p.dmg;
+    if (p.T != T0) P.T[i] = p.T;
+    if (p.status != st0) P.status[i] = p.status;
+    if (p.sres > -__builtin_huge_valf()) {
+        const R old = P.sres[i];
+        if (p.sres > old) P.sres[i] = p.sres;
+    }
+    if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }   // :399
 }
 
 __device__ __forceinline__ void accumulate_stats(double st[kRed], double x, double y, double z, double vm, double T,
@@ -267,7 +282,7 @@ __device__ __forceinline__ void combine_layer(const CellAcc<R> &A, R *s_acc, int
 // walks that cell's particles slot by slot (the memory order is slot-major,
 // so a full tile reads contiguously).  DO_G2P: gather, constitutive update
 // and chip pass.  DO_P2G: scatter into the next step's accumulators.
-template <typename R, bool DO_G2P, bool DO_P2G>
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
 __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
                                                  double dtd, double zcut, G2PFlags flags, double ref_temp,
                                                  double *part, const int32_t *cell_start, TileGeom tg,
@@ -335,7 +350,6 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
                 for (int c = 0; c < 8; ++c) A.v[ch][c] = R(0);
         }
         int my_oow = 0;
-        const bool has_back = P.a0 != nullptr;
         const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int base = 0;
         for (int s = 0; s < maxc; ++s) {
@@ -361,22 +375,21 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
             PState<R> p;
             p.status = P.status[i];
             if (p.status == kInactive) continue;
-            const R mass = P.m[i], vol = P.V[i];
+            const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
             R fex = R(0), fey = R(0), fez = R(0);
+            p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
             if (DO_G2P) {
-                load_state(P, i, p);
                 const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
                 const int lx = f.b[0] - wx0, ly = f.b[1] - wy0, lz = f.b[2] - wz0;
                 const bool in = (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY &&
                                 (unsigned)lz < (unsigned)TZ;
                 const int wb = wnode(lx, ly, lz);
                 my_oow += in ? 0 : 1;
-                g2p_core(p, f, g, M, dt, dtd, has_back, [&](int n, int64_t idx) -> vec4_t<R> {
-                    if (in) return s_node[wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1)];
-                    return G.node[idx];
-                });
-                if (flags.chip) chip_pass(M, p, P.id[i], zcut);
-                store_state(P, i, p);
+                g2p_particle<R, HAS_BACK>(P, i, p, f, g, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut,
+                                          [&](int n, int64_t idx) -> vec4_t<R> {
+                                              if (in) return s_node[wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1)];
+                                              return G.node[idx];
+                                          });
                 if (flags.stats) {
                     const double vx = p.vx, vy = p.vy, vz = p.vz;
                     accumulate_stats(st, p.x, p.y, p.z,
@@ -384,11 +397,15 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
                                      (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
                 }
             } else {
-                p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
                 p.vx = P.vx[i]; p.vy = P.vy[i]; p.vz = P.vz[i];
                 p.T = P.T[i];
-                p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
-                p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+                if (p.status == kActive || p.status == kFixed) {
+                    p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+                    p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) p.s[c] = R(0);
+                }
                 if (P.fx) { fex = P.fx[i]; fey = P.fy[i]; fez = P.fz[i]; }
             }
             if (DO_P2G) {
