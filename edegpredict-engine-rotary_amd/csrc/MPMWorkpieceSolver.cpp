@@ -209,8 +209,7 @@ std::vector<MPMParticle> MPMWorkpieceSolver::getParticlesForExport() {
         sp.x = pos[3 * i]; sp.y = pos[3 * i + 1]; sp.z = pos[3 * i + 2];
         sp.vx = vel[3 * i]; sp.vy = vel[3 * i + 1]; sp.vz = vel[3 * i + 2];
         sp.fx = fext[3 * i]; sp.fy = fext[3 * i + 1]; sp.fz = fext[3 * i + 2];
-        sp.mass = mass[i];
-        sp.volume = vol[i];
+        sp.mass = mass[i];  // volume is not exported (left at its default), as in the reference
         sp.density = dens[i];
         sp.temperature = T[i];
         sp.plasticStrain = eps[i];

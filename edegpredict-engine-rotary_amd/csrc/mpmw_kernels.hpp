@@ -10,6 +10,7 @@
 #pragma once
 
 #include "mpmw_device.hpp"
+#include "mpmw_sort.hpp"
 
 namespace mpmw {
 
@@ -117,32 +118,94 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 
 // ---------------------------------------------------------- grid update ---
 
+// Where the grid update finds this step's P2G sums.  part == nullptr (atomic
+// mode): in the node accumulators.  Otherwise (deterministic mode): in the
+// tile windows, part[(tile * 8 + channel) * WN + window node], written whole
+// by every non-empty tile, plus, when `loose` is set, the node accumulators
+// (loose particles, summed in index order).
+template <typename R> struct TileParts {
+    const R *part;
+    const int32_t *tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
+    TileGeom tg;
+    int32_t loose;
+};
+
+// The tiles whose window holds node coordinate i along one axis, lower
+// tile first, with the window-local coordinate: node i is local T of tile
+// q - 1 and local 0 of tile q when i - kCellOffset = q * T.
+__device__ __forceinline__ int window_tiles(int i, int nt, int T, int tile[2], int local[2]) {
+    const int r = i - kCellOffset;
+    if (r < 0) return 0;
+    const int q = r / T, l = r - q * T;
+    int k = 0;
+    if (l == 0 && q >= 1 && q - 1 < nt) { tile[k] = q - 1; local[k] = T; ++k; }
+    if (q < nt) { tile[k] = q; local[k] = l; ++k; }
+    return k;
+}
+
+// This step's 8 P2G channel sums of node (i, j, k), in a fixed order:
+// tiles by increasing z, then y, then x tile index, then the accumulators.
+template <typename R>
+__device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, int j, int k, R c[8]) {
+    int tx[2], lx[2], ty[2], ly[2], tz[2], lz[2];
+    const int nxs = window_tiles(i, tp.tg.ntx, TX, tx, lx);
+    const int nys = window_tiles(j, tp.tg.nty, TY, ty, ly);
+    const int nzs = window_tiles(k, tp.tg.ntz, TZ, tz, lz);
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch) c[ch] = R(0);
+    for (int a = 0; a < nzs; ++a)
+        for (int b = 0; b < nys; ++b)
+            for (int d = 0; d < nxs; ++d) {
+                const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
+                if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
+                const R *src = tp.part + (int64_t)t * (8 * WN) + (lz[a] * WY + ly[b]) * WX + lx[d];
+#pragma unroll
+                for (int ch = 0; ch < 8; ++ch) c[ch] += src[ch * WN];
+            }
+}
+
 // mpmGridUpdateKernel, MPMWorkpieceSolver.cu:136-193 (point samples) and its
-// analytic-SDF form with tool velocity and Coulomb friction.  Reads and zeroes
-// the accumulators; writes node velocity, mass and T*m.  When `direct_T` is
+// analytic-SDF form with tool velocity and Coulomb friction.  Takes the P2G
+// sums from where `tp` says, zeroing the accumulators it has read; writes
+// node velocity, mass and T*m.  When `direct_T` is
 // set (no diffusion pass follows) it also writes the gather temperature.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, Material<R> M, Tool tool,
-                                                        R dt, int direct_T, double *force_part) {
+__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
+                                                        Tool tool, R dt, int direct_T, double *force_part) {
     const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     double cf = 0.0;
     if (idx < g.nn) {
-        R mass = G.acc[idx];
-        R vx = G.acc[g.nn + idx], vy = G.acc[2 * g.nn + idx], vz = G.acc[3 * g.nn + idx];
-        R fxn = G.acc[4 * g.nn + idx], fyn = G.acc[5 * g.nn + idx], fzn = G.acc[6 * g.nn + idx];
-        R tm = G.acc[7 * g.nn + idx];
+        const int i = (int)(idx % g.nx);
+        const int64_t t = idx / g.nx;
+        const int j = (int)(t % g.ny), k = (int)(t / g.ny);
+        R c[8];
+        if (tp.part) {
+            gather_tile_sums(tp, i, j, k, c);
+            if (tp.loose) {
+#pragma unroll
+                for (int ch = 0; ch < 8; ++ch) {
+                    c[ch] += G.acc[ch * g.nn + idx];
+                    G.acc[ch * g.nn + idx] = R(0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < 8; ++ch) {
+                c[ch] = G.acc[ch * g.nn + idx];
+                G.acc[ch * g.nn + idx] = R(0);
+            }
+        }
         if (G.snap) {
 #pragma unroll
-            for (int c = 0; c < 8; ++c) G.snap[c * g.nn + idx] = G.acc[c * g.nn + idx];
+            for (int ch = 0; ch < 8; ++ch) G.snap[ch * g.nn + idx] = c[ch];
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) G.acc[c * g.nn + idx] = R(0);
+        const R mass = c[0];
+        R vx = c[1], vy = c[2], vz = c[3];
+        const R fxn = c[4], fyn = c[5], fzn = c[6];
+        R tm = c[7];
         bool live = mass > R(1e-18);
         if (live) {
             vx /= mass; vy /= mass; vz /= mass;
-            const int i = (int)(idx % g.nx);
-            const int64_t t = idx / g.nx;
-            const int j = (int)(t % g.ny), k = (int)(t / g.ny);
             if (k <= g.kfix) {
                 vx = vy = vz = R(0);
             } else {

@@ -117,6 +117,9 @@ struct mpmw_solver {
 
     // sorting
     SortState sort{};
+    // deterministic mode: per-tile P2G windows (ntiles x 8 x WN of R)
+    void *tpart = nullptr;
+    int64_t tpart_cap = 0;         // bytes
 
     // timing
     bool timing = false;
@@ -531,28 +534,38 @@ template <typename R> int sort_particles(mpmw_solver *s) {
         return rc;
     S.n_cap = std::min(std::min(ncap, ncap2), std::min(std::min(ncap3, ncap4), ncap5));
     int32_t kc = S.key_cap, kc2 = S.key_cap;
-    if ((rc = grow32(s, S.counts, kc, tg.nkeys + 1)) || (rc = grow32(s, S.cell_start, kc2, tg.nkeys + 1))) return rc;
+    if ((rc = grow32(s, S.counts, kc, tg.nkeys + 2)) || (rc = grow32(s, S.cell_start, kc2, tg.nkeys + 2))) return rc;
     S.key_cap = std::min(kc, kc2);
     if ((rc = grow32(s, S.tile_start, S.tile_cap, tg.ntiles + 1))) return rc;
     if (!S.oow) {
         void *q;
-        if ((rc = dalloc(s, &q, sizeof(int32_t)))) return rc;
+        if ((rc = dalloc(s, &q, 2 * sizeof(int32_t)))) return rc;
         S.oow = (int32_t *)q;
-        HIP_TRY(hipHostMalloc((void **)&S.h_oow, sizeof(int32_t), hipHostMallocDefault));
-        *S.h_oow = 0;
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)1, 0);
+        HIP_TRY(hipHostMalloc((void **)&S.h_oow, 4 * sizeof(int32_t), hipHostMallocDefault));
+        S.h_oow[0] = S.h_oow[1] = S.h_oow[2] = S.h_oow[3] = 0;
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)2, 0);
+    }
+    if (s->deterministic) {
+        const int64_t bytes = (int64_t)tg.ntiles * 8 * WN * (int64_t)s->rsz;
+        if (bytes > s->tpart_cap) {
+            const int64_t c = std::max<int64_t>(bytes, s->tpart_cap * 5 / 4);
+            dfree(s, s->tpart, (size_t)s->tpart_cap);
+            s->tpart_cap = 0;
+            if ((rc = dalloc(s, &s->tpart, (size_t)c))) return rc;
+            s->tpart_cap = c;
+        }
     }
     int nbits = 1;
-    while (((int64_t)1 << nbits) <= tg.nkeys) ++nbits;
+    while (((int64_t)1 << nbits) <= tg.nkeys + 1) ++nbits;
     {
         Timed t(s, MPMW_K_SORT);
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(std::min(blocks_for(tg.nkeys + 1), 4096)), dim3(kBlock), 0,
-                           s->stream, S.counts, (int64_t)tg.nkeys + 1, 0);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(std::min(blocks_for(tg.nkeys + 2), 4096)), dim3(kBlock), 0,
+                           s->stream, S.counts, (int64_t)tg.nkeys + 2, 0);
         hipLaunchKernelGGL(k_bin_keys<R>, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, pview<R>(s),
                            gview<R>(s), tg, S.keys, S.idx, S.counts);
         HIP_TRY(hipGetLastError());
         HIP_TRY(radix_sort_pairs(S.temp, S.keys, S.keys_sorted, S.idx, S.idx_sorted, n, nbits, s->stream));
-        HIP_TRY(exclusive_scan(S.temp2, S.counts, S.cell_start, tg.nkeys + 1, s->stream));
+        HIP_TRY(exclusive_scan(S.temp2, S.counts, S.cell_start, tg.nkeys + 2, s->stream));
         hipLaunchKernelGGL(k_slot_major, dim3(tg.ntiles), dim3(kBlock), 0, s->stream, S.cell_start, S.idx_sorted,
                            S.perm, S.tile_start);
         hipLaunchKernelGGL(k_tail, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, S.cell_start, tg.nkeys,
@@ -560,6 +573,12 @@ template <typename R> int sort_particles(mpmw_solver *s) {
         HIP_TRY(hipGetLastError());
     }
     if ((rc = permute_particles<R>(s, S.perm, n, false))) return rc;
+    // the loose range [cell_start[nkeys], cell_start[nkeys + 1]) for the host
+    HIP_TRY(hipMemcpyAsync(S.h_oow + 2, S.cell_start + tg.nkeys, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    S.loose_begin = S.h_oow[2];
+    S.loose_end = S.h_oow[3];
     S.geom = tg;
     S.valid = true;
     S.origin[0] = s->minx; S.origin[1] = s->miny; S.origin[2] = s->minz;
@@ -584,7 +603,12 @@ template <typename R> int rebuild(mpmw_solver *s) {
         const TileGeom tg = tile_geom(s);
         need = tg.ntx != S.geom.ntx || tg.nty != S.geom.nty || tg.ntz != S.geom.ntz;
         const double o[3] = {s->minx, s->miny, s->minz};
-        for (int a = 0; a < 3; ++a) need = need || std::fabs(o[a] - S.origin[a]) >= 0.5 * s->h;
+        // the deterministic P2G needs every particle in its binned cell: any
+        // origin change re-bins (else the first P2G would be repeated)
+        for (int a = 0; a < 3; ++a) {
+            const double d = std::fabs(o[a] - S.origin[a]);
+            need = need || (s->deterministic ? d > 0.0 : d >= 0.5 * s->h);
+        }
         // particles that left their tile window since the last binning
         need = need || (double)*S.h_oow > 0.01 * (double)s->n * (double)std::max<int64_t>(1, S.steps_since);
     }
@@ -598,17 +622,83 @@ template <typename R> int compact_impl(mpmw_solver *s);
 
 // ------------------------------------------------------------- the step ---
 
-// Launch one tile pass; the backstress arrays select the kernel variant.
+// Blocks of the loose-particle pass (0 when there are none).
+int loose_blocks(const mpmw_solver *s) {
+    const int nl = s->sort.loose_end - s->sort.loose_begin;
+    return nl > 0 ? blocks_for(nl) : 0;
+}
+
+// One particle pass: the tiles, then the loose particles.  The backstress
+// arrays select the kernel variant; the deterministic flag the P2G form.
+// Statistics partials: tiles first, then the loose blocks.
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
+int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
+    const Particles<R> P = pview<R>(s);
+    const GridView<R> g = gview<R>(s);
+    const Grid<R> G = grid_arrays<R>(s);
+    const Material<R> M = material<R>(s);
+    SortState &S = s->sort;
+    const TileGeom tg = S.geom;
+    if (DET && DO_P2G)
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow + 1, (int64_t)1, 0);
+    hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, HAS_BACK, DET>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P,
+                       g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
+                       (R *)s->tpart);
+    HIP_TRY(hipGetLastError());
+    const int nbl = loose_blocks(s);
+    if (nbl > 0) {
+        hipLaunchKernelGGL((k_loose<R, DO_G2P, DO_P2G && !DET, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream,
+                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)tg.ntiles * kRed,
+                           S.loose_begin, S.loose_end);
+        if (DET && DO_P2G)
+            hipLaunchKernelGGL(k_loose_p2g_ordered<R>, dim3(1), dim3(kWave), 0, s->stream, P, g, G, S.loose_begin,
+                               S.loose_end);
+        HIP_TRY(hipGetLastError());
+    }
+    return MPMW_OK;
+}
+
 template <typename R, bool DO_G2P, bool DO_P2G>
-void launch_tile(mpmw_solver *s, const Particles<R> &P, const GridView<R> &g, const Grid<R> &G,
-                 const Material<R> &M, double dt, double zcut, G2PFlags fl) {
-    const TileGeom tg = s->sort.geom;
-    if (P.a0)
-        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, true>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P, g, G,
-                           M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
-    else
-        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, false>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P, g,
-                           G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow);
+int particle_pass(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
+    const bool back = s->has_back, det = s->deterministic;
+    if (back)
+        return det ? particle_pass_t<R, DO_G2P, DO_P2G, true, DO_P2G>(s, dt, zcut, fl)
+                   : particle_pass_t<R, DO_G2P, DO_P2G, true, false>(s, dt, zcut, fl);
+    return det ? particle_pass_t<R, DO_G2P, DO_P2G, false, DO_P2G>(s, dt, zcut, fl)
+               : particle_pass_t<R, DO_G2P, DO_P2G, false, false>(s, dt, zcut, fl);
+}
+
+// P2G alone into empty sums (first step, after a rebuild or a re-binning).
+template <typename R> int p2g_alone(mpmw_solver *s, double dt, double zcut) {
+    int rc;
+    if (!s->deterministic || loose_blocks(s) > 0) {
+        if ((rc = zero_acc<R>(s))) return rc;
+    }
+    G2PFlags fl;
+    fl.chip = 0;
+    fl.stats = 0;
+    Timed t(s, MPMW_K_P2G);
+    return particle_pass<R, false, true>(s, dt, zcut, fl);
+}
+
+// Deterministic mode: when a particle of the last P2G pass had left its
+// binned cell, re-bin the particles and repeat the P2G.  After a binning
+// every particle is in its cell or loose, so the repeat finds none.
+template <typename R> int det_settle(mpmw_solver *s, double dt, double zcut) {
+    SortState &S = s->sort;
+    HIP_TRY(hipMemcpyAsync(S.h_oow + 1, S.oow + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (S.h_oow[1] == 0) return MPMW_OK;
+    int rc;
+    if ((rc = sort_particles<R>(s))) return rc;
+    S.resorts++;
+    if ((rc = p2g_alone<R>(s, dt, zcut))) return rc;
+    HIP_TRY(hipMemcpyAsync(S.h_oow + 1, S.oow + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (S.h_oow[1] != 0)
+        return fail(MPMW_E_STATE, "deterministic P2G: " + std::to_string(S.h_oow[1]) +
+                                      " particles outside their cell right after binning");
+    return MPMW_OK;
 }
 
 // MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791.
@@ -621,26 +711,28 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     if (!s->sort.valid && (rc = sort_particles<R>(s))) return rc;
     const GridView<R> g = gview<R>(s);
     const Grid<R> G = grid_arrays<R>(s);
-    Particles<R> P = pview<R>(s);
     const Material<R> M = material<R>(s);
     const TileGeom tg = s->sort.geom;
-    const int nb = tg.ntiles;
+    const int nb = tg.ntiles + loose_blocks(s);
     if ((rc = ensure_part(s, nb))) return rc;
     const double zcut = s->tip[2] - s->h * 1.5;
-    G2PFlags fl;
-    fl.chip = (s->step % 2 == 0);
-    fl.stats = 0;
+    const bool det = s->deterministic;
     if (!s->p2g_ready) {
-        if ((rc = zero_acc<R>(s))) return rc;
-        Timed t(s, MPMW_K_P2G);
-        launch_tile<R, false, true>(s, P, g, G, M, dt, zcut, fl);
-        HIP_TRY(hipGetLastError());
+        if ((rc = p2g_alone<R>(s, dt, zcut))) return rc;
+        if (det && (rc = det_settle<R>(s, dt, zcut))) return rc;
     }
+    // the binning may have changed in det_settle
+    const TileGeom tgs = s->sort.geom;
+    TileParts<R> tp;
+    tp.part = det ? (const R *)s->tpart : nullptr;
+    tp.tile_start = s->sort.tile_start;
+    tp.tg = tgs;
+    tp.loose = loose_blocks(s) > 0 ? 1 : 0;
     const bool diffusion = s->prm.specific_heat > 0.0 &&
                            s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
     {
         Timed t(s, MPMW_K_GRID);
-        hipLaunchKernelGGL(k_grid_update<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, M,
+        hipLaunchKernelGGL(k_grid_update<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, tp, M,
                            tool_view(s), (R)dt, diffusion ? 0 : 1, s->fpart);
         HIP_TRY(hipGetLastError());
     }
@@ -662,18 +754,19 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     // G2P + chip pass, fused with the next step's P2G unless the next step
     // rebuilds the grid (then the AABB and metrics are reduced instead).
     const bool next_rebuild = ((s->step + 1) % 25 == 0);
+    G2PFlags fl;
+    fl.chip = (s->step % 2 == 0);
     fl.stats = next_rebuild ? 1 : 0;
+    const int nbs = tgs.ntiles + loose_blocks(s);
+    if ((rc = ensure_part(s, nbs))) return rc;
     {
         Timed t(s, MPMW_K_G2P_P2G);
-        if (next_rebuild)
-            launch_tile<R, true, false>(s, P, g, G, M, dt, zcut, fl);
-        else
-            launch_tile<R, true, true>(s, P, g, G, M, dt, zcut, fl);
-        HIP_TRY(hipGetLastError());
+        rc = next_rebuild ? particle_pass<R, true, false>(s, dt, zcut, fl) : particle_pass<R, true, true>(s, dt, zcut, fl);
+        if (rc) return rc;
     }
     if (fl.stats) {
         Timed t(s, MPMW_K_REDUCE);
-        if ((rc = finalize_stats(s, nb))) return rc;
+        if ((rc = finalize_stats(s, nbs))) return rc;
         s->stats_fresh = true;
         s->metrics_pending = true;
     } else {
@@ -686,6 +779,8 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     s->pose.angle += s->pose.omega * dt;
     s->time += dt;
     ++s->step;
+    // the next step's P2G: re-bin and repeat it if a particle changed cell
+    if (det && s->p2g_ready && (rc = det_settle<R>(s, dt, zcut))) return rc;
     if (s->step % 1000 == 0) return compact_impl<R>(s);  // :788-790
     return MPMW_OK;
 }
@@ -742,6 +837,7 @@ template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm,
     for (int k = 0; k < NF; ++k) std::swap(s->f[k], s->alt[k]);
     s->n = count;
     s->p2g_ready = false;
+    s->sort.valid = false;   // a binning sets it again after its permutation
     return MPMW_OK;
 }
 
@@ -1042,6 +1138,7 @@ int mpmw_destroy(mpmw_solver *s) {
     q = s->d_near_out; dfree(s, q, 0);
     q = s->d_near_out_i; dfree(s, q, 0);
     dfree(s, s->samples, 0);
+    dfree(s, s->tpart, 0);
     free_sort_state(s);
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->h_cf) (void)hipHostFree(s->h_cf);
@@ -1432,6 +1529,7 @@ int mpmw_set_deterministic(mpmw_solver *s, int32_t on) {
     CHECK_S(s);
     s->deterministic = on != 0;
     s->p2g_ready = false;
+    s->sort.valid = false;
     return MPMW_OK;
 }
 
@@ -1526,6 +1624,9 @@ extern "C" int mpmw_get_diagnostics(mpmw_solver *s, mpmw_diagnostics *d) {
         HIP_TRY(hipMemcpy(&v, S.oow, sizeof(int32_t), hipMemcpyDeviceToHost));
         d->out_of_window = v;
     }
+    d->loose_particles = S.valid ? S.loose_end - S.loose_begin : 0;
+    d->deterministic = s->deterministic ? 1 : 0;
+    d->det_rebins = S.resorts;
     if (S.valid && S.tile_start && S.geom.ntiles > 0) {
         std::vector<int32_t> ts((size_t)S.geom.ntiles + 1);
         HIP_TRY(hipMemcpy(ts.data(), S.tile_start, sizeof(int32_t) * ts.size(), hipMemcpyDeviceToHost));

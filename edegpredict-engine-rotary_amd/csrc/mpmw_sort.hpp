@@ -1,10 +1,17 @@
 // mpmw_sort.hpp -- particle binning into cell tiles.
 //
-// Particles are kept ordered by tile (a block of TX x TY x TZ cells starting at
-// the AABB-min cell) and, inside a tile, slot-major: the first particle of
-// every cell, then the second of every cell, and so on.  A 64-lane wave then
-// holds particles of 64 different cells, so its LDS scatter for one corner
-// offset touches 64 different nodes, while global loads stay contiguous.
+// Particles are kept ordered by tile (a block of TX x TY x TZ cells; tiles
+// start at cell kCellOffset, one cell inside the AABB-min cell, so a particle
+// on the AABB face that rounds into the cell below stays in a tile) and,
+// inside a tile, slot-major: the first particle of every cell, then the
+// second of every cell, and so on.  A 64-lane wave then holds particles of 64
+// different cells, so its LDS scatter for one corner offset touches 64
+// different nodes, while global loads stay contiguous.
+//
+// Key space: tile-major cell keys [0, nkeys), then key nkeys for particles
+// whose cell is outside the tiles or has a corner node outside the grid
+// ("loose" particles, handled by a per-particle pass), then key nkeys + 1 for
+// INACTIVE particles.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,7 +23,7 @@ constexpr int TX = 8, TY = 8, TZ = 4;                 // cells per tile
 constexpr int kTileCells = TX * TY * TZ;              // 256 = one cell per thread
 constexpr int WX = TX + 1, WY = TY + 1, WZ = TZ + 1;  // node window of a tile
 constexpr int WN = WX * WY * WZ;                      // 405 nodes
-constexpr int kCellOffset = 4;                        // the AABB-min cell (pad = 4h)
+constexpr int kCellOffset = 3;                        // first tiled cell (the AABB-min cell is 4: pad = 4h)
 
 struct ScratchBuf {
     void *ptr = nullptr;
@@ -45,14 +52,19 @@ struct SortState {
     int32_t key_cap = 0, n_cap = 0;
     uint32_t *keys = nullptr, *keys_sorted = nullptr;
     int32_t *idx = nullptr, *idx_sorted = nullptr, *perm = nullptr;
-    int32_t *counts = nullptr;         // nkeys + 1
-    int32_t *cell_start = nullptr;     // nkeys + 1 (exclusive scan of counts)
+    int32_t *counts = nullptr;         // nkeys + 2
+    int32_t *cell_start = nullptr;     // nkeys + 2 (exclusive scan of counts)
     int32_t *tile_start = nullptr;     // ntiles + 1
     int32_t tile_cap = 0;
-    int32_t *oow = nullptr;            // particles processed outside their tile window (device counter)
-    int32_t *h_oow = nullptr;          // pinned copy
+    // device counters: [0] particle visits outside their tile window since
+    // the last rebuild, [1] particles of the last deterministic P2G pass that
+    // were not in their binned cell
+    int32_t *oow = nullptr;
+    int32_t *h_oow = nullptr;          // pinned: [0, 2) copy of the counters, [2, 4) the loose range
+    int32_t loose_begin = 0, loose_end = 0;   // particles binned as loose (host copy)
     mpmw::ScratchBuf temp, temp2;
     double origin[3] = {0, 0, 0};      // grid origin the particles were binned on
     int64_t steps_since = 0;           // steps since the out-of-window counter was reset
     int64_t sorts = 0;
+    int64_t resorts = 0;               // re-binnings forced by the deterministic P2G
 };

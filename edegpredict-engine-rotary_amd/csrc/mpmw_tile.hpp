@@ -239,6 +239,23 @@ __device__ __forceinline__ void scatter_exception(const GridView<R> &g, R *acc, 
     }
 }
 
+// The fields a P2G-only pass reads (mpmP2GKernel :103-133): velocity,
+// temperature, the stress of ACTIVE / FIXED_BOUNDARY particles and the
+// external force.
+template <typename R>
+__device__ __forceinline__ void load_p2g_state(const Particles<R> &P, int i, PState<R> &p, R &fex, R &fey, R &fez) {
+    p.vx = P.vx[i]; p.vy = P.vy[i]; p.vz = P.vz[i];
+    p.T = P.T[i];
+    if (p.status == kActive || p.status == kFixed) {
+        p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
+        p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p.s[c] = R(0);
+    }
+    if (P.fx) { fex = P.fx[i]; fey = P.fy[i]; fez = P.fz[i]; }
+}
+
 template <typename R> __device__ __forceinline__ R shfl_from(R v, int src) { return __shfl(v, src, kWave); }
 
 // Combine the cell accumulators of a wave (an 8 x 8 layer of cells) into
@@ -282,11 +299,17 @@ __device__ __forceinline__ void combine_layer(const CellAcc<R> &A, R *s_acc, int
 // walks that cell's particles slot by slot (the memory order is slot-major,
 // so a full tile reads contiguously).  DO_G2P: gather, constitutive update
 // and chip pass.  DO_P2G: scatter into the next step's accumulators.
-template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+// DET (deterministic P2G): the tile's window is stored whole into its slot
+// of `tpart` (8 channels x WN nodes) instead of being added to device memory,
+// and a particle outside its binned cell is only counted in counters[1]; the
+// caller re-bins and repeats the P2G when that count is not zero.  Every sum
+// then has a fixed order: per cell in slot order, across a wave by the
+// fixed shuffle tree, across tiles in tile order (k_grid_update).
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
 __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
                                                  double dtd, double zcut, G2PFlags flags, double ref_temp,
                                                  double *part, const int32_t *cell_start, TileGeom tg,
-                                                 int32_t *oow) {
+                                                 int32_t *counters, R *tpart) {
     static_assert(TX == 8 && TY == 8 && TZ * TX * TY == kBlock, "one 8x8 cell layer per wave");
     constexpr int kNodeBytes = DO_G2P ? WN * (int)sizeof(vec4_t<R>) : 0;
     constexpr int kAccBytes = DO_P2G ? 8 * WN * (int)sizeof(R) : 0;
@@ -349,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
 #pragma unroll
                 for (int c = 0; c < 8; ++c) A.v[ch][c] = R(0);
         }
-        int my_oow = 0;
+        int my_oow = 0, my_exc = 0;
         const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int base = 0;
         for (int s = 0; s < maxc; ++s) {
@@ -397,16 +420,7 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
                                      (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
                 }
             } else {
-                p.vx = P.vx[i]; p.vy = P.vy[i]; p.vz = P.vz[i];
-                p.T = P.T[i];
-                if (p.status == kActive || p.status == kFixed) {
-                    p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
-                    p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) p.s[c] = R(0);
-                }
-                if (P.fx) { fex = P.fx[i]; fey = P.fy[i]; fez = P.fz[i]; }
+                load_p2g_state(P, i, p, fex, fey, fez);
             }
             if (DO_P2G) {
                 // after G2P the external force has been consumed (:399)
@@ -416,6 +430,8 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
                 if (home_ok && f.b[0] == hgx && f.b[1] == hgy && f.b[2] == hgz) {
                     cell_accumulate(A, f, g.inv_h, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex, fey, fez,
                                     stress_force, ext_only);
+                } else if (DET) {
+                    ++my_exc;
                 } else {
                     ++my_oow;
                     scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex,
@@ -429,49 +445,149 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
             __syncthreads();  // layer z written by wave z only
             combine_layer(A, s_acc, hx, hy, hz, 1);
             __syncthreads();  // layer z+1 written by wave z only
-            for (int w = tid; w < WN; w += kBlock) {
-                const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
-                if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
-                R *a = G.acc + ((int64_t)k * g.ny + j) * g.nx + i;
+            if (DET) {
+                R *dst = tpart + (int64_t)t * (8 * WN);
+                for (int w = tid; w < 8 * WN; w += kBlock) dst[w] = s_acc[w];
+            } else {
+                for (int w = tid; w < WN; w += kBlock) {
+                    const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
+                    if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
+                    R *a = G.acc + ((int64_t)k * g.ny + j) * g.nx + i;
 #pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const R v = s_acc[c * WN + w];
-                    if (v != R(0)) unsafeAtomicAdd(a + c * g.nn, v);
+                    for (int c = 0; c < 8; ++c) {
+                        const R v = s_acc[c * WN + w];
+                        if (v != R(0)) unsafeAtomicAdd(a + c * g.nn, v);
+                    }
                 }
             }
         }
-        int cntv = my_oow;
+        int cntv = my_oow, cnte = my_exc;
 #pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) cntv += __shfl_xor(cntv, o, kWave);
-        if (lane == 0 && cntv) atomicAdd(oow, cntv);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            cntv += __shfl_xor(cntv, o, kWave);
+            cnte += __shfl_xor(cnte, o, kWave);
+        }
+        if (lane == 0 && cntv) atomicAdd(counters, cntv);
+        if (DET && lane == 0 && cnte) atomicAdd(counters + 1, cnte);
     }
     if (DO_G2P && flags.stats) block_reduce_stats(st, part);
 }
 
+// ------------------------------------------------------ loose particles --
+
+// The particles binned as loose (cell outside the tiles, or a stencil node
+// outside the grid), one thread each, nodes read from device memory.  P2G
+// here is the atomic mode's (device atomics into the node accumulators); the
+// deterministic mode runs k_loose_p2g_ordered after this pass instead.
+// Statistics partials go to part[blockIdx.x].
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+__global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
+                                                  double dtd, double zcut, G2PFlags flags, double ref_temp,
+                                                  double *part, int32_t begin, int32_t end) {
+    double st[kRed];
+    stats_identity(st, ref_temp);
+    const int i = begin + (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    if (i < end) {
+        PState<R> p;
+        p.status = P.status[i];
+        if (p.status != kInactive) {
+            const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
+            R fex = R(0), fey = R(0), fez = R(0);
+            p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+            if (DO_G2P) {
+                const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                g2p_particle<R, HAS_BACK>(P, i, p, f, g, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut,
+                                          [&](int, int64_t idx) -> vec4_t<R> { return G.node[idx]; });
+                if (flags.stats) {
+                    const double vx = p.vx, vy = p.vy, vz = p.vz;
+                    accumulate_stats(st, p.x, p.y, p.z,
+                                     (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]),
+                                     (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
+                }
+            } else {
+                load_p2g_state(P, i, p, fex, fey, fez);
+            }
+            if (DO_P2G) {
+                const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                scatter_exception(g, G.acc, (R *)nullptr, f, 0, 0, 0, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex,
+                                  fey, fez, p.status == kActive || p.status == kFixed, p.status == kChip);
+            }
+        }
+    }
+    if (DO_G2P && flags.stats) block_reduce_stats(st, part);
+}
+
+// Deterministic P2G of the loose particles: one wave walks them in memory
+// order, lane 8 * c + ch adding channel ch of corner c to the node
+// accumulators.  One particle's adds go to distinct addresses; the barrier
+// after each particle (workgroup-scope release + acquire) orders them before
+// the next particle's, so every node sums its loose particles in index order.
+template <typename R>
+__global__ void __launch_bounds__(kWave) k_loose_p2g_ordered(Particles<R> P, GridView<R> g, Grid<R> G,
+                                                             int32_t begin, int32_t end) {
+    const int n = (int)threadIdx.x >> 3, ch = (int)threadIdx.x & 7;
+    for (int i = begin; i < end; ++i) {
+        PState<R> p;
+        p.status = P.status[i];
+        if (p.status != kInactive) {
+            const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
+            R fex = R(0), fey = R(0), fez = R(0);
+            load_p2g_state(P, i, p, fex, fey, fez);
+            const Frac<R> f = base_and_frac(g, P.x[i], P.y[i], P.z[i]);
+            int64_t idx;
+            R w, gx, gy, gz;
+            if (corner(g, f, n, idx, w, gx, gy, gz)) {
+                R c[8];
+                const int nch = p2g_corner(mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex, fey, fez,
+                                           p.status == kActive || p.status == kFixed, p.status == kChip, w, gx, gy,
+                                           gz, c);
+                R v = c[0];
+#pragma unroll
+                for (int k = 1; k < 8; ++k) v = (ch == k) ? c[k] : v;
+                if (nch == 8 || ch < 4 || ch == 7) {
+                    R *a = G.acc + (int64_t)ch * g.nn + idx;
+                    *a = *a + v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- binning ----
 
-// Tile-major cell key of every particle on the current grid; INACTIVE
-// particles get key nkeys and sort to the end.
+// Tile-major cell key of every particle on the current grid.  A particle is
+// loose (key nkeys) when its cell (the floor used by the P2G stencil) lies
+// outside the tiles or has a corner node outside the grid; INACTIVE
+// particles get key nkeys + 1 and sort to the end.
 template <typename R>
 __global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, uint32_t *keys, int32_t *idx,
                            int32_t *counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     idx[i] = i;
-    uint32_t key = (uint32_t)tg.nkeys;
+    uint32_t key = (uint32_t)tg.nkeys + 1u;
     if (P.status[i] != kInactive) {
         const double r[3] = {(P.x[i] - g.ox) / g.hd, (P.y[i] - g.oy) / g.hd, (P.z[i] - g.oz) / g.hd};
         const int nt[3] = {tg.ntx * TX, tg.nty * TY, tg.ntz * TZ};
+        const int nodes[3] = {g.nx, g.ny, g.nz};
         int c[3];
+        bool tiled = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            double fl = floor(r[a]) - kCellOffset;
-            fl = fmin(fmax(fl, 0.0), (double)(nt[a] - 1));
-            c[a] = (int)fl;
+            const double fl = floor(r[a]);
+            // in a tile, and both stencil nodes inside the grid (NaN fails too)
+            tiled = tiled && fl >= (double)kCellOffset && fl < (double)(kCellOffset + nt[a]) &&
+                    fl + 1.0 < (double)nodes[a];
+            c[a] = tiled ? (int)fl - kCellOffset : 0;
         }
-        const uint32_t tile = (uint32_t)(((c[2] / TZ) * tg.nty + c[1] / TY) * tg.ntx + c[0] / TX);
-        const uint32_t local = (uint32_t)(((c[2] % TZ) * TY + c[1] % TY) * TX + c[0] % TX);
-        key = tile * kTileCells + local;
+        if (tiled) {
+            const uint32_t tile = (uint32_t)(((c[2] / TZ) * tg.nty + c[1] / TY) * tg.ntx + c[0] / TX);
+            const uint32_t local = (uint32_t)(((c[2] % TZ) * TY + c[1] % TY) * TX + c[0] % TX);
+            key = tile * kTileCells + local;
+        } else {
+            key = (uint32_t)tg.nkeys;
+        }
     }
     keys[i] = key;
     atomicAdd(&counts[key], 1);

@@ -85,6 +85,7 @@ class KernelTimes(C.Structure):
 class Diagnostics(C.Structure):
     _fields_ = [("sorts", C.c_int64), ("out_of_window", C.c_int64), ("steps_since_rebuild", C.c_int64),
                 ("tiles", C.c_int32), ("tile_cells", C.c_int32), ("max_tile_particles", C.c_int32),
+                ("loose_particles", C.c_int32), ("det_rebins", C.c_int64), ("deterministic", C.c_int32),
                 ("pad_", C.c_int32)]
 
 

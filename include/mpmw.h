@@ -202,8 +202,14 @@ int mpmw_temperature_at(mpmw_solver *s, double x, double y, double z, double *t)
 int mpmw_apply_heat_flux(mpmw_solver *s, double x, double y, double z, double q);
 int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t count);
 
-/* --- inspection (parity / measurement) */
+/* Deterministic mode (no counterpart in the reference, whose FP64 atomicAdd
+ * P2G sums in arbitrary order, MPMWorkpieceSolver.cu:114-131): every P2G sum
+ * is formed in a fixed order, without float atomics, so reruns from the same
+ * input agree in every bit.  Particles are re-binned whenever one has changed
+ * cell since the last binning.  Takes effect at the next step. */
 int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
+
+/* --- inspection (parity / measurement) */
 int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);
 /* Keep a copy of each step's post-P2G grid (mass, momentum, force, T*m). */
 int mpmw_set_grid_snapshot(mpmw_solver *s, int32_t on);
@@ -223,6 +229,9 @@ typedef struct mpmw_diagnostics {
     int64_t steps_since_rebuild;
     int32_t tiles, tile_cells; /* tiles of the current binning, cells per tile */
     int32_t max_tile_particles;
+    int32_t loose_particles;   /* particles of the current binning outside every tile */
+    int64_t det_rebins;        /* re-binnings the deterministic P2G forced (a particle changed cell) */
+    int32_t deterministic;     /* 1 when the deterministic P2G is on */
     int32_t pad_;
 } mpmw_diagnostics;
 int mpmw_get_diagnostics(mpmw_solver *s, mpmw_diagnostics *d);

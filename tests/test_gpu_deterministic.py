@@ -1,0 +1,151 @@
+"""Deterministic mode: bit-exact reruns and oracle parity.
+
+The deterministic P2G replaces every float atomic of the step by a
+fixed-order sum (per cell in slot order, across a wave by a fixed shuffle
+tree, across tiles in tile order, loose particles in index order), so two runs
+from the same input must agree in every bit.  The same tolerances against the
+FP64 oracle as the atomic mode apply (1e-9 FP64; 1e-5 FP32 on the gated
+fields).
+
+Motion cases: a slab that translates fast enough that particles change cells
+between binnings (the deterministic P2G then re-bins), and one fast enough
+that particles leave the tiles and the grid between rebuilds (loose
+particles, processed by the per-particle pass).
+"""
+import numpy as np
+import pytest
+
+import parity
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+GATED = ("pos", "temperature", "plastic_strain", "grid_mass", "grid_momentum")
+
+
+def _report(tag, res):
+    e = res["errors"]
+    print(f"\n[{tag}] n={res['n']} status_mismatch={res['status_mismatch']} "
+          + " ".join(f"{k}={v:.2e}" for k, v in sorted(e.items())), flush=True)
+
+
+def _bits(d):
+    return {k: np.ascontiguousarray(v).tobytes() for k, v in d.items() if v is not None}
+
+
+def _run(cfg, precision, nsteps, deterministic, watch=None):
+    s = parity.build_gpu(cfg, precision, deterministic)
+    s.set_grid_snapshot(True)
+    seen = []
+    for _ in range(nsteps):
+        s.step(cfg["dt"])
+        if watch:
+            seen.append(watch(s))
+    out = (_bits(s.get_particles()), _bits(s.grid_snapshot()), s.metrics(), s.diagnostics(), seen)
+    s.close()
+    return out
+
+
+def _assert_same(a, b, what):
+    pa, ga, ma = a[0], a[1], a[2]
+    pb, gb, mb = b[0], b[1], b[2]
+    for k in pa:
+        assert pa[k] == pb[k], f"{what}: particle field {k} differs between runs"
+    for k in ga:
+        assert ga[k] == gb[k], f"{what}: grid channel {k} differs between runs"
+    for f in ("max_stress", "max_temperature", "kinetic_energy", "contact_force"):
+        assert getattr(ma, f) == getattr(mb, f), (what, f)
+
+
+def translating_slab(cells_per_step, steps):
+    """8x8x4-cell slab, no tool, translating at `cells_per_step` cells per
+    step along x (and half that along -y); tip far below (no chip pass)."""
+    cfg = workloads.scaled(workloads.slab_config0(), (8, 8, 4), steps=steps,
+                           name=f"translating_slab_{cells_per_step}")
+    cfg = dict(cfg)
+    v = cells_per_step * workloads.H / cfg["dt"]
+    cfg["vel"] = (v, -0.5 * v, 0.0)
+    cfg["tool"] = ("none",)
+    cfg["tip"] = (0.0, 0.0, -1.0)
+    return cfg
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_det_config0_rerun_bit_exact(prec):
+    cfg = workloads.slab_config0()
+    a = _run(cfg, prec, 60, True)
+    b = _run(cfg, prec, 60, True)
+    print(f"\n[config0 det {prec}] diagnostics {a[3]}", flush=True)
+    _assert_same(a, b, f"config0 {prec}")
+
+
+def test_det_small_matches_oracle():
+    for prec, tol in (("fp64", 1e-9), ("fp32", 1e-5)):
+        res = parity.run_case(parity.small_case(), prec, nsteps=30, deterministic=True)
+        _report(f"small det {prec}", res)
+        assert res["status_mismatch"] == 0 and res["grid_same"]
+        for k in GATED:
+            assert res["errors"][k] <= tol, (prec, k, res["errors"][k])
+
+
+@pytest.fixture(scope="module")
+def config0_states():
+    cfg = dict(workloads.slab_config0())
+    cfg["steps"] = 61
+    states, _ = parity.oracle_trajectory(cfg, {0, 24, 60})
+    return cfg, states
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_det_config0_step_parity(config0_states, prec, tol):
+    cfg, states = config0_states
+    for k, st in sorted(states.items()):
+        res = parity.run_from_state(cfg, prec, st, 2, deterministic=True)
+        _report(f"config0 det {prec} from step {k}", res)
+        assert res["status_mismatch"] == 0, k
+        for f in GATED:
+            assert res["errors"][f] <= tol, (k, f, res["errors"][f])
+
+
+@pytest.mark.parametrize("deterministic", [True, False])
+def test_cell_changes_match_oracle(deterministic):
+    """0.07 cells per step: particles change cells between binnings."""
+    cfg = translating_slab(0.07, 60)
+    for prec, tol in (("fp64", 1e-9), ("fp32", 1e-5)):
+        res = parity.run_case(cfg, prec, deterministic=deterministic)
+        d = res["solver"].diagnostics()
+        _report(f"translating det={deterministic} {prec} rebins={d['det_rebins']} sorts={d['sorts']}", res)
+        assert res["status_mismatch"] == 0 and res["grid_same"]
+        for k in GATED:
+            assert res["errors"][k] <= tol, (prec, k, res["errors"][k])
+        if deterministic:
+            assert d["det_rebins"] > 0
+
+
+def test_cell_changes_det_rerun_bit_exact():
+    cfg = translating_slab(0.07, 60)
+    a = _run(cfg, "fp32", 60, True)
+    b = _run(cfg, "fp32", 60, True)
+    assert a[3]["det_rebins"] > 0
+    _assert_same(a, b, "translating fp32")
+
+
+@pytest.mark.parametrize("deterministic", [True, False])
+def test_loose_particles_match_oracle(deterministic):
+    """0.2 cells per step: between rebuilds particles leave the tiles and
+    the grid (loose particles; outside the grid G2P finds no node)."""
+    cfg = translating_slab(0.2, 45)
+    res = parity.run_case(cfg, "fp64", deterministic=deterministic)
+    _report(f"loose det={deterministic} fp64", res)
+    assert res["status_mismatch"] == 0 and res["grid_same"]
+    for k in GATED:
+        assert res["errors"][k] <= 1e-9, (k, res["errors"][k])
+
+
+def test_loose_particles_det_rerun_bit_exact():
+    cfg = translating_slab(0.2, 45)
+    a = _run(cfg, "fp32", 45, True, watch=lambda s: s.diagnostics()["loose_particles"])
+    b = _run(cfg, "fp32", 45, True)
+    print(f"\n[loose det] loose particles per step {a[4]}", flush=True)
+    assert max(a[4]) > 0, "the case must bin some particles as loose"
+    _assert_same(a, b, "loose fp32")
