@@ -96,12 +96,68 @@ def cpu_baseline(cfg, budget_s, threads):
     }
 
 
+def run_mode(cfg, mode, precision, steps, warmup):
+    """Set up the workload in one solver and time `steps` steps after
+    `warmup` untimed ones; HIP events on the solver's stream bracket the
+    timed region, library events time each kernel class."""
+    P = dict(cfg["params"], precision=mpmw.FP64 if precision == "fp64" else mpmw.FP32,
+             deterministic=1 if mode == "deterministic" else 0)
+    s = mpmw.Solver(**P)
+    t0 = time.time()
+    workloads.apply_to_solver(mpmw, s, cfg)
+    setup_s = time.time() - t0
+    n = s.count()
+    dt = cfg["dt"]
+    for _ in range(warmup):
+        s.step(dt)
+    s.synchronize()
+    ev = HipEvents()
+    stream = s.stream()
+    a, b = ev.create(), ev.create()
+    s.set_kernel_timing(True)
+    s.synchronize()
+    t_wall = time.perf_counter()
+    ev.record(a, stream)
+    for _ in range(steps):
+        s.step(dt)
+    ev.record(b, stream)
+    s.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms_total = ev.elapsed_ms(a, b)
+    kms, kn = s.kernel_times()
+    s.set_kernel_timing(False)
+    diag = s.diagnostics()
+    g = s.grid_info()
+    s.close()
+    names = ["g2p_p2g", "p2g", "grid_update", "diffuse", "sort", "reduce", "other"]
+    return {
+        "n": n, "ms_total": ms_total, "wall": wall, "setup_s": setup_s, "diag": diag,
+        "grid_nodes": g.nx * g.ny * g.nz,
+        "k_avg_ms": kms[mpmw.K_G2P_P2G] / max(kn[mpmw.K_G2P_P2G], 1),
+        "kernels": {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]},
+    }
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused tile kernel on this workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_tile_drill_256x128x64_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=25)
-    ap.add_argument("--mode", choices=["atomic", "deterministic"], default="atomic")
+    ap.add_argument("--modes", type=str, default="atomic,deterministic",
+                    help="P2G modes to time; the first one gives `value`")
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--cells", type=str, default="256,128,64")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -111,53 +167,27 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    modes = [m.strip() for m in args.modes.split(",") if m.strip()]
+    for m in modes:
+        if m not in ("atomic", "deterministic"):
+            ap.error(f"unknown mode {m}")
 
     cells = tuple(int(x) for x in args.cells.split(","))
     cfg = workloads.drill_config1()
     if cells != cfg["cells"]:
         cfg = workloads.scaled(cfg, cells)
-    P = dict(cfg["params"], precision=mpmw.FP64 if args.precision == "fp64" else mpmw.FP32,
-             deterministic=1 if args.mode == "deterministic" else 0)
-    s = mpmw.Solver(**P)
-    t0 = time.time()
-    workloads.apply_to_solver(mpmw, s, cfg)
-    setup_s = time.time() - t0
-    n = s.count()
-    dt = cfg["dt"]
-    for _ in range(args.warmup):
-        s.step(dt)
-    s.synchronize()
-
-    ev = HipEvents()
-    stream = s.stream()
-    a, b = ev.create(), ev.create()
-    s.set_kernel_timing(True)
-    s.synchronize()
-    t_wall = time.perf_counter()
-    ev.record(a, stream)
-    for _ in range(args.steps):
-        s.step(dt)
-    ev.record(b, stream)
-    s.synchronize()
-    wall = time.perf_counter() - t_wall
-    ms_total = ev.elapsed_ms(a, b)
-    kms, kn = s.kernel_times()
-    s.set_kernel_timing(False)
-    diag = s.diagnostics()
-
-    ms_step = ms_total / args.steps
-    pps = n * args.steps / (ms_total / 1e3)
+    runs = {m: run_mode(cfg, m, args.precision, args.steps, args.warmup) for m in modes}
+    r = runs[modes[0]]
+    n = r["n"]
+    ms_step = r["ms_total"] / args.steps
+    pps = n * args.steps / (r["ms_total"] / 1e3)
     nodes = occupied_nodes(cells)
-    g = s.grid_info()
 
     # dominant kernel: fused G2P + constitutive + chip + next-step P2G
-    k = mpmw.K_G2P_P2G
-    k_avg_ms = kms[k] / max(kn[k], 1)
+    k_avg_ms = r["k_avg_ms"]
     bytes_launch = B_PARTICLE * n + B_NODE_G2P * nodes
     achieved = bytes_launch / (k_avg_ms / 1e3) / 1e9
     b_alg = B_PARTICLE + B_NODE_STEP * nodes / n
-    names = ["g2p_p2g", "p2g", "grid_update", "diffuse", "sort", "reduce", "other"]
-    kernels = {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]}
 
     out = {
         "metric": "MPM particle-steps/s at 1 GPU",
@@ -174,29 +204,32 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": cfg["name"], "particles": n, "cells": list(cells),
-            "grid_nodes": g.nx * g.ny * g.nz, "occupied_nodes": nodes, "tool": "analytic two-flute drill",
-            "dt": dt, "mode": args.mode, "precision": args.precision,
+            "grid_nodes": r["grid_nodes"], "occupied_nodes": nodes, "tool": "analytic two-flute drill",
+            "dt": cfg["dt"], "mode": modes[0], "precision": args.precision,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_g2p (fused G2P + J-C return + chip pass + next-step P2G)",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+            "kernel": "k_tile<G2P,P2G> (fused G2P + J-C return + chip pass + next-step P2G)",
             "algorithmic_bytes_per_launch": bytes_launch,
             "avg_launch_ms": k_avg_ms,
             "step_level": {"B_alg_per_particle_step": b_alg,
                            "achieved_GBs": pps * b_alg / 1e9,
                            "frac": pps * b_alg / 1e9 / HBM_PEAK_GBS},
         },
-        "kernels": kernels,
-        "binning": diag,
-        "wall_s": wall,
-        "setup_s": setup_s,
+        "kernels": r["kernels"],
+        "binning": r["diag"],
+        "wall_s": r["wall"],
+        "setup_s": r["setup_s"],
     }
+    for m in modes[1:]:
+        q = runs[m]
+        out[m] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
+                  "fused_kernel_avg_ms": q["k_avg_ms"], "kernels": q["kernels"], "binning": q["diag"]}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
-    s.close()
 
 
 if __name__ == "__main__":
