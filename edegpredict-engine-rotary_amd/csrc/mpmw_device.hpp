@@ -23,13 +23,31 @@ template <typename R> using vec4_t = typename Vec4<R>::type;
 
 // Material constants after MPMWorkpieceSolver::initialize
 // (MPMWorkpieceSolver.cu:428-451), in kernel precision.
+// The divisors of the update that are material constants are passed with
+// their reciprocals (FP32 multiplies by the reciprocal, FP64 divides, see
+// divc), and flow_floor is a lower bound of johnsonCookStress over all
+// arguments (-inf when none is known), below which a trial stress is elastic
+// without evaluating the flow stress.
 template <typename R> struct Material {
     R shear, bulk, density, cp, melt, ref_temp;
     R jcA, jcB, jcN, jcC, jcM, ref_rate;
     R failure_strain, dth, eps_crit, chip_ratio;
     R rc, c_damp, dT_max, H;
+    R d_ref_rate, inv_ref_rate;   // max(ref_rate, 1e-9)
+    R d_t_span, inv_t_span;       // max(melt - ref_temp, 1)
+    R d_3g, inv_3g;               // max(3 shear, 1)
+    R d_rho_cp, inv_rho_cp;       // max(density cp, 1)
+    R d_fail, inv_fail;           // max(failure_strain, 1e-3)
+    R d_deg, inv_deg;             // dth 0.3
+    R flow_floor;
     int32_t clamp;
 };
+
+// a / d for a constant divisor d with reciprocal inv: a multiply in FP32 (one
+// more rounding, far below the FP32 tolerance), the division in FP64 (the
+// full-precision mode keeps the reference's operations).
+__device__ __forceinline__ float divc(float a, float d, float inv) { (void)d; return a * inv; }
+__device__ __forceinline__ double divc(double a, double d, double inv) { (void)inv; return a / d; }
 
 // MPMGridParams (MPMWorkpieceSolver.cuh:59-63) plus the integer fixture
 // threshold: nodes with k <= kfix satisfy the reference's FP64 test
@@ -38,6 +56,8 @@ template <typename R> struct Material {
 // mode, so the cell index and the weight fractions are formed in FP64.
 template <typename R> struct GridView {
     double ox, oy, oz, hd;
+    double inv_hd;    // 1 / h, correctly rounded
+    double guard;     // |r - round(r)| below which (x - o) * inv_hd may floor differently from (x - o) / h
     R inv_h;
     int32_t nx, ny, nz, kfix;
     int64_t nn;
@@ -99,13 +119,19 @@ __device__ __forceinline__ R von_mises(R xx, R yy, R zz, R xy, R xz, R yz) {
 }
 
 // johnsonCookStress, MPMWorkpieceSolver.cu:43-50
+// pow(x, y) for x >= 0, with the common x == 0, y > 0 case (no plastic
+// strain yet, room temperature) answered without the library call.
+template <typename R> __device__ __forceinline__ R pow_nonneg(R x, R y) {
+    return (x == R(0) && y > R(0)) ? R(0) : pow(x, y);
+}
+
 template <typename R>
 __device__ __forceinline__ R jc_flow(const Material<R> &M, R eps, R rate, R T) {
-    R strainTerm = M.jcA + M.jcB * pow(rmax(eps, R(0)), M.jcN);
-    R rateRatio = rmax(rate / rmax(M.ref_rate, R(1e-9)), R(1));
+    R strainTerm = M.jcA + M.jcB * pow_nonneg(rmax(eps, R(0)), M.jcN);
+    R rateRatio = rmax(divc(rate, M.d_ref_rate, M.inv_ref_rate), R(1));
     R rateTerm = R(1) + M.jcC * log(rateRatio);
-    R tStar = rmax(R(0), rmin(R(1), (T - M.ref_temp) / rmax(M.melt - M.ref_temp, R(1))));
-    R thermalTerm = rmax(R(0.01), R(1) - pow(tStar, M.jcM));
+    R tStar = rmax(R(0), rmin(R(1), divc(T - M.ref_temp, M.d_t_span, M.inv_t_span)));
+    R thermalTerm = rmax(R(0.01), R(1) - pow_nonneg(tStar, M.jcM));
     return strainTerm * rateTerm * thermalTerm;
 }
 
@@ -117,15 +143,29 @@ template <typename R> struct Frac {
     R w0[3], w1[3];
 };
 
+//
+// FP32 mode: r = (x - o) * inv_hd is within two ulps of the correctly rounded
+// quotient (x - o) / h, so its floor can differ from the quotient's only when
+// r lies within g.guard of an integer; there the quotient is formed by
+// division.  The base cell is therefore always floor((x - o) / h) in IEEE
+// FP64, the reference's (and the binning's) decision; the fraction differs
+// from the divided one by at most two ulps of r before its rounding to FP32.
+// FP64 mode divides.
 template <typename R>
 __device__ __forceinline__ Frac<R> base_and_frac(const GridView<R> &g, double x, double y, double z) {
     Frac<R> f;
-    const double r[3] = {(x - g.ox) / g.hd, (y - g.oy) / g.hd, (z - g.oz) / g.hd};
+    const double d[3] = {x - g.ox, y - g.oy, z - g.oz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const double fl = floor(r[a]);
+        double r = sizeof(R) == 4 ? d[a] * g.inv_hd : d[a] / g.hd;
+        double fl = floor(r);
+        double t = r - fl;
+        if (sizeof(R) == 4 && !(t > g.guard && t < 1.0 - g.guard)) {
+            r = d[a] / g.hd;
+            fl = floor(r);
+            t = r - fl;
+        }
         f.b[a] = (int)fl;
-        const double t = r[a] - (double)f.b[a];
         f.w1[a] = (R)t;
         f.w0[a] = (R)(1.0 - t);
     }
@@ -191,6 +231,21 @@ __device__ __forceinline__ double tool_phi(const Tool &t, double x, double y, do
     return phi_local(t, q);
 }
 
+// True when the drill's phi at (x, y, z) is certainly >= band (no contact),
+// from lower bounds of phi_local: phi >= body >= r - radius, >= q_z -
+// flute_length, and below the tip plane (q_z < 0) phi >= cone >= -q_z
+// sin(point/2) (the cone is sqrt(r^2 + q_z^2) >= |q_z| or r cos - q_z sin
+// with r cos >= 0).  A 1e-9 m margin keeps the decision clear of rounding.
+__device__ __forceinline__ bool drill_far(const Tool &t, double x, double y, double z) {
+    double q[3];
+    tool_frame(t, x, y, z, q);
+    const double m = t.band + 1e-9;
+    const double rr = t.radius + m;
+    if (rr <= 0.0 || q[0] * q[0] + q[1] * q[1] >= rr * rr) return true;
+    if (q[2] - t.flute_length >= m) return true;
+    return -q[2] * t.sin_half_point >= m;
+}
+
 // ---------------------------------------------------------- constitutive --
 
 // Register copy of one particle's mutable state.  `sres` holds the largest
@@ -213,7 +268,8 @@ __device__ __forceinline__ void constitutive_active(const Material<R> &M, PState
     R Dxz = R(0.5) * (L[2] + L[6]);
     R Dyz = R(0.5) * (L[5] + L[7]);
     R tr = Dxx + Dyy + Dzz;
-    R devXX = Dxx - tr / R(3), devYY = Dyy - tr / R(3), devZZ = Dzz - tr / R(3);
+    const R tr3 = divc(tr, R(3), R(1) / R(3));
+    R devXX = Dxx - tr3, devYY = Dyy - tr3, devZZ = Dzz - tr3;
     R eqRate = sqrt(R(2) / R(3) *
                     (devXX * devXX + devYY * devYY + devZZ * devZZ + R(2) * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz)));
     p.rate = eqRate;
@@ -228,25 +284,26 @@ __device__ __forceinline__ void constitutive_active(const Material<R> &M, PState
 #pragma unroll
     for (int c = 0; c < 6; ++c) sh[c] = HAS_BACK ? p.s[c] - p.a[c] : p.s[c];
     R seq = von_mises(sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]);
-    R flow = jc_flow(M, p.eps, p.rate, p.T);
-    if (seq > flow && seq > R(1e-9)) {
+    // seq <= flow_floor <= flow: elastic, the flow stress is not needed
+    const R flow = (seq > M.flow_floor && seq > R(1e-9)) ? jc_flow(M, p.eps, p.rate, p.T) : R(0);
+    if (seq > flow && seq > R(1e-9) && seq > M.flow_floor) {
         R scale = flow / seq;
 #pragma unroll
         for (int c = 0; c < 6; ++c) p.s[c] = HAS_BACK ? p.a[c] + scale * sh[c] : scale * sh[c];
-        R dp = (seq - flow) / rmax(R(3) * M.shear, R(1));
+        R dp = divc(seq - flow, M.d_3g, M.inv_3g);
         if (HAS_BACK && M.H > R(0) && dp > R(0)) {
             R ns = M.H * dp / seq;
 #pragma unroll
             for (int c = 0; c < 6; ++c) p.a[c] += ns * sh[c];
         }
         p.eps += dp;
-        R dT = R(0.9) * flow * dp / rmax(M.density * M.cp, R(1));
+        R dT = divc(R(0.9) * flow * dp, M.d_rho_cp, M.inv_rho_cp);
         if (M.clamp) dT = rmin(dT, M.dT_max);
         p.T = rmin(p.T + dT, M.melt);
     }
-    if (eqRate > R(0)) p.dmg += (eqRate * dt) / rmax(M.failure_strain, R(1e-3));
+    if (eqRate > R(0)) p.dmg += divc(eqRate * dt, M.d_fail, M.inv_fail);
     if (p.dmg > R(0.7) * M.dth && p.dmg < M.dth) {
-        R degrade = (M.dth - p.dmg) / (M.dth * R(0.3));
+        R degrade = divc(M.dth - p.dmg, M.d_deg, M.inv_deg);
         degrade = rmax(R(0), degrade);
 #pragma unroll
         for (int c = 0; c < 6; ++c) {

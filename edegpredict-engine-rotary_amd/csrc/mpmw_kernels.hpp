@@ -118,23 +118,23 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 
 // ---------------------------------------------------------- grid update ---
 
-// Where the grid update finds this step's P2G sums.  part == nullptr (atomic
-// mode): in the node accumulators.  Otherwise (deterministic mode): in the
-// tile windows, part[(tile * 8 + channel) * WN + window node], written whole
-// by every non-empty tile, plus, when `loose` is set, the node accumulators
-// (loose particles, summed in index order).
+// Where the grid update finds this step's P2G sums: in the tile windows,
+// part[(tile * 8 + channel) * WN + window node], written whole by every
+// non-empty tile, plus, when *acc_flag is set, the node accumulators (loose
+// particles, and in the atomic mode particles scattered outside their tile's
+// window).
 template <typename R> struct TileParts {
     const R *part;
     const int32_t *tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
+    const int32_t *acc_flag;
     TileGeom tg;
-    int32_t loose;
 };
 
-// The tiles whose window holds node coordinate i along one axis, lower
-// tile first, with the window-local coordinate: node i is local T of tile
-// q - 1 and local 0 of tile q when i - kCellOffset = q * T.
-__device__ __forceinline__ int window_tiles(int i, int nt, int T, int tile[2], int local[2]) {
-    const int r = i - kCellOffset;
+// The tiles whose window holds node coordinate i along one axis (tiles start
+// at cell `off`), lower tile first, with the window-local coordinate: node i
+// is local T of tile q - 1 and local 0 of tile q when i - off = q * T.
+__device__ __forceinline__ int window_tiles(int i, int off, int nt, int T, int tile[2], int local[2]) {
+    const int r = i - off;
     if (r < 0) return 0;
     const int q = r / T, l = r - q * T;
     int k = 0;
@@ -148,9 +148,9 @@ __device__ __forceinline__ int window_tiles(int i, int nt, int T, int tile[2], i
 template <typename R>
 __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, int j, int k, R c[8]) {
     int tx[2], lx[2], ty[2], ly[2], tz[2], lz[2];
-    const int nxs = window_tiles(i, tp.tg.ntx, TX, tx, lx);
-    const int nys = window_tiles(j, tp.tg.nty, TY, ty, ly);
-    const int nzs = window_tiles(k, tp.tg.ntz, TZ, tz, lz);
+    const int nxs = window_tiles(i, tp.tg.ox, tp.tg.ntx, TX, tx, lx);
+    const int nys = window_tiles(j, tp.tg.oy, tp.tg.nty, TY, ty, ly);
+    const int nzs = window_tiles(k, tp.tg.oz, tp.tg.ntz, TZ, tz, lz);
 #pragma unroll
     for (int ch = 0; ch < 8; ++ch) c[ch] = R(0);
     for (int a = 0; a < nzs; ++a)
@@ -165,33 +165,23 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
 }
 
 // mpmGridUpdateKernel, MPMWorkpieceSolver.cu:136-193 (point samples) and its
-// analytic-SDF form with tool velocity and Coulomb friction.  Takes the P2G
-// sums from where `tp` says, zeroing the accumulators it has read; writes
-// node velocity, mass and T*m.  When `direct_T` is
-// set (no diffusion pass follows) it also writes the gather temperature.
+// analytic-SDF form with tool velocity and Coulomb friction, for node
+// (i, j, k) = idx.  Takes the P2G sums from where `tp` says, zeroing the
+// accumulators it has read; writes node velocity, mass and T*m.  When
+// `direct_T` is set (no diffusion pass follows) it also writes the gather
+// temperature.  Returns the node's contact-force contribution.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
-                                                        Tool tool, R dt, int direct_T, double *force_part) {
-    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> &G, const TileParts<R> &tp,
+                                            const Material<R> &M, const Tool &tool, R dt, int direct_T, int64_t idx,
+                                            int i, int j, int k) {
     double cf = 0.0;
-    if (idx < g.nn) {
-        const int i = (int)(idx % g.nx);
-        const int64_t t = idx / g.nx;
-        const int j = (int)(t % g.ny), k = (int)(t / g.ny);
+    {
         R c[8];
-        if (tp.part) {
-            gather_tile_sums(tp, i, j, k, c);
-            if (tp.loose) {
-#pragma unroll
-                for (int ch = 0; ch < 8; ++ch) {
-                    c[ch] += G.acc[ch * g.nn + idx];
-                    G.acc[ch * g.nn + idx] = R(0);
-                }
-            }
-        } else {
+        gather_tile_sums(tp, i, j, k, c);
+        if (*tp.acc_flag) {
 #pragma unroll
             for (int ch = 0; ch < 8; ++ch) {
-                c[ch] = G.acc[ch * g.nn + idx];
+                c[ch] += G.acc[ch * g.nn + idx];
                 G.acc[ch * g.nn + idx] = R(0);
             }
         }
@@ -244,7 +234,8 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
                             }
                         }
                     }
-                } else if (tool.kind == kToolWedge || tool.kind == kToolDrill) {
+                } else if (tool.kind == kToolWedge ||
+                           (tool.kind == kToolDrill && !drill_far(tool, px, py, pz))) {
                     const double phi = tool_phi(tool, px, py, pz);
                     if (phi < tool.band) {
                         const double e = tool.eps;
@@ -288,15 +279,31 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
         G.gm[idx] = mass;
         G.gtm[idx] = tm;
     }
+    return cf;
+}
+
+// The grid update, one thread per node in node order (x fastest).
+// Contact-force partials per workgroup, summed in a fixed order.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
+                                                        Tool tool, R dt, int direct_T, double *force_part) {
+    double cf = 0.0;
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx < g.nn) {
+        const int i = (int)(idx % g.nx);
+        const int64_t row = idx / g.nx;
+        const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+        cf = grid_node(g, G, tp, M, tool, dt, direct_T, idx, i, j, k);
+    }
     // fixed-order block sum of the contact force
     __shared__ double sh[kBlock / kWave];
     cf = wave_sum(cf);
     if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = cf;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t = sh[0];
-        for (int w = 1; w < kBlock / kWave; ++w) t += sh[w];
-        force_part[blockIdx.x] = t;
+        double r = sh[0];
+        for (int w = 1; w < kBlock / kWave; ++w) r += sh[w];
+        force_part[blockIdx.x] = r;
     }
 }
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -75,6 +76,7 @@ struct mpmw_solver {
     // grid
     double minx = 0, miny = 0, minz = 0, h = 0;
     int32_t nx = 1, ny = 1, nz = 1, kfix = -1;
+    int32_t tile_off[3] = {4, 4, 4};  // first tiled cell per axis (the AABB-min particle's cell)
     int64_t nn = 0, nn_cap = 0;
     void *acc = nullptr, *node = nullptr, *gm = nullptr, *gtm = nullptr, *snap = nullptr;
     bool snap_on = false, snap_valid = false;
@@ -270,6 +272,9 @@ template <typename R> GridView<R> gview(const mpmw_solver *s) {
     GridView<R> g;
     g.ox = s->minx; g.oy = s->miny; g.oz = s->minz;
     g.hd = s->h;
+    g.inv_hd = 1.0 / s->h;
+    // two ulps of the largest cell coordinate, with margin
+    g.guard = 8.0 * std::ldexp(1.0, -52) * (double)std::max(std::max(s->nx, s->ny), std::max(s->nz, 1));
     g.inv_h = (R)(1.0 / s->h);
     g.nx = s->nx; g.ny = s->ny; g.nz = s->nz;
     g.kfix = s->kfix;
@@ -301,6 +306,26 @@ template <typename R> Material<R> material(const mpmw_solver *s) {
     M.rc = (R)p.tool_contact_radius; M.c_damp = (R)p.contact_damping;
     M.dT_max = (R)p.max_contact_temp_rise; M.H = (R)p.kinematic_hardening;
     M.clamp = p.thermal_clamp_enabled;
+    // divisors formed as the kernels formed them (in R), and their reciprocals
+    M.d_ref_rate = std::max(M.ref_rate, (R)1e-9);
+    M.d_t_span = std::max(M.melt - M.ref_temp, (R)1);
+    M.d_3g = std::max((R)3 * M.shear, (R)1);
+    M.d_rho_cp = std::max(M.density * M.cp, (R)1);
+    M.d_fail = std::max(M.failure_strain, (R)1e-3);
+    M.d_deg = M.dth * (R)0.3;
+    M.inv_ref_rate = (R)(1.0 / (double)M.d_ref_rate);
+    M.inv_t_span = (R)(1.0 / (double)M.d_t_span);
+    M.inv_3g = (R)(1.0 / (double)M.d_3g);
+    M.inv_rho_cp = (R)(1.0 / (double)M.d_rho_cp);
+    M.inv_fail = (R)(1.0 / (double)M.d_fail);
+    M.inv_deg = (R)(1.0 / (double)M.d_deg);
+    // johnsonCookStress >= A * 1 * 0.01 when A, B, C >= 0 (strain term >= A,
+    // rate term >= 1 since the rate ratio is >= 1, thermal term >= 0.01); the
+    // bound is rounded like the kernel's product, which rounding keeps monotone
+    if (p.jc_A >= 0.0 && p.jc_B >= 0.0 && p.jc_C >= 0.0 && p.jc_n > 0.0)
+        M.flow_floor = M.jcA * (R)0.01;
+    else
+        M.flow_floor = -std::numeric_limits<R>::infinity();
     return M;
 }
 
@@ -431,6 +456,13 @@ int rebuild_grid_from_stats(mpmw_solver *s) {
     s->minx = mn[0] - pad;
     s->miny = mn[1] - pad;
     s->minz = mn[2] - pad;
+    // tiles start at the AABB-min particle's cell, computed with the kernels'
+    // expression floor((x - origin) / h): 4, or 3 when the division rounds down
+    const double org[3] = {s->minx, s->miny, s->minz};
+    for (int a = 0; a < 3; ++a) {
+        const double fl = std::floor((mn[a] - org[a]) / cell);
+        s->tile_off[a] = (fl >= 0.0 && fl <= 4.0) ? (int32_t)fl : 4;
+    }
     // max(4, (int)ceil(extent / h) + 1), with the int conversion kept in range
     auto dim = [&](double lo, double hi, int32_t &out) {
         const double c = std::ceil((hi - lo + 2 * pad) / cell) + 1.0;
@@ -483,11 +515,14 @@ template <typename R> int zero_acc(mpmw_solver *s) {
 // ------------------------------------------------------------ binning -----
 
 TileGeom tile_geom(const mpmw_solver *s) {
-    auto tiles = [](int n, int T) { return std::max(1, (n - 1 - kCellOffset + T - 1) / T); };
+    auto tiles = [](int n, int off, int T) { return std::max(1, (n - 1 - off + T - 1) / T); };
     TileGeom t;
-    t.ntx = tiles(s->nx, TX);
-    t.nty = tiles(s->ny, TY);
-    t.ntz = tiles(s->nz, TZ);
+    t.ox = s->tile_off[0];
+    t.oy = s->tile_off[1];
+    t.oz = s->tile_off[2];
+    t.ntx = tiles(s->nx, t.ox, TX);
+    t.nty = tiles(s->ny, t.oy, TY);
+    t.ntz = tiles(s->nz, t.oz, TZ);
     t.ntiles = t.ntx * t.nty * t.ntz;
     t.nkeys = t.ntiles * kTileCells;
     return t;
@@ -539,13 +574,13 @@ template <typename R> int sort_particles(mpmw_solver *s) {
     if ((rc = grow32(s, S.tile_start, S.tile_cap, tg.ntiles + 1))) return rc;
     if (!S.oow) {
         void *q;
-        if ((rc = dalloc(s, &q, 2 * sizeof(int32_t)))) return rc;
+        if ((rc = dalloc(s, &q, 3 * sizeof(int32_t)))) return rc;
         S.oow = (int32_t *)q;
         HIP_TRY(hipHostMalloc((void **)&S.h_oow, 4 * sizeof(int32_t), hipHostMallocDefault));
         S.h_oow[0] = S.h_oow[1] = S.h_oow[2] = S.h_oow[3] = 0;
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)2, 0);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)3, 0);
     }
-    if (s->deterministic) {
+    {
         const int64_t bytes = (int64_t)tg.ntiles * 8 * WN * (int64_t)s->rsz;
         if (bytes > s->tpart_cap) {
             const int64_t c = std::max<int64_t>(bytes, s->tpart_cap * 5 / 4);
@@ -601,7 +636,8 @@ template <typename R> int rebuild(mpmw_solver *s) {
     bool need = !S.valid;
     if (S.valid) {
         const TileGeom tg = tile_geom(s);
-        need = tg.ntx != S.geom.ntx || tg.nty != S.geom.nty || tg.ntz != S.geom.ntz;
+        need = tg.ntx != S.geom.ntx || tg.nty != S.geom.nty || tg.ntz != S.geom.ntz || tg.ox != S.geom.ox ||
+               tg.oy != S.geom.oy || tg.oz != S.geom.oz;
         const double o[3] = {s->minx, s->miny, s->minz};
         // the deterministic P2G needs every particle in its binned cell: any
         // origin change re-bins (else the first P2G would be repeated)
@@ -639,8 +675,9 @@ int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     const Material<R> M = material<R>(s);
     SortState &S = s->sort;
     const TileGeom tg = S.geom;
-    if (DET && DO_P2G)
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow + 1, (int64_t)1, 0);
+    // counters [1] (deterministic: particles off their cell) and [2] (node
+    // accumulators written) start at zero for every P2G pass
+    if (DO_P2G) hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow + 1, (int64_t)2, 0);
     hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, HAS_BACK, DET>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P,
                        g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
                        (R *)s->tpart);
@@ -649,10 +686,10 @@ int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     if (nbl > 0) {
         hipLaunchKernelGGL((k_loose<R, DO_G2P, DO_P2G && !DET, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream,
                            P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)tg.ntiles * kRed,
-                           S.loose_begin, S.loose_end);
+                           S.loose_begin, S.loose_end, S.oow + 2);
         if (DET && DO_P2G)
             hipLaunchKernelGGL(k_loose_p2g_ordered<R>, dim3(1), dim3(kWave), 0, s->stream, P, g, G, S.loose_begin,
-                               S.loose_end);
+                               S.loose_end, S.oow + 2);
         HIP_TRY(hipGetLastError());
     }
     return MPMW_OK;
@@ -669,11 +706,11 @@ int particle_pass(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
 }
 
 // P2G alone into empty sums (first step, after a rebuild or a re-binning).
+// The node accumulators are cleared: a repeated P2G may follow a pass that
+// wrote them without a grid update in between.
 template <typename R> int p2g_alone(mpmw_solver *s, double dt, double zcut) {
     int rc;
-    if (!s->deterministic || loose_blocks(s) > 0) {
-        if ((rc = zero_acc<R>(s))) return rc;
-    }
+    if ((rc = zero_acc<R>(s))) return rc;
     G2PFlags fl;
     fl.chip = 0;
     fl.stats = 0;
@@ -724,21 +761,22 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     // the binning may have changed in det_settle
     const TileGeom tgs = s->sort.geom;
     TileParts<R> tp;
-    tp.part = det ? (const R *)s->tpart : nullptr;
+    tp.part = (const R *)s->tpart;
     tp.tile_start = s->sort.tile_start;
+    tp.acc_flag = s->sort.oow + 2;
     tp.tg = tgs;
-    tp.loose = loose_blocks(s) > 0 ? 1 : 0;
     const bool diffusion = s->prm.specific_heat > 0.0 &&
                            s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
+    const int gblocks = blocks_for(s->nn);
     {
         Timed t(s, MPMW_K_GRID);
-        hipLaunchKernelGGL(k_grid_update<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, tp, M,
+        hipLaunchKernelGGL(k_grid_update<R>, dim3(gblocks), dim3(kBlock), 0, s->stream, g, G, tp, M,
                            tool_view(s), (R)dt, diffusion ? 0 : 1, s->fpart);
         HIP_TRY(hipGetLastError());
     }
     {
         Timed t(s, MPMW_K_REDUCE);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, s->fpart, blocks_for(s->nn), 1,
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, s->fpart, gblocks, 1,
                            s->d_modes + kRed, s->d_cf);
         HIP_TRY(hipGetLastError());
     }

@@ -1,8 +1,9 @@
 // mpmw_sort.hpp -- particle binning into cell tiles.
 //
-// Particles are kept ordered by tile (a block of TX x TY x TZ cells; tiles
-// start at cell kCellOffset, one cell inside the AABB-min cell, so a particle
-// on the AABB face that rounds into the cell below stays in a tile) and,
+// Particles are kept ordered by tile (a block of TX x TY x TZ cells; along
+// each axis the tiles start at the cell of the AABB-min particle, found with
+// the same FP64 expression the kernels use, so the tiles line up with the
+// workpiece) and,
 // inside a tile, slot-major: the first particle of every cell, then the
 // second of every cell, and so on.  A 64-lane wave then holds particles of 64
 // different cells, so its LDS scatter for one corner offset touches 64
@@ -23,7 +24,6 @@ constexpr int TX = 8, TY = 8, TZ = 4;                 // cells per tile
 constexpr int kTileCells = TX * TY * TZ;              // 256 = one cell per thread
 constexpr int WX = TX + 1, WY = TY + 1, WZ = TZ + 1;  // node window of a tile
 constexpr int WN = WX * WY * WZ;                      // 405 nodes
-constexpr int kCellOffset = 3;                        // first tiled cell (the AABB-min cell is 4: pad = 4h)
 
 struct ScratchBuf {
     void *ptr = nullptr;
@@ -32,6 +32,7 @@ struct ScratchBuf {
 
 // Tile geometry over the current grid.
 struct TileGeom {
+    int32_t ox = 4, oy = 4, oz = 4;    // first tiled cell along x, y, z
     int32_t ntx = 0, nty = 0, ntz = 0;
     int32_t ntiles = 0;
     int32_t nkeys = 0;                 // ntiles * kTileCells

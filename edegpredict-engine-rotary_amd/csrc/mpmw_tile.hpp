@@ -4,12 +4,15 @@
 // tile's node window (WX x WY x WZ nodes) is staged in LDS: the gather record
 // (v, T) for G2P and eight accumulator channels for P2G.  A thread keeps the
 // 8 x 8 P2G corner sums of its cell's particles in registers; after the
-// particle loop the sums are combined
-// across the wave with lane shuffles into node values and added to the LDS
-// window layer by layer, then the window is flushed once to HBM.  No LDS
-// float atomics are used on this path (they serialise on gfx950).  A
+// particle loop the sums are combined across the wave with lane shuffles into
+// node values and added to the LDS window layer by layer, then the whole
+// window is stored, with plain coalesced stores, into the tile's own slot of
+// a per-tile window buffer; the grid update sums the windows holding each
+// node.  No float atomics are used on this path (LDS float atomics serialise
+// on gfx950; device float atomics are bound by the L2 atomic rate).  A
 // particle that has left its cell since the last binning is scattered with
-// atomics (LDS inside the window, device memory outside it).
+// atomics (LDS inside the window, the node accumulators outside it), or, in
+// the deterministic mode, counted for re-binning.
 //
 // Reference kernels: mpmP2GKernel (MPMWorkpieceSolver.cu:98-134),
 // mpmG2PKernel (:264-400), chipFractureAndSeparationKernel (:667-725).
@@ -27,62 +30,125 @@ struct G2PFlags {
 
 // ------------------------------------------------------- particle state ---
 
+// The node sums of the G2P gather (mpmG2PKernel :276-293): velocity and
+// temperature weighted by w, the weight sum and the velocity gradient.
+template <typename R> struct GatherSums {
+    R v0, v1, v2, temp, wsum;
+    R L[9];
+};
+
+// Gather over the 8 corners from device memory; corners outside the grid,
+// with w <= 0 or with an empty node (NaN record) are skipped (:280-286).
+template <typename R>
+__device__ __forceinline__ void gather_global(const GridView<R> &g, const vec4_t<R> *node, const Frac<R> &f,
+                                              GatherSums<R> &q) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        int64_t idx;
+        R w, gx, gy, gz;
+        if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
+        const vec4_t<R> nd = node[idx];
+        if (nd.w != nd.w) continue;
+        q.v0 += nd.x * w; q.v1 += nd.y * w; q.v2 += nd.z * w;
+        q.temp += nd.w * w;
+        q.wsum += w;
+        q.L[0] += nd.x * gx; q.L[1] += nd.x * gy; q.L[2] += nd.x * gz;
+        q.L[3] += nd.y * gx; q.L[4] += nd.y * gy; q.L[5] += nd.y * gz;
+        q.L[6] += nd.z * gx; q.L[7] += nd.z * gy; q.L[8] += nd.z * gz;
+    }
+}
+
+__device__ __forceinline__ int wnode(int x, int y, int z) { return (z * WY + y) * WX + x; }
+
+// The same gather from the tile's LDS window, base corner at window index
+// wb: every corner is inside the window, and window nodes outside the grid
+// hold the empty (NaN) record.  Branch-free: a skipped corner adds zeros.
+template <typename R>
+__device__ __forceinline__ void gather_window(const vec4_t<R> *s_node, int wb, const Frac<R> &f, R inv_h,
+                                              GatherSums<R> &q) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
+        const R wx = dx ? f.w1[0] : f.w0[0];
+        const R wy = dy ? f.w1[1] : f.w0[1];
+        const R wz = dz ? f.w1[2] : f.w0[2];
+        const R w = wx * wy * wz;
+        const vec4_t<R> nd = s_node[wb + wnode(dx, dy, dz)];
+        const bool ok = (w > R(0)) && (nd.w == nd.w);
+        const R nx = ok ? nd.x : R(0), ny = ok ? nd.y : R(0), nz = ok ? nd.z : R(0), nt = ok ? nd.w : R(0);
+        const R gx = (dx ? inv_h : -inv_h) * wy * wz;
+        const R gy = wx * (dy ? inv_h : -inv_h) * wz;
+        const R gz = wx * wy * (dz ? inv_h : -inv_h);
+        q.v0 += nx * w; q.v1 += ny * w; q.v2 += nz * w;
+        q.temp += nt * w;
+        q.wsum += ok ? w : R(0);
+        q.L[0] += nx * gx; q.L[1] += nx * gy; q.L[2] += nx * gz;
+        q.L[3] += ny * gx; q.L[4] += ny * gy; q.L[5] += ny * gz;
+        q.L[6] += nz * gx; q.L[7] += nz * gy; q.L[8] += nz * gz;
+    }
+}
+
 // One particle through mpmG2PKernel (MPMWorkpieceSolver.cu:264-400) and, on
 // chip steps, chipFractureAndSeparationKernel (:667-725).  The caller has
 // loaded status and position.  Only the fields the particle's status makes
 // the reference read are loaded, and only fields that changed are stored.
-// node_at(n, idx) returns the gather record of corner n (global index idx).
-template <typename R, bool HAS_BACK, typename NodeAt>
-__device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, PState<R> &p, const Frac<R> &f,
-                                             const GridView<R> &g, const Material<R> &M, R dt, double dtd,
-                                             bool chip_step, bool need_stress, double zcut, NodeAt node_at) {
-    const int32_t st0 = p.status;
+// gather(q) adds the node sums of the particle's corners to q.
+// The per-particle inputs of the G2P pass, loaded together (a tile pass
+// loads the next slot's while it computes the current one).  Stress, plastic
+// strain and damage are loaded whatever the status; g2p_particle uses them
+// only where the reference reads them.
+template <typename R> struct G2PIn {
+    double x, y, z;
+    R T, eps, dmg;
+    R s[6];
+    int32_t status;
+};
+
+template <typename R> __device__ __forceinline__ void load_g2p_in(const Particles<R> &P, int i, G2PIn<R> &in) {
+    in.status = P.status[i];
+    in.x = P.x[i]; in.y = P.y[i]; in.z = P.z[i];
+    in.T = P.T[i];
+    in.eps = P.eps[i];
+    in.dmg = P.dmg[i];
+    in.s[0] = P.s0[i]; in.s[1] = P.s1[i]; in.s[2] = P.s2[i];
+    in.s[3] = P.s3[i]; in.s[4] = P.s4[i]; in.s[5] = P.s5[i];
+}
+
+template <typename R, bool HAS_BACK, typename Gather>
+__device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
+                                             const Material<R> &M, R dt, double dtd, bool chip_step,
+                                             bool need_stress, double zcut, Gather gather) {
+    const int32_t st0 = in.status;
     const bool act = st0 == kActive, fixed = st0 == kFixed;
-    p.T = P.T[i];
+    p.status = st0;
+    p.x = in.x; p.y = in.y; p.z = in.z;
+    p.T = in.T;
     p.sres = -__builtin_huge_valf();
     p.rate = R(0);
-    p.eps = R(0);
-    p.dmg = R(0);
+    const bool use_s = act || fixed || need_stress;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) { p.s[c] = R(0); p.a[c] = R(0); }
-    if (act || fixed || need_stress) {
-        p.s[0] = P.s0[i]; p.s[1] = P.s1[i]; p.s[2] = P.s2[i];
-        p.s[3] = P.s3[i]; p.s[4] = P.s4[i]; p.s[5] = P.s5[i];
+    for (int c = 0; c < 6; ++c) { p.s[c] = use_s ? in.s[c] : R(0); p.a[c] = R(0); }
+    p.eps = act ? in.eps : R(0);
+    if (HAS_BACK && act) {
+        p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
+        p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
     }
-    if (act) {
-        p.eps = P.eps[i];
-        if (HAS_BACK) {
-            p.a[0] = P.a0[i]; p.a[1] = P.a1[i]; p.a[2] = P.a2[i];
-            p.a[3] = P.a3[i]; p.a[4] = P.a4[i]; p.a[5] = P.a5[i];
-        }
-    }
-    if (act || st0 == kChip) p.dmg = P.dmg[i];
+    p.dmg = (act || st0 == kChip) ? in.dmg : R(0);
     const R eps0 = p.eps, dmg0 = p.dmg, T0 = p.T;
     if (fixed) {  // :296-300
         p.vx = p.vy = p.vz = R(0);
     } else {
-        R v0 = 0, v1 = 0, v2 = 0, temp = 0, wsum = 0;
-        R L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        GatherSums<R> q;
+        q.v0 = q.v1 = q.v2 = q.temp = q.wsum = R(0);
 #pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            int64_t idx;
-            R w, gx, gy, gz;
-            if (!corner(g, f, n, idx, w, gx, gy, gz)) continue;
-            const vec4_t<R> nd = node_at(n, idx);
-            if (nd.w != nd.w) continue;  // node mass <= 1e-18 (:286)
-            v0 += nd.x * w; v1 += nd.y * w; v2 += nd.z * w;
-            temp += nd.w * w;
-            wsum += w;
-            L[0] += nd.x * gx; L[1] += nd.x * gy; L[2] += nd.x * gz;
-            L[3] += nd.y * gx; L[4] += nd.y * gy; L[5] += nd.y * gz;
-            L[6] += nd.z * gx; L[7] += nd.z * gy; L[8] += nd.z * gz;
-        }
-        p.vx = v0; p.vy = v1; p.vz = v2;
+        for (int c = 0; c < 9; ++c) q.L[c] = R(0);
+        gather(q);
+        p.vx = q.v0; p.vy = q.v1; p.vz = q.v2;
         p.x += (double)p.vx * dtd;
         p.y += (double)p.vy * dtd;
         p.z += (double)p.vz * dtd;
-        if (wsum > R(1e-8)) p.T = rmin(rmax(temp / wsum, M.ref_temp), M.melt);
-        if (act) constitutive_active<R, HAS_BACK>(M, p, L, dt);
+        if (q.wsum > R(1e-8)) p.T = rmin(rmax(q.temp / q.wsum, M.ref_temp), M.melt);
+        if (act) constitutive_active<R, HAS_BACK>(M, p, q.L, dt);
         chip_transition(M, p);
     }
     if (chip_step) chip_pass(M, p, P.id, i, zcut);
@@ -149,8 +215,6 @@ __device__ __forceinline__ int p2g_corner(R mass, R vol, R vx, R vy, R vz, R T, 
 
 // ----------------------------------------------------------- tile kernel --
 
-__device__ __forceinline__ int wnode(int x, int y, int z) { return (z * WY + y) * WX + x; }
-
 // The 8 corner contributions of the particles of one cell, 8 channels each
 // (mass, momentum xyz, force xyz, T*m), held in registers.  Corner c has
 // offsets dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1 (the node order
@@ -166,6 +230,12 @@ template <typename R>
 __device__ __forceinline__ void cell_accumulate(CellAcc<R> &A, const Frac<R> &f, R inv_h, R mass, R vol, R vx, R vy,
                                                 R vz, R T, const R s[6], R fex, R fey, R fez, bool stress_force,
                                                 bool ext_only) {
+    // branch-free form of p2g_corner: a corner with w <= 0 adds zeros, the
+    // stress term is zero unless stress_force, the external force unless
+    // stress_force or ext_only
+    const R volm = stress_force ? vol : R(0);
+    const bool fe = stress_force || ext_only;
+    const R ex = fe ? fex : R(0), ey = fe ? fey : R(0), ez = fe ? fez : R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
@@ -173,30 +243,28 @@ __device__ __forceinline__ void cell_accumulate(CellAcc<R> &A, const Frac<R> &f,
         const R wy = dy ? f.w1[1] : f.w0[1];
         const R wz = dz ? f.w1[2] : f.w0[2];
         const R w = wx * wy * wz;
-        if (!(w > R(0))) continue;
-        const R gx = (dx ? inv_h : -inv_h) * wy * wz;
-        const R gy = wx * (dy ? inv_h : -inv_h) * wz;
-        const R gz = wx * wy * (dz ? inv_h : -inv_h);
-        R ch[8];
-        const int nch = p2g_corner(mass, vol, vx, vy, vz, T, s, fex, fey, fez, stress_force, ext_only, w, gx, gy,
-                                   gz, ch);
-        A.v[0][c] += ch[0];
-        A.v[1][c] += ch[1];
-        A.v[2][c] += ch[2];
-        A.v[3][c] += ch[3];
-        A.v[7][c] += ch[7];
-        if (nch == 8) {
-            A.v[4][c] += ch[4];
-            A.v[5][c] += ch[5];
-            A.v[6][c] += ch[6];
-        }
+        const bool ok = w > R(0);
+        const R gx = ok ? (dx ? inv_h : -inv_h) * wy * wz : R(0);
+        const R gy = ok ? wx * (dy ? inv_h : -inv_h) * wz : R(0);
+        const R gz = ok ? wx * wy * (dz ? inv_h : -inv_h) : R(0);
+        const R m = ok ? mass * w : R(0);
+        const R we = ok ? w : R(0);
+        A.v[0][c] += m;
+        A.v[1][c] += vx * m;
+        A.v[2][c] += vy * m;
+        A.v[3][c] += vz * m;
+        A.v[7][c] += T * m;
+        A.v[4][c] += -(s[0] * gx + s[3] * gy + s[4] * gz) * volm + ex * we;
+        A.v[5][c] += -(s[3] * gx + s[1] * gy + s[5] * gz) * volm + ey * we;
+        A.v[6][c] += -(s[4] * gx + s[5] * gy + s[2] * gz) * volm + ez * we;
     }
 }
 
 // Scatter of one particle whose stencil is not its home cell: through LDS
-// atomics when the stencil lies in the tile window, else device atomics.
+// atomics when the stencil lies in the tile window, else device atomics into
+// the node accumulators (then returns true).
 template <typename R>
-__device__ __forceinline__ void scatter_exception(const GridView<R> &g, R *acc, R *s_acc, const Frac<R> &f,
+__device__ __forceinline__ bool scatter_exception(const GridView<R> &g, R *acc, R *s_acc, const Frac<R> &f,
                                                   int wx0, int wy0, int wz0, R mass, R vol, R vx, R vy, R vz, R T,
                                                   const R s[6], R fex, R fey, R fez, bool stress_force,
                                                   bool ext_only) {
@@ -237,6 +305,7 @@ __device__ __forceinline__ void scatter_exception(const GridView<R> &g, R *acc, 
             }
         }
     }
+    return !in;
 }
 
 // The fields a P2G-only pass reads (mpmP2GKernel :103-133): velocity,
@@ -299,14 +368,24 @@ __device__ __forceinline__ void combine_layer(const CellAcc<R> &A, R *s_acc, int
 // walks that cell's particles slot by slot (the memory order is slot-major,
 // so a full tile reads contiguously).  DO_G2P: gather, constitutive update
 // and chip pass.  DO_P2G: scatter into the next step's accumulators.
-// DET (deterministic P2G): the tile's window is stored whole into its slot
-// of `tpart` (8 channels x WN nodes) instead of being added to device memory,
-// and a particle outside its binned cell is only counted in counters[1]; the
-// caller re-bins and repeats the P2G when that count is not zero.  Every sum
-// then has a fixed order: per cell in slot order, across a wave by the
-// fixed shuffle tree, across tiles in tile order (k_grid_update).
+// The window goes to tpart[t] (8 channels x WN nodes).  counters[2] is set
+// when a particle was scattered into the node accumulators.  DET
+// (deterministic P2G): a particle outside its binned cell is only counted in
+// counters[1]; the caller re-bins and repeats the P2G when that count is not
+// zero.  Every sum then has a fixed order: per cell in slot order, across a
+// wave by the fixed shuffle tree, across tiles in tile order (k_grid_update).
+// Waves per SIMD the tile pass is compiled for (register budget 512 / waves).
+// FP32: 2 (a budget of 3 waves spills to scratch and measured 18 % slower);
+// FP64 keeps the compiler's choice.
+#ifndef MPMW_TILE_WAVES_F32
+#define MPMW_TILE_WAVES_F32 2
+#endif
+template <typename R> struct TileWaves {
+    static constexpr int value = sizeof(R) == 4 ? MPMW_TILE_WAVES_F32 : 1;
+};
+
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
-__global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
+__global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
                                                  double dtd, double zcut, G2PFlags flags, double ref_temp,
                                                  double *part, const int32_t *cell_start, TileGeom tg,
                                                  int32_t *counters, R *tpart) {
@@ -319,11 +398,14 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
     vec4_t<R> *s_node = reinterpret_cast<vec4_t<R> *>(smem);
     R *s_acc = DO_P2G ? reinterpret_cast<R *>(smem + kNodeBytes) : nullptr;
 
+    // statistics are reduced only by the G2P-only pass (the step before a
+    // rebuild); the fused pass carries no statistics registers
+    const bool stats = DO_G2P && !DO_P2G && flags.stats != 0;
     const int t = blockIdx.x, tid = threadIdx.x;
     const int lane = tid % kWave, wv = tid / kWave;
     const int hx = tid % TX, hy = (tid / TX) % TY, hz = tid / (TX * TY);
     const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
-    const int wx0 = kCellOffset + tx * TX, wy0 = kCellOffset + ty * TY, wz0 = kCellOffset + tz * TZ;
+    const int wx0 = tg.ox + tx * TX, wy0 = tg.oy + ty * TY, wz0 = tg.oz + tz * TZ;
     const int key = t * kTileCells + tid;
     const int c_begin = cell_start[key];
     const int cnt = cell_start[key + 1] - c_begin;
@@ -372,11 +454,20 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
 #pragma unroll
                 for (int c = 0; c < 8; ++c) A.v[ch][c] = R(0);
         }
-        int my_oow = 0, my_exc = 0;
+        int my_oow = 0, my_exc = 0, my_acc = 0;
         const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int base = 0;
+        // uniform tiles: slot s of this thread's cell is particle b0 + s * kBlock
+        // + tid; its G2P inputs are loaded one slot ahead
+        G2PIn<R> nxt;
+        if (DO_G2P && uniform && cnt > 0) load_g2p_in(P, b0 + tid, nxt);
         for (int s = 0; s < maxc; ++s) {
             const bool active = cnt > s;
+            G2PIn<R> cur;
+            if (DO_G2P && uniform) {
+                cur = nxt;
+                if (cnt > s + 1) load_g2p_in(P, b0 + (s + 1) * kBlock + tid, nxt);
+            }
             int i;
             if (uniform) {
                 i = b0 + s * kBlock + tid;
@@ -396,30 +487,35 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
             }
             if (!active) continue;
             PState<R> p;
-            p.status = P.status[i];
+            if (DO_G2P) {
+                if (!uniform) load_g2p_in(P, i, cur);
+                p.status = cur.status;
+            } else {
+                p.status = P.status[i];
+            }
             if (p.status == kInactive) continue;
             const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
             R fex = R(0), fey = R(0), fez = R(0);
-            p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
             if (DO_G2P) {
-                const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                const Frac<R> f = base_and_frac(g, cur.x, cur.y, cur.z);
                 const int lx = f.b[0] - wx0, ly = f.b[1] - wy0, lz = f.b[2] - wz0;
                 const bool in = (unsigned)lx < (unsigned)TX && (unsigned)ly < (unsigned)TY &&
                                 (unsigned)lz < (unsigned)TZ;
                 const int wb = wnode(lx, ly, lz);
                 my_oow += in ? 0 : 1;
-                g2p_particle<R, HAS_BACK>(P, i, p, f, g, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut,
-                                          [&](int n, int64_t idx) -> vec4_t<R> {
-                                              if (in) return s_node[wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1)];
-                                              return G.node[idx];
+                g2p_particle<R, HAS_BACK>(P, i, cur, p, M, dt, dtd, flags.chip != 0, stats, zcut,
+                                          [&](GatherSums<R> &q) {
+                                              if (in) gather_window(s_node, wb, f, g.inv_h, q);
+                                              else gather_global(g, G.node, f, q);
                                           });
-                if (flags.stats) {
+                if (stats) {
                     const double vx = p.vx, vy = p.vy, vz = p.vz;
                     accumulate_stats(st, p.x, p.y, p.z,
                                      (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]),
                                      (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
                 }
             } else {
+                p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
                 load_p2g_state(P, i, p, fex, fey, fez);
             }
             if (DO_P2G) {
@@ -434,8 +530,10 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
                     ++my_exc;
                 } else {
                     ++my_oow;
-                    scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex,
-                                      fey, fez, stress_force, ext_only);
+                    my_acc |= scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz, p.T,
+                                                p.s, fex, fey, fez, stress_force, ext_only)
+                                  ? 1
+                                  : 0;
                 }
             }
         }
@@ -445,45 +543,34 @@ __global__ void __launch_bounds__(kBlock) k_tile(Particles<R> P, GridView<R> g, 
             __syncthreads();  // layer z written by wave z only
             combine_layer(A, s_acc, hx, hy, hz, 1);
             __syncthreads();  // layer z+1 written by wave z only
-            if (DET) {
-                R *dst = tpart + (int64_t)t * (8 * WN);
-                for (int w = tid; w < 8 * WN; w += kBlock) dst[w] = s_acc[w];
-            } else {
-                for (int w = tid; w < WN; w += kBlock) {
-                    const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
-                    if (i >= g.nx || j >= g.ny || k >= g.nz) continue;
-                    R *a = G.acc + ((int64_t)k * g.ny + j) * g.nx + i;
-#pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        const R v = s_acc[c * WN + w];
-                        if (v != R(0)) unsafeAtomicAdd(a + c * g.nn, v);
-                    }
-                }
-            }
+            R *dst = tpart + (int64_t)t * (8 * WN);
+            for (int w = tid; w < 8 * WN; w += kBlock) dst[w] = s_acc[w];
         }
-        int cntv = my_oow, cnte = my_exc;
+        int cntv = my_oow, cnte = my_exc, cnta = my_acc;
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) {
             cntv += __shfl_xor(cntv, o, kWave);
             cnte += __shfl_xor(cnte, o, kWave);
+            cnta |= __shfl_xor(cnta, o, kWave);
         }
         if (lane == 0 && cntv) atomicAdd(counters, cntv);
         if (DET && lane == 0 && cnte) atomicAdd(counters + 1, cnte);
+        if (!DET && lane == 0 && cnta) atomicOr(counters + 2, 1);
     }
-    if (DO_G2P && flags.stats) block_reduce_stats(st, part);
+    if (stats) block_reduce_stats(st, part);
 }
 
 // ------------------------------------------------------ loose particles --
 
 // The particles binned as loose (cell outside the tiles, or a stencil node
 // outside the grid), one thread each, nodes read from device memory.  P2G
-// here is the atomic mode's (device atomics into the node accumulators); the
-// deterministic mode runs k_loose_p2g_ordered after this pass instead.
-// Statistics partials go to part[blockIdx.x].
+// here is the atomic mode's (device atomics into the node accumulators,
+// setting acc_flag); the deterministic mode runs k_loose_p2g_ordered after
+// this pass instead.  Statistics partials go to part[blockIdx.x].
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
 __global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
                                                   double dtd, double zcut, G2PFlags flags, double ref_temp,
-                                                  double *part, int32_t begin, int32_t end) {
+                                                  double *part, int32_t begin, int32_t end, int32_t *acc_flag) {
     double st[kRed];
     stats_identity(st, ref_temp);
     const int i = begin + (int)blockIdx.x * kBlock + (int)threadIdx.x;
@@ -495,9 +582,11 @@ __global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g,
             R fex = R(0), fey = R(0), fez = R(0);
             p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
             if (DO_G2P) {
+                G2PIn<R> in;
+                load_g2p_in(P, i, in);
                 const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
-                g2p_particle<R, HAS_BACK>(P, i, p, f, g, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut,
-                                          [&](int, int64_t idx) -> vec4_t<R> { return G.node[idx]; });
+                g2p_particle<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut,
+                                          [&](GatherSums<R> &q) { gather_global(g, G.node, f, q); });
                 if (flags.stats) {
                     const double vx = p.vx, vy = p.vy, vz = p.vz;
                     accumulate_stats(st, p.x, p.y, p.z,
@@ -511,6 +600,7 @@ __global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g,
                 const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
                 scatter_exception(g, G.acc, (R *)nullptr, f, 0, 0, 0, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex,
                                   fey, fez, p.status == kActive || p.status == kFixed, p.status == kChip);
+                atomicOr(acc_flag, 1);  // loose particles are few
             }
         }
     }
@@ -522,10 +612,12 @@ __global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g,
 // accumulators.  One particle's adds go to distinct addresses; the barrier
 // after each particle (workgroup-scope release + acquire) orders them before
 // the next particle's, so every node sums its loose particles in index order.
+// Sets acc_flag.
 template <typename R>
 __global__ void __launch_bounds__(kWave) k_loose_p2g_ordered(Particles<R> P, GridView<R> g, Grid<R> G,
-                                                             int32_t begin, int32_t end) {
+                                                             int32_t begin, int32_t end, int32_t *acc_flag) {
     const int n = (int)threadIdx.x >> 3, ch = (int)threadIdx.x & 7;
+    if (threadIdx.x == 0 && begin < end) *acc_flag = 1;
     for (int i = begin; i < end; ++i) {
         PState<R> p;
         p.status = P.status[i];
@@ -571,15 +663,15 @@ __global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, uint32_t 
         const double r[3] = {(P.x[i] - g.ox) / g.hd, (P.y[i] - g.oy) / g.hd, (P.z[i] - g.oz) / g.hd};
         const int nt[3] = {tg.ntx * TX, tg.nty * TY, tg.ntz * TZ};
         const int nodes[3] = {g.nx, g.ny, g.nz};
+        const int off[3] = {tg.ox, tg.oy, tg.oz};
         int c[3];
         bool tiled = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double fl = floor(r[a]);
             // in a tile, and both stencil nodes inside the grid (NaN fails too)
-            tiled = tiled && fl >= (double)kCellOffset && fl < (double)(kCellOffset + nt[a]) &&
-                    fl + 1.0 < (double)nodes[a];
-            c[a] = tiled ? (int)fl - kCellOffset : 0;
+            tiled = tiled && fl >= (double)off[a] && fl < (double)(off[a] + nt[a]) && fl + 1.0 < (double)nodes[a];
+            c[a] = tiled ? (int)fl - off[a] : 0;
         }
         if (tiled) {
             const uint32_t tile = (uint32_t)(((c[2] / TZ) * tg.nty + c[1] / TY) * tg.ntx + c[0] / TX);
