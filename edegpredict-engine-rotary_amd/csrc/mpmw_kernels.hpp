@@ -37,9 +37,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Fixed-order block reduction of the kRed particle statistics into
-// part[blockIdx.x * kRed + c].  Slots 0-5 and 6-7 reduce with min/max, slot 8
+// part[blk * kRed + c].  Slots 0-5 and 6-7 reduce with min/max, slot 8
 // with a sum whose order depends only on thread and block indices.
-__device__ __forceinline__ void block_reduce_stats(double v[kRed], double *part) {
+__device__ __forceinline__ void block_reduce_stats(double v[kRed], double *part, int blk) {
     __shared__ double sh[kBlock / kWave][kRed];
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
@@ -53,7 +53,7 @@ __device__ __forceinline__ void block_reduce_stats(double v[kRed], double *part)
         double r = sh[0][c];
         for (int w = 1; w < kBlock / kWave; ++w)
             r = (c < 6) ? fmin(r, sh[w][c]) : (c < 8 ? fmax(r, sh[w][c]) : r + sh[w][c]);
-        part[(int64_t)blockIdx.x * kRed + c] = r;
+        part[(int64_t)blk * kRed + c] = r;
     }
 }
 
@@ -67,8 +67,8 @@ __device__ __forceinline__ void stats_identity(double v[kRed], double ref_temp) 
 
 // Single-block fixed-order reduction of nparts partial records of width w.
 // mode per column: 0 min, 1 max, 2 sum.
-__global__ void __launch_bounds__(kBlock) k_finalize(const double *part, int nparts, int w,
-                                                     const int32_t *modes, double *out) {
+__device__ __forceinline__ void finalize_block(const double *part, int nparts, int w, const int32_t *modes,
+                                               double *out) {
     __shared__ double sh[kBlock / kWave];
     for (int c = 0; c < w; ++c) {
         const int mode = modes[c];
@@ -88,6 +88,11 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const double *part, int npa
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(const double *part, int nparts, int w,
+                                                     const int32_t *modes, double *out) {
+    finalize_block(part, nparts, w, modes, out);
 }
 
 template <typename R> __global__ void k_fill(R *p, int64_t n, R v) {
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
         double vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
         v[8] += 0.5 * (double)(P.m ? P.m[i] : P.m0) * (vx * vx + vy * vy + vz * vz);
     }
-    block_reduce_stats(v, part);
+    block_reduce_stats(v, part, (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------- grid update ---
@@ -282,13 +287,15 @@ __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> 
     return cf;
 }
 
-// The grid update, one thread per node in node order (x fastest).
-// Contact-force partials per workgroup, summed in a fixed order.
+// The grid update, one thread per node in node order (x fastest), block
+// `blk` of the grid.  Contact-force partial per block, summed in a fixed
+// order.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
-                                                        Tool tool, R dt, int direct_T, double *force_part) {
+__device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Grid<R> &G, const TileParts<R> &tp,
+                                                  const Material<R> &M, const Tool &tool, R dt, int direct_T,
+                                                  double *force_part, int blk) {
     double cf = 0.0;
-    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t idx = (int64_t)blk * kBlock + threadIdx.x;
     if (idx < g.nn) {
         const int i = (int)(idx % g.nx);
         const int64_t row = idx / g.nx;
@@ -303,15 +310,21 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
     if (threadIdx.x == 0) {
         double r = sh[0];
         for (int w = 1; w < kBlock / kWave; ++w) r += sh[w];
-        force_part[blockIdx.x] = r;
+        force_part[blk] = r;
     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
+                                                        Tool tool, R dt, int direct_T, double *force_part) {
+    grid_update_block(g, G, tp, M, tool, dt, direct_T, force_part, (int)blockIdx.x);
 }
 
 // mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, Jacobi form:
 // reads the post-update field (gm, gtm), writes the gather temperature.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R alpha, R dt) {
-    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void diffuse_block(const GridView<R> &g, const Grid<R> &G, R alpha, R dt, int blk) {
+    const int64_t idx = (int64_t)blk * kBlock + threadIdx.x;
     if (idx >= g.nn) return;
     const R mass = G.gm[idx];
     if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
@@ -334,6 +347,75 @@ __global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R 
         tm = rmax(T_new, R(0)) * mass;
     }
     G.node[idx].w = tm / mass;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R alpha, R dt) {
+    diffuse_block(g, G, alpha, dt, (int)blockIdx.x);
+}
+
+// ------------------------------------------------------- batched launches --
+//
+// One launch serves many solvers: work item w owns blocks [block0, block0 +
+// blocks) of the launch and carries the arguments the single-solver kernel
+// would get.  A block finds its item by binary search over block0 (uniform
+// per block) and runs exactly the single-solver body on its local block
+// index, so a solver's results do not depend on what it is batched with.
+
+// Work item of block `b`: the last w with items[w].block0 <= b.
+template <typename Item> __device__ __forceinline__ int find_item(const Item *items, int nitems, int b) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].block0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename R> struct GridWork {
+    GridView<R> g;
+    Grid<R> G;
+    TileParts<R> tp;
+    Material<R> M;
+    Tool tool;
+    R dt, alpha;
+    int32_t direct_T, diffuse;
+    double *force_part;
+    int32_t block0, blocks;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *items, int nitems) {
+    const int w = find_item(items, nitems, (int)blockIdx.x);
+    const GridWork<R> &W = items[w];
+    grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, (int)blockIdx.x - W.block0);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_diffuse_batch(const GridWork<R> *items, int nitems) {
+    const int w = find_item(items, nitems, (int)blockIdx.x);
+    const GridWork<R> &W = items[w];
+    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, (int)blockIdx.x - W.block0);
+}
+
+// One block per item: the fixed-order sum of its contact-force partials.
+struct SumWork {
+    const double *part;
+    int32_t nparts, pad_;
+    const int32_t *modes;
+    double *out;
+};
+
+__global__ void __launch_bounds__(kBlock) k_finalize_batch(const SumWork *items) {
+    const SumWork &W = items[blockIdx.x];
+    finalize_block(W.part, W.nparts, 1, W.modes, W.out);
+}
+
+// Zero counters[1] and counters[2] of every item (the P2G-pass flags).
+__global__ void k_clear_flags_batch(int32_t *const *counters, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { counters[i][1] = 0; counters[i][2] = 0; }
 }
 
 // ------------------------------------------------------------ small ops ---

@@ -60,6 +60,21 @@ inline int blocks_for(int64_t n, int b = kBlock) { return (int)std::max<int64_t>
 
 }  // namespace
 
+// Staging of the work items of one step (see step_many): two pinned host
+// areas used in turn, an event guarding each area's reuse, one device area.
+struct StepStaging {
+    struct Area {
+        void *h = nullptr;
+        size_t cap = 0;
+        hipEvent_t copied = nullptr;
+        bool pending = false;
+    };
+    Area area[2];
+    int parity = 0;
+    void *d_blob = nullptr;
+    size_t d_cap = 0;
+};
+
 struct mpmw_solver {
     mpmw_params prm{};
     hipStream_t stream = nullptr;
@@ -132,6 +147,12 @@ struct mpmw_solver {
     int64_t t_n[8] = {};
 
     int64_t device_bytes = 0;
+
+    // batch membership: while in a batch the solver launches on the batch's
+    // stream; own_stream is its own, restored when it leaves
+    mpmw_batch *batch = nullptr;
+    hipStream_t own_stream = nullptr;
+    StepStaging staging;
 };
 
 template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm, int32_t count, bool host_perm);
@@ -372,7 +393,7 @@ struct Timed {
     mpmw_solver *s;
     mpmw_solver::Ev ev{};
     bool on;
-    Timed(mpmw_solver *s_, int cls) : s(s_), on(s_->timing) {
+    Timed(mpmw_solver *s_, int cls) : s(s_), on(s_ && s_->timing) {
         if (!on) return;
         ev.a = take_event(s);
         ev.b = take_event(s);
@@ -667,6 +688,27 @@ int loose_blocks(const mpmw_solver *s) {
 // One particle pass: the tiles, then the loose particles.  The backstress
 // arrays select the kernel variant; the deterministic flag the P2G form.
 // Statistics partials: tiles first, then the loose blocks.
+// The loose particles of one pass (after the tile pass; statistics
+// partials after the tiles').
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
+int loose_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
+    const int nbl = loose_blocks(s);
+    if (nbl <= 0) return MPMW_OK;
+    const Particles<R> P = pview<R>(s);
+    const GridView<R> g = gview<R>(s);
+    const Grid<R> G = grid_arrays<R>(s);
+    const Material<R> M = material<R>(s);
+    SortState &S = s->sort;
+    hipLaunchKernelGGL((k_loose<R, DO_G2P, DO_P2G && !DET, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream, P, g,
+                       G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)S.geom.ntiles * kRed,
+                       S.loose_begin, S.loose_end, S.oow + 2);
+    if (DET && DO_P2G)
+        hipLaunchKernelGGL(k_loose_p2g_ordered<R>, dim3(1), dim3(kWave), 0, s->stream, P, g, G, S.loose_begin,
+                           S.loose_end, S.oow + 2);
+    HIP_TRY(hipGetLastError());
+    return MPMW_OK;
+}
+
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
 int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     const Particles<R> P = pview<R>(s);
@@ -682,17 +724,7 @@ int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
                        g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
                        (R *)s->tpart);
     HIP_TRY(hipGetLastError());
-    const int nbl = loose_blocks(s);
-    if (nbl > 0) {
-        hipLaunchKernelGGL((k_loose<R, DO_G2P, DO_P2G && !DET, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream,
-                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)tg.ntiles * kRed,
-                           S.loose_begin, S.loose_end, S.oow + 2);
-        if (DET && DO_P2G)
-            hipLaunchKernelGGL(k_loose_p2g_ordered<R>, dim3(1), dim3(kWave), 0, s->stream, P, g, G, S.loose_begin,
-                               S.loose_end, S.oow + 2);
-        HIP_TRY(hipGetLastError());
-    }
-    return MPMW_OK;
+    return loose_pass_t<R, DO_G2P, DO_P2G, HAS_BACK, DET>(s, dt, zcut, fl);
 }
 
 template <typename R, bool DO_G2P, bool DO_P2G>
@@ -738,79 +770,73 @@ template <typename R> int det_settle(mpmw_solver *s, double dt, double zcut) {
     return MPMW_OK;
 }
 
-// MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791.
-template <typename R> int step_impl(mpmw_solver *s, double dt) {
+// Host plan of one step (MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:
+// 727-791), shared by the single-solver step and the batched step: what
+// step_begin prepared and what the launches and step_end need.
+struct StepPlan {
+    bool run = false;         // false: nothing to step (:728 silent return)
+    bool diffusion = false;
+    bool next_rebuild = false;
+    double alpha = 0.0, zcut = 0.0;
+    G2PFlags fl{};
+};
+
+// Everything before the grid update: the rebuild every 25 steps, binning,
+// the P2G when the sums are not ready (first step, after a rebuild or a
+// re-binning) and, in the deterministic mode, its check.
+template <typename R> int step_begin(mpmw_solver *s, double dt, StepPlan &sp) {
+    sp = StepPlan();
     if (s->n <= 0 || s->nn <= 0) return MPMW_OK;  // :728 silent return
     int rc;
     if (s->step % 25 == 0) {
         if ((rc = rebuild<R>(s))) return rc;
     }
     if (!s->sort.valid && (rc = sort_particles<R>(s))) return rc;
-    const GridView<R> g = gview<R>(s);
-    const Grid<R> G = grid_arrays<R>(s);
-    const Material<R> M = material<R>(s);
-    const TileGeom tg = s->sort.geom;
-    const int nb = tg.ntiles + loose_blocks(s);
-    if ((rc = ensure_part(s, nb))) return rc;
-    const double zcut = s->tip[2] - s->h * 1.5;
-    const bool det = s->deterministic;
+    sp.zcut = s->tip[2] - s->h * 1.5;
+    if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
     if (!s->p2g_ready) {
-        if ((rc = p2g_alone<R>(s, dt, zcut))) return rc;
-        if (det && (rc = det_settle<R>(s, dt, zcut))) return rc;
+        if ((rc = p2g_alone<R>(s, dt, sp.zcut))) return rc;
+        if (s->deterministic && (rc = det_settle<R>(s, dt, sp.zcut))) return rc;
     }
     // the binning may have changed in det_settle
-    const TileGeom tgs = s->sort.geom;
+    if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
+    sp.diffusion = s->prm.specific_heat > 0.0 &&
+                   s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
+    sp.alpha = sp.diffusion ? s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) : 0.0;
+    // G2P + chip pass, fused with the next step's P2G unless the next step
+    // rebuilds the grid (then the AABB and metrics are reduced instead)
+    sp.next_rebuild = ((s->step + 1) % 25 == 0);
+    sp.fl.chip = (s->step % 2 == 0);
+    sp.fl.stats = sp.next_rebuild ? 1 : 0;
+    sp.run = true;
+    return MPMW_OK;
+}
+
+template <typename R> TileParts<R> tile_parts(const mpmw_solver *s) {
     TileParts<R> tp;
     tp.part = (const R *)s->tpart;
     tp.tile_start = s->sort.tile_start;
     tp.acc_flag = s->sort.oow + 2;
-    tp.tg = tgs;
-    const bool diffusion = s->prm.specific_heat > 0.0 &&
-                           s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
-    const int gblocks = blocks_for(s->nn);
-    {
-        Timed t(s, MPMW_K_GRID);
-        hipLaunchKernelGGL(k_grid_update<R>, dim3(gblocks), dim3(kBlock), 0, s->stream, g, G, tp, M,
-                           tool_view(s), (R)dt, diffusion ? 0 : 1, s->fpart);
-        HIP_TRY(hipGetLastError());
-    }
-    {
-        Timed t(s, MPMW_K_REDUCE);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, s->fpart, gblocks, 1,
-                           s->d_modes + kRed, s->d_cf);
-        HIP_TRY(hipGetLastError());
-    }
+    tp.tg = s->sort.geom;
+    return tp;
+}
+
+// Everything after the particle pass: statistics, the kinematic tool, time
+// and step count, the deterministic check of the next step's P2G, and the
+// compaction every 1000 steps.
+template <typename R> int step_end(mpmw_solver *s, double dt, const StepPlan &sp) {
+    int rc;
     s->cf_pending = true;
     s->snap_valid = s->snap_on;
-    if (diffusion) {
-        const double alpha = s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat);
-        Timed t(s, MPMW_K_DIFFUSE);
-        hipLaunchKernelGGL(k_diffuse<R>, dim3(blocks_for(s->nn)), dim3(kBlock), 0, s->stream, g, G, (R)alpha,
-                           (R)dt);
-        HIP_TRY(hipGetLastError());
-    }
-    // G2P + chip pass, fused with the next step's P2G unless the next step
-    // rebuilds the grid (then the AABB and metrics are reduced instead).
-    const bool next_rebuild = ((s->step + 1) % 25 == 0);
-    G2PFlags fl;
-    fl.chip = (s->step % 2 == 0);
-    fl.stats = next_rebuild ? 1 : 0;
-    const int nbs = tgs.ntiles + loose_blocks(s);
-    if ((rc = ensure_part(s, nbs))) return rc;
-    {
-        Timed t(s, MPMW_K_G2P_P2G);
-        rc = next_rebuild ? particle_pass<R, true, false>(s, dt, zcut, fl) : particle_pass<R, true, true>(s, dt, zcut, fl);
-        if (rc) return rc;
-    }
-    if (fl.stats) {
+    if (sp.fl.stats) {
         Timed t(s, MPMW_K_REDUCE);
-        if ((rc = finalize_stats(s, nbs))) return rc;
+        if ((rc = finalize_stats(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
         s->stats_fresh = true;
         s->metrics_pending = true;
     } else {
         s->stats_fresh = false;
     }
-    s->p2g_ready = !next_rebuild;
+    s->p2g_ready = !sp.next_rebuild;
     s->sort.steps_since++;
     // kinematic tool
     for (int a = 0; a < 3; ++a) s->pose.center[a] += s->pose.vel[a] * dt;
@@ -818,9 +844,19 @@ template <typename R> int step_impl(mpmw_solver *s, double dt) {
     s->time += dt;
     ++s->step;
     // the next step's P2G: re-bin and repeat it if a particle changed cell
-    if (det && s->p2g_ready && (rc = det_settle<R>(s, dt, zcut))) return rc;
+    if (s->deterministic && s->p2g_ready && (rc = det_settle<R>(s, dt, sp.zcut))) return rc;
     if (s->step % 1000 == 0) return compact_impl<R>(s);  // :788-790
     return MPMW_OK;
+}
+
+template <typename R> int step_many(mpmw_solver *const *members, size_t nm, const double *dts,
+                                   hipStream_t stream, StepStaging &st);
+void free_staging(StepStaging &st);
+
+// MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791: a step of a batch
+// of one, so a solver runs the same kernels alone and in a batch.
+template <typename R> int step_impl(mpmw_solver *s, double dt) {
+    return step_many<R>(&s, 1, &dt, s->stream, s->staging);
 }
 
 // Order-preserving removal of INACTIVE particles (compactParticles :475-527).
@@ -1160,6 +1196,7 @@ int mpmw_create(const mpmw_params *p, mpmw_solver **out) {
 int mpmw_destroy(mpmw_solver *s) {
     if (!s) return MPMW_OK;
     (void)hipSetDevice(s->prm.device);
+    if (s->batch) (void)mpmw_batch_remove(s->batch, s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (auto &e : s->events) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : s->ev_pool) (void)hipEventDestroy(e);
@@ -1178,6 +1215,7 @@ int mpmw_destroy(mpmw_solver *s) {
     dfree(s, s->samples, 0);
     dfree(s, s->tpart, 0);
     free_sort_state(s);
+    free_staging(s->staging);
     if (s->h_stats) (void)hipHostFree(s->h_stats);
     if (s->h_cf) (void)hipHostFree(s->h_cf);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1672,3 +1710,310 @@ extern "C" int mpmw_get_diagnostics(mpmw_solver *s, mpmw_diagnostics *d) {
     }
     return MPMW_OK;
 }
+
+// =============================================================== batch ====
+//
+// Batched stepping: each kernel of the step is launched once for every
+// member over a flattened block list (k_*_batch, find_item); the host work
+// of each member (rebuilds, binning, standalone P2G, statistics) runs as in
+// its own step.  Work items travel in one blob per step, staged through one
+// of two pinned host areas (an event guards each area's reuse).
+
+struct mpmw_batch {
+    int32_t device = 0;
+    bool fp64 = false;
+    hipStream_t stream = nullptr;
+    std::vector<mpmw_solver *> members;
+    StepStaging staging;
+};
+
+namespace {
+
+void batch_detach(mpmw_batch *b, mpmw_solver *s) {
+    (void)hipStreamSynchronize(b->stream);
+    s->stream = s->own_stream;
+    s->own_stream = nullptr;
+    s->batch = nullptr;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename R> void launch_tile_batch(hipStream_t stream, int variant, const TileWork<R> *items, int n,
+                                             int blocks) {
+    const dim3 grid(blocks), blk(kBlock);
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, false>), grid, blk, 0, stream, items, n); break;
+    case 1: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, false>), grid, blk, 0, stream, items, n); break;
+    case 2: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, true>), grid, blk, 0, stream, items, n); break;
+    case 3: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, true>), grid, blk, 0, stream, items, n); break;
+    case 4: hipLaunchKernelGGL((k_tile_batch<R, true, false, false, false>), grid, blk, 0, stream, items, n); break;
+    default: hipLaunchKernelGGL((k_tile_batch<R, true, false, true, false>), grid, blk, 0, stream, items, n); break;
+    }
+}
+
+// Tile-pass variant of a member: fused (0-3: backstress, deterministic) or
+// G2P-only before a rebuild (4-5: backstress).
+int tile_variant(const mpmw_solver *s, const StepPlan &sp) {
+    const int back = s->has_back ? 1 : 0;
+    if (sp.next_rebuild) return 4 + back;
+    return back + (s->deterministic ? 2 : 0);
+}
+
+template <typename R> int loose_after_batch(mpmw_solver *s, double dt, const StepPlan &sp) {
+    const bool back = s->has_back, det = s->deterministic;
+    if (sp.next_rebuild)
+        return back ? loose_pass_t<R, true, false, true, false>(s, dt, sp.zcut, sp.fl)
+                    : loose_pass_t<R, true, false, false, false>(s, dt, sp.zcut, sp.fl);
+    if (back)
+        return det ? loose_pass_t<R, true, true, true, true>(s, dt, sp.zcut, sp.fl)
+                   : loose_pass_t<R, true, true, true, false>(s, dt, sp.zcut, sp.fl);
+    return det ? loose_pass_t<R, true, true, false, true>(s, dt, sp.zcut, sp.fl)
+               : loose_pass_t<R, true, true, false, false>(s, dt, sp.zcut, sp.fl);
+}
+
+// Free the staging areas of a solver or a batch.
+void free_staging(StepStaging &st) {
+    for (auto &a : st.area) {
+        if (a.copied) (void)hipEventDestroy(a.copied);
+        if (a.h) (void)hipHostFree(a.h);
+        a = StepStaging::Area();
+    }
+    if (st.d_blob) (void)hipFree(st.d_blob);
+    st.d_blob = nullptr;
+    st.d_cap = 0;
+}
+
+// One step of every solver in `members` (all of precision R, on `stream`):
+// the host phases of each solver's own step, and each kernel of the step
+// launched once for all of them over a flattened block list.
+template <typename R> int step_many(mpmw_solver *const *members, size_t nm, const double *dts,
+                                   hipStream_t stream, StepStaging &st) {
+    std::vector<StepPlan> plans(nm);
+    int rc;
+    for (size_t m = 0; m < nm; ++m)
+        if ((rc = step_begin<R>(members[m], dts[m], plans[m]))) return rc;
+    std::vector<int> run;
+    for (size_t m = 0; m < nm; ++m)
+        if (plans[m].run) run.push_back((int)m);
+    if (run.empty()) return MPMW_OK;
+    const int nr = (int)run.size();
+
+    // work items
+    std::vector<GridWork<R>> gw(nr);
+    std::vector<SumWork> sw(nr);
+    std::vector<int32_t *> cnt;
+    std::vector<TileWork<R>> tw[6];
+    int gblocks = 0, tblocks[6] = {0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < nr; ++r) {
+        mpmw_solver *s = members[run[r]];
+        const StepPlan &sp = plans[run[r]];
+        const double dt = dts[run[r]];
+        GridWork<R> &g = gw[r];
+        g.g = gview<R>(s);
+        g.G = grid_arrays<R>(s);
+        g.tp = tile_parts<R>(s);
+        g.M = material<R>(s);
+        g.tool = tool_view(s);
+        g.dt = (R)dt;
+        g.alpha = (R)sp.alpha;
+        g.direct_T = sp.diffusion ? 0 : 1;
+        g.diffuse = sp.diffusion ? 1 : 0;
+        g.force_part = s->fpart;
+        g.block0 = gblocks;
+        g.blocks = blocks_for(s->nn);
+        gblocks += g.blocks;
+        sw[r].part = s->fpart;
+        sw[r].nparts = g.blocks;
+        sw[r].modes = s->d_modes + kRed;
+        sw[r].out = s->d_cf;
+        if (!sp.next_rebuild) cnt.push_back(s->sort.oow);
+        const int v = tile_variant(s, sp);
+        TileWork<R> t;
+        t.P = pview<R>(s);
+        t.g = g.g;
+        t.G = g.G;
+        t.M = g.M;
+        t.dt = (R)dt;
+        t.dtd = dt;
+        t.zcut = sp.zcut;
+        t.ref_temp = s->prm.ref_temp;
+        t.flags = sp.fl;
+        t.part = s->part;
+        t.cell_start = s->sort.cell_start;
+        t.tg = s->sort.geom;
+        t.counters = s->sort.oow;
+        t.tpart = (R *)s->tpart;
+        t.block0 = tblocks[v];
+        t.blocks = t.tg.ntiles;
+        tblocks[v] += t.blocks;
+        tw[v].push_back(t);
+    }
+    // one blob: grid items, sum items, counter pointers, tile items per variant
+    size_t off_sw = align256(sizeof(GridWork<R>) * nr);
+    size_t off_cnt = off_sw + align256(sizeof(SumWork) * nr);
+    size_t off_tw[6];
+    size_t end = off_cnt + align256(sizeof(int32_t *) * std::max<size_t>(cnt.size(), 1));
+    for (int v = 0; v < 6; ++v) {
+        off_tw[v] = end;
+        end += align256(sizeof(TileWork<R>) * tw[v].size());
+    }
+    StepStaging::Area &A = st.area[st.parity];
+    st.parity ^= 1;
+    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
+    if (A.pending) {
+        HIP_TRY(hipEventSynchronize(A.copied));
+        A.pending = false;
+    }
+    if (end > A.cap) {
+        if (A.h) (void)hipHostFree(A.h);
+        A.h = nullptr;
+        A.cap = 0;
+        HIP_TRY(hipHostMalloc(&A.h, end * 2, hipHostMallocDefault));
+        A.cap = end * 2;
+    }
+    if (end > st.d_cap) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (st.d_blob) (void)hipFree(st.d_blob);
+        st.d_blob = nullptr;
+        st.d_cap = 0;
+        HIP_TRY(hipMalloc(&st.d_blob, end * 2));
+        st.d_cap = end * 2;
+    }
+    char *h = (char *)A.h;
+    std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
+    std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
+    if (!cnt.empty()) std::memcpy(h + off_cnt, cnt.data(), sizeof(int32_t *) * cnt.size());
+    for (int v = 0; v < 6; ++v)
+        if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
+    HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(A.copied, stream));
+    A.pending = true;
+    char *d = (char *)st.d_blob;
+    const GridWork<R> *d_gw = (const GridWork<R> *)d;
+    // a batch of one keeps the per-kernel timing of its solver
+    mpmw_solver *one = nr == 1 ? members[run[0]] : nullptr;
+    bool any_diffusion = false;
+    for (int r = 0; r < nr; ++r) any_diffusion = any_diffusion || gw[r].diffuse;
+    {
+        Timed t(one, MPMW_K_GRID);
+        hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
+    }
+    {
+        Timed t(one, MPMW_K_REDUCE);
+        hipLaunchKernelGGL(k_finalize_batch, dim3(nr), dim3(kBlock), 0, stream, (const SumWork *)(d + off_sw));
+    }
+    if (any_diffusion) {
+        Timed t(one, MPMW_K_DIFFUSE);
+        hipLaunchKernelGGL(k_diffuse_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
+    }
+    if (!cnt.empty())
+        hipLaunchKernelGGL(k_clear_flags_batch, dim3((int)(cnt.size() + 63) / 64), dim3(64), 0, stream,
+                           (int32_t *const *)(d + off_cnt), (int)cnt.size());
+    {
+        Timed t(one, MPMW_K_G2P_P2G);
+        for (int v = 0; v < 6; ++v)
+            if (!tw[v].empty())
+                launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
+                                     tblocks[v]);
+    }
+    HIP_TRY(hipGetLastError());
+    for (int r = 0; r < nr; ++r)
+        if ((rc = loose_after_batch<R>(members[run[r]], dts[run[r]], plans[run[r]]))) return rc;
+    for (int r = 0; r < nr; ++r)
+        if ((rc = step_end<R>(members[run[r]], dts[run[r]], plans[run[r]]))) return rc;
+    return MPMW_OK;
+}
+
+int check_b(mpmw_batch *b) {
+    if (!b) return fail(MPMW_E_INVAL, "null batch handle");
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return fail(MPMW_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return MPMW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpmw_batch_create(int32_t device, mpmw_batch **out) {
+    if (!out) return fail(MPMW_E_INVAL, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(MPMW_E_HIP, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(MPMW_E_INVAL, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device));
+    mpmw_batch *b = new mpmw_batch();
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return fail(MPMW_E_HIP, "hipStreamCreate failed");
+    }
+    *out = b;
+    return MPMW_OK;
+}
+
+int mpmw_batch_destroy(mpmw_batch *b) {
+    if (!b) return MPMW_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (mpmw_solver *s : b->members) batch_detach(b, s);
+    b->members.clear();
+    free_staging(b->staging);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return MPMW_OK;
+}
+
+int mpmw_batch_add(mpmw_batch *b, mpmw_solver *s) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    if (!s) return fail(MPMW_E_INVAL, "null solver handle");
+    if (s->batch) return fail(MPMW_E_STATE, "solver is already in a batch");
+    if (s->prm.device != b->device) return fail(MPMW_E_INVAL, "solver and batch are on different devices");
+    if (!b->members.empty() && s->fp64 != b->fp64)
+        return fail(MPMW_E_INVAL, "a batch holds solvers of one precision");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (b->members.empty()) b->fp64 = s->fp64;
+    s->own_stream = s->stream;
+    s->stream = b->stream;
+    s->batch = b;
+    b->members.push_back(s);
+    return MPMW_OK;
+}
+
+int mpmw_batch_remove(mpmw_batch *b, mpmw_solver *s) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    auto it = std::find(b->members.begin(), b->members.end(), s);
+    if (!s || it == b->members.end()) return fail(MPMW_E_INVAL, "solver is not in this batch");
+    b->members.erase(it);
+    batch_detach(b, s);
+    return MPMW_OK;
+}
+
+int mpmw_batch_count(mpmw_batch *b, int32_t *n) {
+    if (!b || !n) return fail(MPMW_E_INVAL, "null");
+    *n = (int32_t)b->members.size();
+    return MPMW_OK;
+}
+
+int mpmw_batch_step(mpmw_batch *b, const double *dt, int32_t n) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    if (n != (int32_t)b->members.size()) return fail(MPMW_E_INVAL, "batch_step: one dt per member");
+    if (n == 0) return MPMW_OK;
+    if (!dt) return fail(MPMW_E_INVAL, "null dt");
+    for (int32_t i = 0; i < n; ++i)
+        if (!std::isfinite(dt[i])) return fail(MPMW_E_INVAL, "batch_step: dt must be finite");
+    return b->fp64 ? step_many<double>(b->members.data(), b->members.size(), dt, b->stream, b->staging)
+                   : step_many<float>(b->members.data(), b->members.size(), dt, b->stream, b->staging);
+}
+
+int mpmw_batch_synchronize(mpmw_batch *b) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MPMW_OK;
+}
+
+}  // extern "C"

@@ -385,10 +385,10 @@ template <typename R> struct TileWaves {
 };
 
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
-__global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt,
-                                                 double dtd, double zcut, G2PFlags flags, double ref_temp,
-                                                 double *part, const int32_t *cell_start, TileGeom tg,
-                                                 int32_t *counters, R *tpart) {
+__device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView<R> &g, const Grid<R> &G,
+                                           const Material<R> &M, R dt, double dtd, double zcut, G2PFlags flags,
+                                           double ref_temp, double *part, const int32_t *cell_start,
+                                           const TileGeom &tg, int32_t *counters, R *tpart, int t) {
     static_assert(TX == 8 && TY == 8 && TZ * TX * TY == kBlock, "one 8x8 cell layer per wave");
     constexpr int kNodeBytes = DO_G2P ? WN * (int)sizeof(vec4_t<R>) : 0;
     constexpr int kAccBytes = DO_P2G ? 8 * WN * (int)sizeof(R) : 0;
@@ -401,7 +401,7 @@ __global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile(Particles<
     // statistics are reduced only by the G2P-only pass (the step before a
     // rebuild); the fused pass carries no statistics registers
     const bool stats = DO_G2P && !DO_P2G && flags.stats != 0;
-    const int t = blockIdx.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const int lane = tid % kWave, wv = tid / kWave;
     const int hx = tid % TX, hy = (tid / TX) % TY, hz = tid / (TX * TY);
     const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
@@ -557,7 +557,42 @@ __global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile(Particles<
         if (DET && lane == 0 && cnte) atomicAdd(counters + 1, cnte);
         if (!DET && lane == 0 && cnta) atomicOr(counters + 2, 1);
     }
-    if (stats) block_reduce_stats(st, part);
+    if (stats) block_reduce_stats(st, part, t);
+}
+
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
+__global__ void __launch_bounds__(kBlock, TileWaves<R>::value)
+    k_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt, double dtd, double zcut, G2PFlags flags,
+           double ref_temp, double *part, const int32_t *cell_start, TileGeom tg, int32_t *counters, R *tpart) {
+    tile_block<R, DO_G2P, DO_P2G, HAS_BACK, DET>(P, g, G, M, dt, dtd, zcut, flags, ref_temp, part, cell_start, tg,
+                                                 counters, tpart, (int)blockIdx.x);
+}
+
+// The tile pass of many solvers in one launch (see find_item): item w's
+// tiles are blocks [block0, block0 + ntiles).
+template <typename R> struct TileWork {
+    Particles<R> P;
+    GridView<R> g;
+    Grid<R> G;
+    Material<R> M;
+    R dt;
+    double dtd, zcut, ref_temp;
+    G2PFlags flags;
+    double *part;
+    const int32_t *cell_start;
+    TileGeom tg;
+    int32_t *counters;
+    R *tpart;
+    int32_t block0, blocks;
+};
+
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
+__global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile_batch(const TileWork<R> *items, int nitems) {
+    const int w = find_item(items, nitems, (int)blockIdx.x);
+    const TileWork<R> &W = items[w];
+    tile_block<R, DO_G2P, DO_P2G, HAS_BACK, DET>(W.P, W.g, W.G, W.M, W.dt, W.dtd, W.zcut, W.flags, W.ref_temp, W.part,
+                                                 W.cell_start, W.tg, W.counters, W.tpart,
+                                                 (int)blockIdx.x - W.block0);
 }
 
 // ------------------------------------------------------ loose particles --
@@ -604,7 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_loose(Particles<R> P, GridView<R> g,
             }
         }
     }
-    if (DO_G2P && flags.stats) block_reduce_stats(st, part);
+    if (DO_G2P && flags.stats) block_reduce_stats(st, part, (int)blockIdx.x);
 }
 
 // Deterministic P2G of the loose particles: one wave walks them in memory

@@ -31,6 +31,8 @@ EXPORTS = (
     "mpmw_apply_heat_flux", "mpmw_set_particle_temperatures", "mpmw_set_deterministic",
     "mpmw_get_grid_info", "mpmw_set_grid_snapshot", "mpmw_get_grid_snapshot", "mpmw_set_kernel_timing",
     "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes", "mpmw_get_diagnostics",
+    "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
+    "mpmw_batch_step", "mpmw_batch_synchronize",
 )
 
 
@@ -146,6 +148,13 @@ def lib():
         "mpmw_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "mpmw_device_bytes": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "mpmw_get_diagnostics": (C.c_int, [vp, C.POINTER(Diagnostics)]),
+        "mpmw_batch_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
+        "mpmw_batch_destroy": (C.c_int, [vp]),
+        "mpmw_batch_add": (C.c_int, [vp, vp]),
+        "mpmw_batch_remove": (C.c_int, [vp, vp]),
+        "mpmw_batch_count": (C.c_int, [vp, _ip]),
+        "mpmw_batch_step": (C.c_int, [vp, _dp, C.c_int32]),
+        "mpmw_batch_synchronize": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -369,3 +378,51 @@ class Solver:
         v = C.c_int64()
         _check(lib().mpmw_device_bytes(self.h, C.byref(v)), "device_bytes")
         return v.value
+
+
+class Batch:
+    """Solvers stepped together: each kernel of the step runs once for all
+    members over a flattened block list (mpmw_batch_*).  The batch holds
+    references to its members; removing or closing does not destroy them."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().mpmw_batch_create(device, C.byref(h)), "mpmw_batch_create")
+        self.h = h
+        self.members = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mpmw_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, solver):
+        _check(lib().mpmw_batch_add(self.h, solver.h), "batch_add")
+        self.members.append(solver)
+
+    def remove(self, solver):
+        _check(lib().mpmw_batch_remove(self.h, solver.h), "batch_remove")
+        self.members.remove(solver)
+
+    def count(self):
+        self.members = [m for m in self.members if getattr(m, "h", None)]
+        n = C.c_int32()
+        _check(lib().mpmw_batch_count(self.h, C.byref(n)), "batch_count")
+        return n.value
+
+    def step(self, dt):
+        """One step of every member; dt is a scalar or one value per member."""
+        self.members = [m for m in self.members if getattr(m, "h", None)]  # closed solvers left the batch
+        n = len(self.members)
+        d = np.full(n, float(dt)) if np.isscalar(dt) else np.ascontiguousarray(dt, dtype=np.float64)
+        _check(lib().mpmw_batch_step(self.h, _p(d), n), "batch_step")
+
+    def synchronize(self):
+        _check(lib().mpmw_batch_synchronize(self.h), "batch_synchronize")

@@ -12,6 +12,8 @@ import numpy as np
 H = 2.5e-4
 SEED_SLAB = 0xED6E0001
 SEED_DRILL = 0xED6E0002
+SEED_BATCH = 0xED6E0003
+SEED_SERVE = 0xED6E0004
 
 MATERIAL = dict(
     density=4430.0, young=113.8e9, poisson=0.342, specific_heat=526.0,
@@ -111,6 +113,45 @@ def scaled(cfg, cells, steps=None, name=None):
     return c
 
 
+def wedge_desc_values(rake_deg=5.0, clearance_deg=7.0, friction=0.3):
+    return dict(kind=2, band=0.0, friction=friction, grad_eps=0.05 * H, rake_angle=math.radians(rake_deg),
+                clearance_angle=math.radians(clearance_deg), radius=0.0, helix_angle=0.0, point_angle=0.0,
+                flute_offset=0.0, flute_radius=0.0, flute_length=0.0)
+
+
+def sample_workpiece(rng, index, seed, steps):
+    """One workpiece of the batched / served distributions: N log-uniform in
+    [2e4, 2e6] particles (cells = N/8 in a 2:2:1 box), J-C A, B, n, C, m each
+    uniform within +-15 % of config 0, an analytic 5 deg rake wedge entering at
+    the -x face with cutting speed uniform in [0.5, 5] m/s and depth of cut
+    uniform in [0.1, 1.0] mm, fixture 0.5 mm, dt = 1e-8 s.  The cutting edge
+    starts one cell inside the -x face, so the rake face is in contact from
+    the first step (a 500-step run moves the tool by at most 25 um)."""
+    n = math.exp(rng.uniform(math.log(2e4), math.log(2e6)))
+    a = (n / 32.0) ** (1.0 / 3.0)
+    cx, cy, cz = max(2, int(round(2 * a))), max(2, int(round(2 * a))), max(2, int(round(a)))
+    jc = {k: MATERIAL[k] * rng.uniform(0.85, 1.15) for k in ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")}
+    speed = rng.uniform(0.5, 5.0)
+    doc = min(rng.uniform(0.1e-3, 1.0e-3), 0.6 * cz * H)
+    top = cz * H
+    edge = (H, 0.0, top - doc)
+    return dict(
+        name=f"batch_{index:03d}_{cx}x{cy}x{cz}",
+        cells=(cx, cy, cz), seed=(seed * 0x9E3779B97F4A7C15 + index) & 0xFFFFFFFFFFFFFFFF,
+        origin=(0.0, 0.0, 0.0), vel=(0.0, 0.0, 0.0), temperature=293.0,
+        params=params_dict(fixture=0.5e-3, **jc),
+        tool=("wedge", wedge_desc_values(), dict(center=edge, angle=0.0, vel=(speed, 0.0, 0.0), omega=0.0)),
+        tip=edge, dt=1e-8, steps=steps, speed=speed, depth_of_cut=doc,
+    )
+
+
+def batch_config2(count=64, seed=SEED_BATCH, steps=500):
+    """Config 2: `count` independent workpieces of the batched distribution
+    (sample_workpiece), 500 steps each."""
+    rng = np.random.default_rng(seed)
+    return [sample_workpiece(rng, i, seed, steps) for i in range(count)]
+
+
 def apply_to_solver(mpmw, solver, cfg):
     """Load a configuration into an mpmw.Solver."""
     cx, cy, cz = cfg["cells"]
@@ -118,7 +159,7 @@ def apply_to_solver(mpmw, solver, cfg):
     kind = cfg["tool"][0]
     if kind == "points":
         solver.set_tool_samples(cfg["tool"][1], cfg["tool"][2])
-    elif kind == "drill":
+    elif kind in ("drill", "wedge"):
         d = mpmw.ToolDesc()
         for k, v in cfg["tool"][1].items():
             setattr(d, k, v)

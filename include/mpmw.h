@@ -139,6 +139,7 @@ enum mpmw_kernel_class {
 };
 
 typedef struct mpmw_solver mpmw_solver;
+typedef struct mpmw_batch mpmw_batch;
 
 const char *mpmw_last_error(void);
 int mpmw_abi_version(void);
@@ -221,6 +222,26 @@ int mpmw_get_kernel_times(mpmw_solver *s, mpmw_kernel_times *t);
 int mpmw_get_stream(mpmw_solver *s, void **stream);
 /* Device bytes currently held by the solver. */
 int mpmw_device_bytes(mpmw_solver *s, int64_t *bytes);
+
+/* --- batched stepping (no counterpart in the reference, which steps one
+ *     solver per call).  A batch steps many solvers of one precision on one
+ *     device together: each kernel of the step is launched once for all
+ *     members over a flattened block list, every member running exactly the
+ *     arithmetic of its own mpmw_step, so a member's results are the same
+ *     alone and in any batch (bit for bit in the deterministic mode).
+ *     Members keep their own state, step count and rebuild phase; they may
+ *     join and leave between steps.  While in a batch a solver launches on
+ *     the batch's stream.  Destroying a member removes it; destroying the
+ *     batch removes every member (none is destroyed). */
+int mpmw_batch_create(int32_t device, mpmw_batch **out);
+int mpmw_batch_destroy(mpmw_batch *b);
+int mpmw_batch_add(mpmw_batch *b, mpmw_solver *s);
+int mpmw_batch_remove(mpmw_batch *b, mpmw_solver *s);
+int mpmw_batch_count(mpmw_batch *b, int32_t *n);
+/* One step of every member; dt[i] is the step of the i-th member in the
+ * order they were added (removals close the gaps). */
+int mpmw_batch_step(mpmw_batch *b, const double *dt, int32_t n);
+int mpmw_batch_synchronize(mpmw_batch *b);
 
 /* Layout diagnostics of the particle binning. */
 typedef struct mpmw_diagnostics {
