@@ -22,6 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "edegpredict-engine-rotary_amd")
 sys.path.insert(0, PKG)
 
+import numpy as np  # noqa: E402
+
 import mpmw  # noqa: E402
 import workloads  # noqa: E402
 
@@ -138,6 +140,50 @@ def run_mode(cfg, mode, precision, steps, warmup):
     }
 
 
+def batched_bench(cfgs, precision, steps):
+    """The batched workload (64 workpieces of uneven size): wall time of the
+    serial plain solves (one solver after another, each set up, stepped and
+    synchronised) against the same set-ups stepped together in one batch.
+    Set-up (generating each slab on the device) counts in both."""
+    P = lambda c: dict(c["params"], precision=mpmw.FP64 if precision == "fp64" else mpmw.FP32)
+    nsim = len(cfgs)
+    parts = 0
+    t0 = time.perf_counter()
+    for c in cfgs:
+        s = mpmw.Solver(**P(c))
+        workloads.apply_to_solver(mpmw, s, c)
+        parts += s.count()
+        s.run(steps, c["dt"])
+        s.synchronize()
+        s.close()
+    serial = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    solvers = []
+    b = mpmw.Batch()
+    for c in cfgs:
+        s = mpmw.Solver(**P(c))
+        workloads.apply_to_solver(mpmw, s, c)
+        solvers.append(s)
+        b.add(s)
+    dts = np.array([c["dt"] for c in cfgs])
+    for _ in range(steps):
+        b.step(dts)
+    b.synchronize()
+    batched = time.perf_counter() - t0
+    b.close()
+    for s in solvers:
+        s.close()
+    return {
+        "workload": f"{nsim} workpieces of the batched distribution, {steps} steps each",
+        "particles": parts, "steps": steps,
+        "serial_s": serial, "batched_s": batched,
+        "serial_sims_per_hour": nsim / serial * 3600.0,
+        "batched_sims_per_hour": nsim / batched * 3600.0,
+        "speedup": serial / batched,
+        "batched_particle_steps_per_s": parts * steps / batched,
+    }
+
+
 def measured_traffic():
     """HBM bytes per launch of the fused tile kernel on this workload, from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
@@ -162,6 +208,8 @@ def main():
     ap.add_argument("--cells", type=str, default="256,128,64")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=64, help="workpieces of the batched run (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
@@ -226,6 +274,9 @@ def main():
         q = runs[m]
         out[m] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
                   "fused_kernel_avg_ms": q["k_avg_ms"], "kernels": q["kernels"], "binning": q["diag"]}
+    if args.batch_count > 0:
+        out["batched"] = batched_bench(workloads.batch_config2(args.batch_count, steps=args.batch_steps),
+                                       args.precision, args.batch_steps)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, threads)

@@ -386,14 +386,14 @@ template <typename R> struct GridWork {
 };
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *items, int nitems) {
+__global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *__restrict__ items, int nitems) {
     const int w = find_item(items, nitems, (int)blockIdx.x);
     const GridWork<R> &W = items[w];
     grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, (int)blockIdx.x - W.block0);
 }
 
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_diffuse_batch(const GridWork<R> *items, int nitems) {
+__global__ void __launch_bounds__(kBlock) k_diffuse_batch(const GridWork<R> *__restrict__ items, int nitems) {
     const int w = find_item(items, nitems, (int)blockIdx.x);
     const GridWork<R> &W = items[w];
     if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, (int)blockIdx.x - W.block0);
@@ -407,7 +407,7 @@ struct SumWork {
     double *out;
 };
 
-__global__ void __launch_bounds__(kBlock) k_finalize_batch(const SumWork *items) {
+__global__ void __launch_bounds__(kBlock) k_finalize_batch(const SumWork *__restrict__ items) {
     const SumWork &W = items[blockIdx.x];
     finalize_block(W.part, W.nparts, 1, W.modes, W.out);
 }
@@ -419,6 +419,49 @@ __global__ void k_clear_flags_batch(int32_t *const *counters, int n) {
 }
 
 // ------------------------------------------------------------ small ops ---
+
+// SplitMix64 (Steele, Lea and Flood 2014): the counter-based generator of
+// the jittered slab.
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Jittered slab on the device: particle pi = cell * 8 + k sits in sub-cell k
+// of its cell at origin + (cell + sub / 2 + u / 2) * h (u from SplitMix64 on
+// counter pi * 3 + axis), rounded to float; the same values as the host
+// generator (the product and sum are rounded separately, no FMA).  Every
+// other field gets its initial value.  m and V are not written (uniform).
+template <typename R>
+__global__ void k_init_slab(Particles<R> P, R *dens, uint64_t seed, int32_t cx, int32_t cy, double ox, double oy,
+                            double oz, double h, double z_fix, R vx, R vy, R vz, R T0, R rho, int32_t n) {
+    const int32_t pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= n) return;
+    const int32_t c = pi >> 3, k = pi & 7;
+    const int32_t cell[3] = {c % cx, (c / cx) % cy, c / (cx * cy)};
+    const double org[3] = {ox, oy, oz};
+    double x[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const uint64_t ctr = (uint64_t)((int64_t)pi * 3 + ax);
+        const double u = (double)(splitmix64(seed ^ (ctr * 0xD1B54A32D192ED03ull)) >> 11) * 0x1.0p-53;
+        const double t = ((double)cell[ax] + 0.5 * ((k >> ax) & 1)) + 0.5 * u;
+        x[ax] = (double)(float)__dadd_rn(org[ax], __dmul_rn(t, h));
+    }
+    const bool fixed = x[2] <= z_fix;
+    P.x[pi] = x[0]; P.y[pi] = x[1]; P.z[pi] = x[2];
+    P.vx[pi] = fixed ? R(0) : vx; P.vy[pi] = fixed ? R(0) : vy; P.vz[pi] = fixed ? R(0) : vz;
+    P.s0[pi] = R(0); P.s1[pi] = R(0); P.s2[pi] = R(0); P.s3[pi] = R(0); P.s4[pi] = R(0); P.s5[pi] = R(0);
+    if (P.a0) { P.a0[pi] = R(0); P.a1[pi] = R(0); P.a2[pi] = R(0); P.a3[pi] = R(0); P.a4[pi] = R(0); P.a5[pi] = R(0); }
+    if (P.fx) { P.fx[pi] = R(0); P.fy[pi] = R(0); P.fz[pi] = R(0); }
+    P.eps[pi] = R(0); P.rate[pi] = R(0); P.dmg[pi] = R(0); P.sres[pi] = R(0);
+    P.T[pi] = T0;
+    dens[pi] = rho;
+    P.status[pi] = fixed ? kFixed : kActive;
+    P.id[pi] = pi;
+}
 
 // applyMPMExternalForceKernel, MPMWorkpieceSolver.cu:402-414
 template <typename R>

@@ -17,6 +17,10 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <unordered_map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -161,23 +165,120 @@ namespace {
 
 // ------------------------------------------------------------ allocation --
 
+// Device and pinned host blocks are kept for reuse after release (per
+// device, by size class), and streams come from a small per-device pool:
+// creating a stream or allocating and freeing memory costs milliseconds,
+// which a server admitting and retiring simulations would otherwise pay per
+// request.  Callers release a block only after the stream that used it has
+// drained.  Size classes: 256 B, then multiples of 1/8 of the enclosing power
+// of two (at most 12.5 % over the request).
+namespace pool {
+
+std::mutex mu;
+std::map<std::tuple<int, int, size_t>, std::vector<void *>> free_blocks;
+std::unordered_map<void *, std::tuple<int, int, size_t>> live_blocks;
+std::map<int, std::vector<hipStream_t>> streams;
+std::map<int, size_t> next_stream;
+constexpr size_t kStreamsPerDevice = 4;
+
+size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    size_t p = 1;
+    while (p <= b / 2) p *= 2;
+    const size_t step = p / 8;
+    return (b + step - 1) / step * step;
+}
+
+hipError_t get(int dev, bool host, size_t bytes, void **out) {
+    const size_t c = size_class(bytes);
+    const auto key = std::make_tuple(dev, host ? 1 : 0, c);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = free_blocks.find(key);
+        if (it != free_blocks.end() && !it->second.empty()) {
+            *out = it->second.back();
+            it->second.pop_back();
+            live_blocks[*out] = key;
+            return hipSuccess;
+        }
+    }
+    hipError_t e = host ? hipHostMalloc(out, c, hipHostMallocDefault) : hipMalloc(out, c);
+    if (e != hipSuccess) {
+        // free the cached blocks of this device and try once more
+        std::lock_guard<std::mutex> g(mu);
+        for (auto &kv : free_blocks) {
+            if (std::get<0>(kv.first) != dev || std::get<1>(kv.first) != (host ? 1 : 0)) continue;
+            for (void *q : kv.second) (void)(host ? hipHostFree(q) : hipFree(q));
+            kv.second.clear();
+        }
+        (void)hipGetLastError();
+        e = host ? hipHostMalloc(out, c, hipHostMallocDefault) : hipMalloc(out, c);
+        if (e != hipSuccess) return e;
+        live_blocks[*out] = key;
+        return e;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    live_blocks[*out] = key;
+    return e;
+}
+
+size_t block_size(void *p) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = live_blocks.find(p);
+    return it == live_blocks.end() ? 0 : std::get<2>(it->second);
+}
+
+void put(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = live_blocks.find(p);
+    if (it == live_blocks.end()) return;
+    free_blocks[it->second].push_back(p);
+    live_blocks.erase(it);
+}
+
+// A stream of device `dev` (the current device) from the pool, round robin.
+hipError_t stream(int dev, hipStream_t *out) {
+    std::lock_guard<std::mutex> g(mu);
+    auto &v = streams[dev];
+    if (v.size() < kStreamsPerDevice) {
+        hipStream_t st;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            if (v.empty()) return e;
+        } else {
+            v.push_back(st);
+            *out = st;
+            return hipSuccess;
+        }
+    }
+    *out = v[next_stream[dev]++ % v.size()];
+    return hipSuccess;
+}
+
+}  // namespace pool
+
 int dalloc(mpmw_solver *s, void **p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) return MPMW_OK;
-    hipError_t e = hipMalloc(p, bytes);
+    hipError_t e = pool::get(s->prm.device, false, bytes, p);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
         return fail(MPMW_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     }
-    s->device_bytes += (int64_t)bytes;
+    s->device_bytes += (int64_t)pool::block_size(*p);
     return MPMW_OK;
 }
 
+// Release a device block of the solver to the pool once the solver's stream
+// has drained (work in flight may still use it).
 void dfree(mpmw_solver *s, void *&p, size_t bytes) {
+    (void)bytes;
     if (p) {
-        (void)hipFree(p);
-        s->device_bytes -= (int64_t)bytes;
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        s->device_bytes -= (int64_t)pool::block_size(p);
+        pool::put(p);
     }
     p = nullptr;
 }
@@ -568,7 +669,7 @@ void free_sort_state(mpmw_solver *s) {
     q = S.cell_start; dfree(s, q, 0);
     q = S.tile_start; dfree(s, q, 0);
     q = S.oow; dfree(s, q, 0);
-    if (S.h_oow) (void)hipHostFree(S.h_oow);
+    pool::put(S.h_oow);
     free_scratch(S.temp);
     free_scratch(S.temp2);
     S = SortState();
@@ -597,7 +698,7 @@ template <typename R> int sort_particles(mpmw_solver *s) {
         void *q;
         if ((rc = dalloc(s, &q, 3 * sizeof(int32_t)))) return rc;
         S.oow = (int32_t *)q;
-        HIP_TRY(hipHostMalloc((void **)&S.h_oow, 4 * sizeof(int32_t), hipHostMallocDefault));
+        HIP_TRY(pool::get(s->prm.device, true, 4 * sizeof(int32_t), (void **)&S.h_oow));
         S.h_oow[0] = S.h_oow[1] = S.h_oow[2] = S.h_oow[3] = 0;
         hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)3, 0);
     }
@@ -1077,12 +1178,6 @@ struct Lattice {
     }
 };
 
-uint64_t splitmix64(uint64_t z) {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
 
 }  // namespace
 
@@ -1098,6 +1193,56 @@ template <typename R> int nearest_impl(mpmw_solver *s, double x, double y, doubl
     HIP_TRY(hipGetLastError());
     return MPMW_OK;
 }
+}  // namespace
+
+namespace {
+
+// The slab of mpmw_gen_jittered_slab generated on the device (k_init_slab),
+// then what set_particles does after an upload.
+template <typename R> int init_slab_impl(mpmw_solver *s, uint64_t seed, int32_t cx, int32_t cy, int32_t cz,
+                                         const double origin[3], const double vel[3], double temperature) {
+    const int64_t n = (int64_t)cx * cy * cz * 8;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->metrics_pending) {
+        absorb_stats(s);
+        s->metrics_pending = false;
+    }
+    s->has_back = s->prm.kinematic_hardening > 0.0;
+    s->has_fext = false;
+    s->n = 0;
+    int rc = alloc_particles(s, (int32_t)n);
+    if (rc) return rc;
+    s->n = (int32_t)n;
+    const double h = s->prm.cell_size;
+    const double vol = h * h * h / 8.0;
+    s->uniform_mv = true;
+    s->m0 = s->prm.density * vol;
+    s->V0 = vol;
+    const double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.0;
+    hipLaunchKernelGGL(k_init_slab<R>, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, pview<R>(s),
+                       (R *)s->f[F_DENS], seed, cx, cy, origin[0], origin[1], origin[2], h, origin[2] + fixture,
+                       (R)vel[0], (R)vel[1], (R)vel[2], (R)temperature, (R)s->prm.density, (int32_t)n);
+    HIP_TRY(hipGetLastError());
+    // the per-particle mass and volume arrays (kept for export)
+    hipLaunchKernelGGL(k_fill<R>, dim3(std::min(blocks_for(n), 4096)), dim3(kBlock), 0, s->stream, (R *)s->f[F_M],
+                       n, (R)s->m0);
+    hipLaunchKernelGGL(k_fill<R>, dim3(std::min(blocks_for(n), 4096)), dim3(kBlock), 0, s->stream, (R *)s->f[F_V],
+                       n, (R)s->V0);
+    HIP_TRY(hipGetLastError());
+    s->p2g_ready = false;
+    s->stats_fresh = false;
+    s->sort.valid = false;
+    if ((rc = reduce_particles<R>(s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->stats_fresh = true;
+    {
+        const double *v = s->h_stats;
+        s->bounds[0] = v[0]; s->bounds[1] = v[1]; s->bounds[2] = v[2];
+        s->bounds[3] = -v[3]; s->bounds[4] = -v[4]; s->bounds[5] = -v[5];
+    }
+    return rebuild_grid_from_stats(s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1164,8 +1309,7 @@ int mpmw_create(const mpmw_params *p, mpmw_solver **out) {
     s->max_temp = p->ref_temp;
     int rc = MPMW_OK;
     auto bail = [&](int code) { mpmw_destroy(s); return code; };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(MPMW_E_HIP, "hipStreamCreate failed"));
+    if (pool::stream(p->device, &s->stream) != hipSuccess) return bail(fail(MPMW_E_HIP, "hipStreamCreate failed"));
     void *q;
     if ((rc = dalloc(s, &q, sizeof(double) * kRed))) return bail(rc);
     s->d_stats = (double *)q;
@@ -1176,8 +1320,8 @@ int mpmw_create(const mpmw_params *p, mpmw_solver **out) {
     int32_t modes[kRed + 1] = {0, 0, 0, 0, 0, 0, 1, 1, 2, 2};
     if (hipMemcpy(s->d_modes, modes, sizeof(modes), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(MPMW_E_HIP, "hipMemcpy modes failed"));
-    if (hipHostMalloc((void **)&s->h_stats, sizeof(double) * kRed, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_cf, sizeof(double), hipHostMallocDefault) != hipSuccess)
+    if (pool::get(p->device, true, sizeof(double) * kRed, (void **)&s->h_stats) != hipSuccess ||
+        pool::get(p->device, true, sizeof(double), (void **)&s->h_cf) != hipSuccess)
         return bail(fail(MPMW_E_HIP, "hipHostMalloc failed"));
     std::memset(s->h_stats, 0, sizeof(double) * kRed);
     *s->h_cf = 0.0;
@@ -1216,9 +1360,8 @@ int mpmw_destroy(mpmw_solver *s) {
     dfree(s, s->tpart, 0);
     free_sort_state(s);
     free_staging(s->staging);
-    if (s->h_stats) (void)hipHostFree(s->h_stats);
-    if (s->h_cf) (void)hipHostFree(s->h_cf);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    pool::put(s->h_stats);
+    pool::put(s->h_cf);
     delete s;
     return MPMW_OK;
 }
@@ -1316,6 +1459,7 @@ int mpmw_init_cylinder(mpmw_solver *s, const double c[3], double radius, double 
 int mpmw_gen_jittered_slab(uint64_t seed, int32_t cx, int32_t cy, int32_t cz, double h, const double origin[3],
                            double density, double fixture, double *pos, double *mass, double *volume, int32_t *id,
                            int32_t *status) {
+#pragma clang fp contract(off)
     if (cx <= 0 || cy <= 0 || cz <= 0 || !(h > 0) || !origin || !pos || !mass || !volume || !id || !status)
         return fail(MPMW_E_INVAL, "gen_jittered_slab: bad arguments");
     const int64_t ncell = (int64_t)cx * cy * cz;
@@ -1329,7 +1473,8 @@ int mpmw_gen_jittered_slab(uint64_t seed, int32_t cx, int32_t cy, int32_t cz, do
             for (int ax = 0; ax < 3; ++ax) {
                 const uint64_t ctr = (uint64_t)(pi * 3 + ax);
                 const double u = (double)(splitmix64(seed ^ (ctr * 0xD1B54A32D192ED03ull)) >> 11) * 0x1.0p-53;
-                const double x = origin[ax] + ((double)cell[ax] + 0.5 * sub[ax] + 0.5 * u) * h;
+                const double t = ((double)cell[ax] + 0.5 * sub[ax]) + 0.5 * u;
+                const double x = origin[ax] + t * h;  // two roundings, as k_init_slab
                 pos[3 * pi + ax] = (double)(float)x;
             }
             mass[pi] = density * vol;
@@ -1347,20 +1492,9 @@ int mpmw_init_jittered_slab(mpmw_solver *s, uint64_t seed, int32_t cx, int32_t c
     if (!origin || !vel) return fail(MPMW_E_INVAL, "init_jittered_slab: null argument");
     const int64_t n = (int64_t)cx * cy * cz * 8;
     if (cx <= 0 || cy <= 0 || cz <= 0 || n > INT32_MAX) return fail(MPMW_E_RANGE, "init_jittered_slab: bad size");
-    std::vector<double> pos(3 * n), mass(n), vol(n), velv(3 * n), T(n, temperature), z1(n, 0.0), z6(6 * n, 0.0);
-    std::vector<int32_t> id(n), st(n);
-    const double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.0;
-    int rc = mpmw_gen_jittered_slab(seed, cx, cy, cz, s->prm.cell_size, origin, s->prm.density, fixture,
-                                    pos.data(), mass.data(), vol.data(), id.data(), st.data());
-    if (rc) return rc;
-    for (int64_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) velv[3 * i + a] = (st[i] == MPMW_FIXED_BOUNDARY) ? 0.0 : vel[a];
-    mpmw_particle_arrays a{};
-    a.n = (int32_t)n;
-    a.pos = pos.data(); a.vel = velv.data(); a.mass = mass.data(); a.volume = vol.data();
-    a.temperature = T.data(); a.plastic_strain = z1.data(); a.strain_rate = z1.data(); a.damage = z1.data();
-    a.residual_stress = z1.data(); a.stress = z6.data(); a.id = id.data(); a.status = st.data();
-    return mpmw_set_particles(s, &a);
+    if (!finite_array(origin, 3) || !finite_array(vel, 3) || !std::isfinite(temperature))
+        return fail(MPMW_E_INVAL, "init_jittered_slab: non-finite argument");
+    return DISPATCH(s, init_slab_impl, s, seed, cx, cy, cz, origin, vel, temperature);
 }
 
 int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double contact_radius) {
@@ -1775,10 +1909,10 @@ template <typename R> int loose_after_batch(mpmw_solver *s, double dt, const Ste
 void free_staging(StepStaging &st) {
     for (auto &a : st.area) {
         if (a.copied) (void)hipEventDestroy(a.copied);
-        if (a.h) (void)hipHostFree(a.h);
+        pool::put(a.h);
         a = StepStaging::Area();
     }
-    if (st.d_blob) (void)hipFree(st.d_blob);
+    pool::put(st.d_blob);
     st.d_blob = nullptr;
     st.d_cap = 0;
 }
@@ -1865,18 +1999,22 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         A.pending = false;
     }
     if (end > A.cap) {
-        if (A.h) (void)hipHostFree(A.h);
+        pool::put(A.h);
         A.h = nullptr;
         A.cap = 0;
-        HIP_TRY(hipHostMalloc(&A.h, end * 2, hipHostMallocDefault));
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(pool::get(dev, true, end * 2, &A.h));
         A.cap = end * 2;
     }
     if (end > st.d_cap) {
         HIP_TRY(hipStreamSynchronize(stream));
-        if (st.d_blob) (void)hipFree(st.d_blob);
+        pool::put(st.d_blob);
         st.d_blob = nullptr;
         st.d_cap = 0;
-        HIP_TRY(hipMalloc(&st.d_blob, end * 2));
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(pool::get(dev, false, end * 2, &st.d_blob));
         st.d_cap = end * 2;
     }
     char *h = (char *)A.h;
@@ -1944,7 +2082,7 @@ int mpmw_batch_create(int32_t device, mpmw_batch **out) {
     HIP_TRY(hipSetDevice(device));
     mpmw_batch *b = new mpmw_batch();
     b->device = device;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (pool::stream(device, &b->stream) != hipSuccess) {
         delete b;
         return fail(MPMW_E_HIP, "hipStreamCreate failed");
     }
@@ -1959,7 +2097,6 @@ int mpmw_batch_destroy(mpmw_batch *b) {
     for (mpmw_solver *s : b->members) batch_detach(b, s);
     b->members.clear();
     free_staging(b->staging);
-    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MPMW_OK;
 }

@@ -587,9 +587,14 @@ template <typename R> struct TileWork {
 };
 
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
-__global__ void __launch_bounds__(kBlock, TileWaves<R>::value) k_tile_batch(const TileWork<R> *items, int nitems) {
+__global__ void __launch_bounds__(kBlock, TileWaves<R>::value)
+    k_tile_batch(const TileWork<R> *__restrict__ items, int nitems) {
     const int w = find_item(items, nitems, (int)blockIdx.x);
+#ifdef MPMW_ITEM_COPY
+    const TileWork<R> W = items[w];
+#else
     const TileWork<R> &W = items[w];
+#endif
     tile_block<R, DO_G2P, DO_P2G, HAS_BACK, DET>(W.P, W.g, W.G, W.M, W.dt, W.dtd, W.zcut, W.flags, W.ref_temp, W.part,
                                                  W.cell_start, W.tg, W.counters, W.tpart,
                                                  (int)blockIdx.x - W.block0);

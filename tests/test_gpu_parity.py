@@ -237,3 +237,28 @@ def test_config1_full_size_invariants():
         assert np.isfinite(d[k]).all(), k
     m = s.metrics()
     assert m.step == 30 and m.contact_force > 0.0
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_device_slab_matches_host_generator(prec):
+    """init_jittered_slab generates the slab on the device; positions, ids,
+    statuses, velocities and mass must equal the host generator (itself
+    pinned bit for bit against the oracle's)."""
+    import mpmw
+    cfg = workloads.scaled(workloads.drill_config1(), (13, 7, 9), steps=1)
+    P = dict(cfg["params"], precision=mpmw.FP64 if prec == "fp64" else mpmw.FP32)
+    s = mpmw.Solver(**P)
+    vel = (0.5, -0.25, 0.125)
+    s.init_jittered_slab(cfg["seed"], 13, 7, 9, (1e-3, -2e-3, 5e-4), vel, 301.0)
+    g = s.get_particles()
+    h = mpmw.gen_jittered_slab(cfg["seed"], 13, 7, 9, workloads.H, (1e-3, -2e-3, 5e-4), P["density"],
+                               P["fixture_thickness"])
+    assert np.array_equal(g["pos"], h["pos"])
+    assert np.array_equal(g["id"], h["id"]) and np.array_equal(g["status"], h["status"])
+    fixed = h["status"] == mpmw.FIXED_BOUNDARY
+    assert fixed.any() and (~fixed).any()
+    assert np.all(g["vel"][fixed] == 0.0)
+    want = np.array(vel, dtype=np.float32 if prec == "fp32" else np.float64).astype(np.float64)
+    assert np.all(g["vel"][~fixed] == want)
+    assert np.allclose(g["mass"], h["mass"], rtol=1e-7) and np.all(g["temperature"] == 301.0)
+    s.close()
