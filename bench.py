@@ -25,6 +25,7 @@ sys.path.insert(0, PKG)
 import numpy as np  # noqa: E402
 
 import mpmw  # noqa: E402
+import serve  # noqa: E402
 import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -210,6 +211,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=64, help="workpieces of the batched run (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="seconds of Poisson arrivals (20/s) for the served run (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
@@ -277,6 +280,15 @@ def main():
     if args.batch_count > 0:
         out["batched"] = batched_bench(workloads.batch_config2(args.batch_count, steps=args.batch_steps),
                                        args.precision, args.batch_steps)
+    if args.serve_seconds > 0:
+        sv = {"workload": f"Poisson stream of 20 requests/s for {args.serve_seconds:g} s, sizes and parameters of "
+                          "the batched distribution, 200-2000 steps each"}
+        sv["served"] = serve.summary(serve.serve_batched(serve.request_stream(duration=args.serve_seconds),
+                                                         args.precision))
+        sv["serial"] = serve.summary(serve.serve_serial(serve.request_stream(duration=args.serve_seconds),
+                                                        args.precision))
+        sv["sims_per_hour_ratio"] = sv["served"]["sims_per_hour"] / sv["serial"]["sims_per_hour"]
+        out["served"] = sv
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, threads)

@@ -2082,7 +2082,9 @@ int mpmw_batch_create(int32_t device, mpmw_batch **out) {
     HIP_TRY(hipSetDevice(device));
     mpmw_batch *b = new mpmw_batch();
     b->device = device;
-    if (pool::stream(device, &b->stream) != hipSuccess) {
+    // a stream of its own: members being set up on pooled streams do not
+    // wait for the batch's queued steps
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
         delete b;
         return fail(MPMW_E_HIP, "hipStreamCreate failed");
     }
@@ -2097,6 +2099,7 @@ int mpmw_batch_destroy(mpmw_batch *b) {
     for (mpmw_solver *s : b->members) batch_detach(b, s);
     b->members.clear();
     free_staging(b->staging);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MPMW_OK;
 }
