@@ -14,7 +14,7 @@
 namespace mpmw {
 
 enum : int32_t { kActive = 0, kInactive = 1, kChip = 2, kBoundary = 3, kFixed = 4 };
-enum : int32_t { kToolNone = 0, kToolPoints = 1, kToolWedge = 2, kToolDrill = 3 };
+enum : int32_t { kToolNone = 0, kToolPoints = 1, kToolWedge = 2, kToolDrill = 3, kToolVoxel = 4 };
 
 template <typename R> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
@@ -104,6 +104,11 @@ struct Tool {
     double band, mu, eps;
     double sin_rake, cos_rake, sin_clear, cos_clear;
     double radius, tan_helix, sin_half_point, cos_half_point, flute_offset, flute_radius, flute_length;
+    // voxel SDF (rest frame): value of voxel (i, j, k) at origin + (i, j, k
+    // + 0.5) * voxel, index i + nx * (j + ny * k)
+    const float *sdf;
+    int32_t snx, sny, snz, pad_;
+    double sox, soy, soz, svox;
 };
 
 // --------------------------------------------------------------- helpers --
@@ -225,9 +230,64 @@ __device__ __forceinline__ double phi_local(const Tool &t, const double q[3]) {
     return fmax(body, -gmin);
 }
 
+// Voxel SDF sampler (the ToolSDF texture, ToolSDF.cu:534-567): trilinear over
+// voxel centres with the edge voxels repeated outside (clamp addressing),
+// exact weights in FP64 over the FP32 voxel values.
+__device__ __forceinline__ double sdf_sample(const Tool &t, const double q[3]) {
+    const double org[3] = {t.sox, t.soy, t.soz};
+    const int dims[3] = {t.snx, t.sny, t.snz};
+    int i0[3], i1[3];
+    double f[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double u = (q[k] - org[k]) / t.svox - 0.5;
+        const double fl = floor(u);
+        f[k] = u - fl;
+        const int a = fl < -1.0 ? -1 : (fl > (double)dims[k] ? dims[k] : (int)fl);
+        i0[k] = min(max(a, 0), dims[k] - 1);
+        i1[k] = min(max(a + 1, 0), dims[k] - 1);
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
+        const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+        const int64_t idx = (dx ? i1[0] : i0[0]) +
+                            (int64_t)t.snx * ((dy ? i1[1] : i0[1]) + (int64_t)t.sny * (dz ? i1[2] : i0[2]));
+        v += w * (double)t.sdf[idx];
+    }
+    return v;
+}
+
+// The sampler's range test (ContactSolver.cu:837-840): texel coordinate
+// (q - origin) / voxel in [0, n - 1] on every axis.
+__device__ __forceinline__ bool sdf_in_range(const Tool &t, const double q[3]) {
+    const double org[3] = {t.sox, t.soy, t.soz};
+    const int dims[3] = {t.snx, t.sny, t.snz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double u = (q[k] - org[k]) / t.svox;
+        if (!(u >= 0.0) || u > (double)(dims[k] - 1)) return false;
+    }
+    return true;
+}
+
+// phi at world point (x, y, z); the voxel tool has no contact outside its
+// texture range.
 __device__ __forceinline__ double tool_phi(const Tool &t, double x, double y, double z) {
     double q[3];
     tool_frame(t, x, y, z, q);
+    if (t.kind == kToolVoxel) return sdf_in_range(t, q) ? sdf_sample(t, q) : 1e300;
+    return phi_local(t, q);
+}
+
+// phi for the normal's central differences: the voxel sampler clamps
+// instead of the range test, as the reference's gradient samples do
+// (ContactSolver.cu:848-859).
+__device__ __forceinline__ double tool_phi_grad(const Tool &t, double x, double y, double z) {
+    double q[3];
+    tool_frame(t, x, y, z, q);
+    if (t.kind == kToolVoxel) return sdf_sample(t, q);
     return phi_local(t, q);
 }
 

@@ -239,14 +239,17 @@ __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> 
                             }
                         }
                     }
-                } else if (tool.kind == kToolWedge ||
+                } else if (tool.kind == kToolWedge || tool.kind == kToolVoxel ||
                            (tool.kind == kToolDrill && !drill_far(tool, px, py, pz))) {
                     const double phi = tool_phi(tool, px, py, pz);
                     if (phi < tool.band) {
                         const double e = tool.eps;
-                        double gx = (tool_phi(tool, px + e, py, pz) - tool_phi(tool, px - e, py, pz)) / (2.0 * e);
-                        double gy = (tool_phi(tool, px, py + e, pz) - tool_phi(tool, px, py - e, pz)) / (2.0 * e);
-                        double gz = (tool_phi(tool, px, py, pz + e) - tool_phi(tool, px, py, pz - e)) / (2.0 * e);
+                        double gx = (tool_phi_grad(tool, px + e, py, pz) - tool_phi_grad(tool, px - e, py, pz)) /
+                                    (2.0 * e);
+                        double gy = (tool_phi_grad(tool, px, py + e, pz) - tool_phi_grad(tool, px, py - e, pz)) /
+                                    (2.0 * e);
+                        double gz = (tool_phi_grad(tool, px, py, pz + e) - tool_phi_grad(tool, px, py, pz - e)) /
+                                    (2.0 * e);
                         double len = sqrt(gx * gx + gy * gy + gz * gz);
                         if (len > 1e-12) {
                             R nx = (R)(gx / len), ny = (R)(gy / len), nz = (R)(gz / len);

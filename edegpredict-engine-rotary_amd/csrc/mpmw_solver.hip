@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "mpmw.h"
+#include "mpmw_sdf.hpp"
 #include "mpmw_tile.hpp"
 
 using namespace mpmw;
@@ -122,6 +123,9 @@ struct mpmw_solver {
     double sample_lo[3] = {0, 0, 0}, sample_hi[3] = {0, 0, 0};
     mpmw_tool_desc desc{};
     mpmw_tool_pose pose{};
+    void *sdf = nullptr;           // voxel SDF of the tool mesh (float, x fastest)
+    int32_t sdf_dims[3] = {0, 0, 0};
+    double sdf_origin[3] = {0, 0, 0}, sdf_voxel = 0.0;
     double tip[3] = {0, 0, 0};
 
     // stepping state
@@ -461,7 +465,7 @@ Tool tool_view(const mpmw_solver *s) {
         t.sz = t.sy + s->nsamples;
         for (int a = 0; a < 3; ++a) { t.lo[a] = s->sample_lo[a]; t.hi[a] = s->sample_hi[a]; }
         t.rc2 = s->prm.tool_contact_radius * s->prm.tool_contact_radius;
-    } else if (t.kind == MPMW_TOOL_WEDGE || t.kind == MPMW_TOOL_DRILL) {
+    } else if (t.kind == MPMW_TOOL_WEDGE || t.kind == MPMW_TOOL_DRILL || t.kind == MPMW_TOOL_VOXEL) {
         const mpmw_tool_desc &d = s->desc;
         const mpmw_tool_pose &p = s->pose;
         t.cx = p.center[0]; t.cy = p.center[1]; t.cz = p.center[2];
@@ -473,6 +477,12 @@ Tool tool_view(const mpmw_solver *s) {
         t.radius = d.radius; t.tan_helix = std::tan(d.helix_angle);
         t.sin_half_point = std::sin(0.5 * d.point_angle); t.cos_half_point = std::cos(0.5 * d.point_angle);
         t.flute_offset = d.flute_offset; t.flute_radius = d.flute_radius; t.flute_length = d.flute_length;
+        if (t.kind == MPMW_TOOL_VOXEL) {
+            t.sdf = (const float *)s->sdf;
+            t.snx = s->sdf_dims[0]; t.sny = s->sdf_dims[1]; t.snz = s->sdf_dims[2];
+            t.sox = s->sdf_origin[0]; t.soy = s->sdf_origin[1]; t.soz = s->sdf_origin[2];
+            t.svox = s->sdf_voxel;
+        }
     }
     return t;
 }
@@ -1357,6 +1367,7 @@ int mpmw_destroy(mpmw_solver *s) {
     q = s->d_near_out; dfree(s, q, 0);
     q = s->d_near_out_i; dfree(s, q, 0);
     dfree(s, s->samples, 0);
+    dfree(s, s->sdf, 0);
     dfree(s, s->tpart, 0);
     free_sort_state(s);
     free_staging(s->staging);
@@ -1545,13 +1556,163 @@ int mpmw_set_tool_analytic(mpmw_solver *s, const mpmw_tool_desc *d) {
         s->tool_kind = MPMW_TOOL_NONE;
         return MPMW_OK;
     }
-    if (d->kind != MPMW_TOOL_WEDGE && d->kind != MPMW_TOOL_DRILL)
-        return fail(MPMW_E_INVAL, "tool kind must be NONE, WEDGE or DRILL here");
+    if (d->kind == MPMW_TOOL_VOXEL && !s->sdf)
+        return fail(MPMW_E_STATE, "voxel tool without an SDF: build one with mpmw_set_tool_sdf_mesh");
+    if (d->kind != MPMW_TOOL_WEDGE && d->kind != MPMW_TOOL_DRILL && d->kind != MPMW_TOOL_VOXEL)
+        return fail(MPMW_E_INVAL, "tool kind must be NONE, WEDGE, DRILL or VOXEL here");
     if (!(d->grad_eps > 0.0)) return fail(MPMW_E_INVAL, "tool grad_eps must be > 0");
     if (d->kind == MPMW_TOOL_DRILL && !(d->radius > 0.0)) return fail(MPMW_E_INVAL, "drill radius must be > 0");
     if (d->friction < 0.0) return fail(MPMW_E_INVAL, "friction must be >= 0");
     s->desc = *d;
     s->tool_kind = d->kind;
+    return MPMW_OK;
+}
+
+int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double *verts, int32_t nverts,
+                           const int32_t *tris, const double *normals, int32_t ntris, double padding,
+                           int32_t voxels_per_axis) {
+#pragma clang fp contract(off)
+    CHECK_S(s);
+    if (!d || !verts || !tris || !normals) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: null argument");
+    // build, ToolSDF.cu:219-222: an empty mesh builds nothing
+    if (nverts <= 0 || ntris <= 0) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: empty mesh");
+    if (voxels_per_axis <= 0) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: voxels_per_axis must be > 0");
+    if (!(d->grad_eps > 0.0)) return fail(MPMW_E_INVAL, "tool grad_eps must be > 0");
+    if (d->friction < 0.0) return fail(MPMW_E_INVAL, "friction must be >= 0");
+    if (!finite_array(verts, 3 * (int64_t)nverts) || !finite_array(normals, 3 * (int64_t)ntris) ||
+        !std::isfinite(padding))
+        return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: non-finite input");
+    for (int64_t i = 0; i < 3 * (int64_t)ntris; ++i)
+        if (tris[i] < 0 || tris[i] >= nverts) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: vertex index out of range");
+    // box, voxel size and dimensions in float (:224-266)
+    const float fmax_ = std::numeric_limits<float>::max();
+    float lo[3] = {fmax_, fmax_, fmax_}, hi[3] = {-fmax_, -fmax_, -fmax_};
+    for (int32_t i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(lo[k], (float)verts[3 * i + k]);
+            hi[k] = std::fmax(hi[k], (float)verts[3 * i + k]);
+        }
+    const float pad = (float)padding;
+    for (int k = 0; k < 3; ++k) { lo[k] -= pad; hi[k] += pad; }
+    const float ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    const float vox = std::fmax(ext[0], std::fmax(ext[1], ext[2])) / (float)voxels_per_axis;
+    if (!(vox > 0.0f) || !std::isfinite(vox)) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: degenerate mesh box");
+    int dims[3];
+    for (int k = 0; k < 3; ++k) dims[k] = std::min(256, (int)(ext[k] / vox) + 2);
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const int64_t total = (int64_t)nx * ny * nz;
+    // triangles with their boxes (buildFromVerts, :348-378) and the voxel
+    // index boxes of the scatter (computeSDFCPU, :394-408)
+    std::vector<SdfTri> T((size_t)ntris);
+    const float inv = 1.0f / vox;
+    for (int32_t t = 0; t < ntris; ++t) {
+        SdfTri &q = T[(size_t)t];
+        for (int k = 0; k < 3; ++k) {
+            q.a[k] = (float)verts[3 * tris[3 * t + 0] + k];
+            q.b[k] = (float)verts[3 * tris[3 * t + 1] + k];
+            q.c[k] = (float)verts[3 * tris[3 * t + 2] + k];
+            q.n[k] = (float)normals[3 * t + k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            q.lo[k] = std::fmin(q.a[k], std::fmin(q.b[k], q.c[k]));
+            q.hi[k] = std::fmax(q.a[k], std::fmax(q.b[k], q.c[k]));
+            q.i0[k] = std::max(0, (int)((q.lo[k] - lo[k]) * inv) - 1);
+            q.i1[k] = std::min(dims[k] - 1, (int)((q.hi[k] - lo[k]) * inv) + 2);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    void *d_tris = nullptr, *d_bits = nullptr, *d_grid = nullptr;
+    auto release = [&]() {
+        dfree(s, d_tris, 0);
+        dfree(s, d_bits, 0);
+        dfree(s, d_grid, 0);
+    };
+    int rc = dalloc(s, &d_tris, sizeof(SdfTri) * (size_t)ntris);
+    if (!rc) rc = dalloc(s, &d_bits, sizeof(uint32_t) * (size_t)total);
+    if (!rc) rc = dalloc(s, &d_grid, sizeof(float) * (size_t)total);
+    if (rc) {
+        release();
+        return rc;
+    }
+    std::vector<float> G((size_t)total);
+    uint32_t big_bits;
+    std::memcpy(&big_bits, &fmax_, 4);
+    hipError_t e = hipMemcpyAsync(d_tris, T.data(), sizeof(SdfTri) * T.size(), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        const unsigned nb = (unsigned)((total + 255) / 256);
+        k_sdf_fill<<<nb, 256, 0, s->stream>>>((uint32_t *)d_bits, total, big_bits);
+        k_sdf_scatter<<<(unsigned)ntris, 256, 0, s->stream>>>((const SdfTri *)d_tris, lo[0], lo[1], lo[2], vox, nx,
+                                                              ny, (uint32_t *)d_bits);
+        k_sdf_sign<<<nb, 256, 0, s->stream>>>((const SdfTri *)d_tris, ntris, lo[0], lo[1], lo[2], vox, nx, ny, total,
+                                              vox * 2.0f, (const uint32_t *)d_bits, (float *)d_grid);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(G.data(), d_grid, sizeof(float) * (size_t)total, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->stream);
+        release();
+        return fail(MPMW_E_HIP, std::string("set_tool_sdf_mesh: ") + hipGetErrorString(e));
+    }
+    // voxels no triangle reached: the smallest |value| of the 26 neighbours
+    // plus half a voxel, three in-place passes over the interior in index
+    // order (:489-520), then the rest at voxel * max(dims) (:523-527)
+    const float half = vox * 0.5f;
+    for (int pass = 0; pass < 3; ++pass)
+        for (int iz = 1; iz < nz - 1; ++iz)
+            for (int iy = 1; iy < ny - 1; ++iy)
+                for (int ix = 1; ix < nx - 1; ++ix) {
+                    const int64_t idx = ix + (int64_t)nx * (iy + (int64_t)ny * iz);
+                    if (G[(size_t)idx] < fmax_) continue;
+                    float nearest = fmax_;
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                if (dx == 0 && dy == 0 && dz == 0) continue;
+                                const float v = std::fabs(G[(size_t)(idx + dx + (int64_t)nx * (dy + (int64_t)ny * dz))]);
+                                if (v < nearest) nearest = v;
+                            }
+                    if (nearest < fmax_) G[(size_t)idx] = nearest + half;
+                }
+    const float rest = vox * (float)std::max(nx, std::max(ny, nz));
+    for (auto &v : G)
+        if (v >= fmax_) v = rest;
+    e = hipMemcpyAsync(d_grid, G.data(), sizeof(float) * (size_t)total, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(s->stream);
+        release();
+        return fail(MPMW_E_HIP, std::string("set_tool_sdf_mesh: ") + hipGetErrorString(e));
+    }
+    dfree(s, s->sdf, 0);
+    s->sdf = d_grid;
+    d_grid = nullptr;
+    release();
+    for (int k = 0; k < 3; ++k) {
+        s->sdf_dims[k] = dims[k];
+        s->sdf_origin[k] = (double)lo[k];
+    }
+    s->sdf_voxel = (double)vox;
+    s->desc = *d;
+    s->desc.kind = MPMW_TOOL_VOXEL;
+    s->tool_kind = MPMW_TOOL_VOXEL;
+    return MPMW_OK;
+}
+
+int mpmw_get_tool_sdf(mpmw_solver *s, float *grid, int32_t dims[3], double origin[3], double *voxel) {
+    CHECK_S(s);
+    if (!s->sdf) return fail(MPMW_E_STATE, "no voxel SDF built");
+    if (dims)
+        for (int k = 0; k < 3; ++k) dims[k] = s->sdf_dims[k];
+    if (origin)
+        for (int k = 0; k < 3; ++k) origin[k] = s->sdf_origin[k];
+    if (voxel) *voxel = s->sdf_voxel;
+    if (grid) {
+        const size_t n = (size_t)s->sdf_dims[0] * s->sdf_dims[1] * s->sdf_dims[2];
+        HIP_TRY(hipMemcpyAsync(grid, s->sdf, sizeof(float) * n, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    }
     return MPMW_OK;
 }
 

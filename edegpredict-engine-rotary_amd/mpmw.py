@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmpmw_hip.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
 FP32, FP64 = 0, 1
-TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL = 0, 1, 2, 3
+TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
 K_G2P_P2G, K_P2G, K_GRID, K_DIFFUSE, K_SORT, K_REDUCE, K_OTHER = range(7)
 
 _dp = C.POINTER(C.c_double)
@@ -32,7 +32,7 @@ EXPORTS = (
     "mpmw_get_grid_info", "mpmw_set_grid_snapshot", "mpmw_get_grid_snapshot", "mpmw_set_kernel_timing",
     "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes", "mpmw_get_diagnostics",
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
-    "mpmw_batch_step", "mpmw_batch_synchronize",
+    "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
 )
 
 
@@ -126,6 +126,9 @@ def lib():
         "mpmw_set_tool_samples": (C.c_int, [vp, _dp, C.c_int32, C.c_double]),
         "mpmw_set_tool_analytic": (C.c_int, [vp, C.POINTER(ToolDesc)]),
         "mpmw_set_tool_pose": (C.c_int, [vp, C.POINTER(ToolPose)]),
+        "mpmw_set_tool_sdf_mesh": (C.c_int, [vp, C.POINTER(ToolDesc), _dp, C.c_int32, _ip, _dp, C.c_int32,
+                                             C.c_double, C.c_int32]),
+        "mpmw_get_tool_sdf": (C.c_int, [vp, C.POINTER(C.c_float), _ip, _dp, _dp]),
         "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
         "mpmw_step": (C.c_int, [vp, C.c_double]),
@@ -282,6 +285,30 @@ class Solver:
 
     def set_tool_analytic(self, desc):
         _check(lib().mpmw_set_tool_analytic(self.h, C.byref(desc)), "set_tool_analytic")
+
+    def set_tool_sdf_mesh(self, desc, verts, tris, normals, padding, voxels_per_axis):
+        """Voxel SDF tool from a triangle mesh (rest frame) with the contact
+        law of `desc` (band, friction, grad_eps)."""
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        if len(n) != len(t):
+            raise MPMError("set_tool_sdf_mesh: one normal per triangle")
+        _check(lib().mpmw_set_tool_sdf_mesh(self.h, C.byref(desc), _p(v), len(v),
+                                            t.ctypes.data_as(_ip), _p(n), len(t), padding, voxels_per_axis),
+               "set_tool_sdf_mesh")
+
+    def get_tool_sdf(self):
+        """(grid[nz, ny, nx] float32, origin, voxel) of the voxel SDF in use."""
+        dims = (C.c_int32 * 3)()
+        org = (C.c_double * 3)()
+        vox = C.c_double()
+        _check(lib().mpmw_get_tool_sdf(self.h, None, dims, org, C.byref(vox)), "get_tool_sdf")
+        nx, ny, nz = dims
+        g = np.empty((nz, ny, nx), dtype=np.float32)
+        _check(lib().mpmw_get_tool_sdf(self.h, g.ctypes.data_as(C.POINTER(C.c_float)), dims, org, C.byref(vox)),
+               "get_tool_sdf")
+        return g, tuple(org), vox.value
 
     def set_tool_pose(self, pose):
         _check(lib().mpmw_set_tool_pose(self.h, C.byref(pose)), "set_tool_pose")

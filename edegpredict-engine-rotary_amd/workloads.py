@@ -119,6 +119,60 @@ def wedge_desc_values(rake_deg=5.0, clearance_deg=7.0, friction=0.3):
                 flute_offset=0.0, flute_radius=0.0, flute_length=0.0)
 
 
+def prism_tool_mesh(width, length=1.5e-3, rake_deg=5.0, clearance_deg=7.0):
+    """Triangle mesh of a straight cutting insert in its rest frame: the
+    cutting edge on the y axis (|y| <= width/2), the rake face leaning back by
+    the rake angle from the edge up to `length`, the flank face rising by the
+    clearance angle behind the edge, a back face closing the section (the
+    solid the analytic wedge describes, cut to a finite prism).  Returns
+    (verts, tris, normals) with outward unit normals."""
+    g, a = math.radians(rake_deg), math.radians(clearance_deg)
+    edge = np.array([0.0, 0.0])
+    rake = length * np.array([-math.sin(g), math.cos(g)])
+    flank = length * np.array([-math.cos(a), math.sin(a)])
+    sec = [edge, rake, flank]
+    y0, y1 = -0.5 * width, 0.5 * width
+    V = np.array([[p[0], y, p[1]] for y in (y0, y1) for p in sec])  # 0-2 at y0, 3-5 at y1
+    quads = [(0, 3, 4, 1), (0, 2, 5, 3), (1, 4, 5, 2)]
+    T = []
+    for i, j, k, l in quads:
+        T += [(i, j, k), (i, k, l)]
+    T += [(0, 1, 2), (3, 5, 4)]
+    T = np.array(T, dtype=np.int32)
+    centroid = V.mean(axis=0)
+    N = []
+    for t in T:
+        n = np.cross(V[t[1]] - V[t[0]], V[t[2]] - V[t[0]])
+        n /= np.linalg.norm(n)
+        if np.dot(n, V[t].mean(axis=0) - centroid) < 0:
+            n = -n
+        N.append(n)
+    return V, T, np.array(N)
+
+
+def voxel_desc_values(friction=0.3):
+    return dict(kind=4, band=0.0, friction=friction, grad_eps=0.05 * H, rake_angle=0.0, clearance_angle=0.0,
+                radius=0.0, helix_angle=0.0, point_angle=0.0, flute_offset=0.0, flute_radius=0.0,
+                flute_length=0.0)
+
+
+def voxel_tool_case(cfg):
+    """A wedge workpiece of the batched distribution cut by the insert mesh
+    instead of the analytic wedge: voxel SDF built with the reference's
+    padding of 2 mm and 128 voxels along the longest axis (ContactSolver.cu:1019),
+    the insert spanning the workpiece width plus a cell each side."""
+    c = dict(cfg)
+    cx, cy, cz = cfg["cells"]
+    pose = dict(cfg["tool"][2])
+    ex, _, ez = pose["center"]
+    pose["center"] = (ex, 0.5 * cy * H, ez)
+    mesh = prism_tool_mesh(cy * H + 2 * H)
+    c["tool"] = ("voxel", voxel_desc_values(), pose, mesh, 0.002, 128)
+    c["tip"] = pose["center"]
+    c["name"] = cfg["name"].replace("batch", "voxel")
+    return c
+
+
 def sample_workpiece(rng, index, seed, steps):
     """One workpiece of the batched / served distributions: N log-uniform in
     [2e4, 2e6] particles (cells = N/8 in a 2:2:1 box), J-C A, B, n, C, m each
@@ -164,6 +218,19 @@ def apply_to_solver(mpmw, solver, cfg):
         for k, v in cfg["tool"][1].items():
             setattr(d, k, v)
         solver.set_tool_analytic(d)
+        p = mpmw.ToolPose()
+        pd = cfg["tool"][2]
+        p.center[:] = pd["center"]
+        p.angle = pd["angle"]
+        p.vel[:] = pd["vel"]
+        p.omega = pd["omega"]
+        solver.set_tool_pose(p)
+    elif kind == "voxel":
+        d = mpmw.ToolDesc()
+        for k, v in cfg["tool"][1].items():
+            setattr(d, k, v)
+        verts, tris, normals = cfg["tool"][3]
+        solver.set_tool_sdf_mesh(d, verts, tris, normals, cfg["tool"][4], cfg["tool"][5])
         p = mpmw.ToolPose()
         pd = cfg["tool"][2]
         p.center[:] = pd["center"]

@@ -49,7 +49,8 @@ enum mpmw_tool_kind {
     MPMW_TOOL_NONE = 0,
     MPMW_TOOL_POINTS = 1,  /* reference point samples (setToolMesh) */
     MPMW_TOOL_WEDGE = 2,   /* analytic rake/flank wedge */
-    MPMW_TOOL_DRILL = 3    /* analytic two-flute helical drill */
+    MPMW_TOOL_DRILL = 3,   /* analytic two-flute helical drill */
+    MPMW_TOOL_VOXEL = 4    /* voxel SDF of a tool mesh (ToolSDF) */
 };
 
 /* Solver parameters.  Material fields are MPMMaterialParams
@@ -176,6 +177,25 @@ int mpmw_init_jittered_slab(mpmw_solver *s, uint64_t seed, int32_t cx, int32_t c
 int mpmw_set_tool_samples(mpmw_solver *s, const double *xyz, int32_t n, double contact_radius);
 int mpmw_set_tool_analytic(mpmw_solver *s, const mpmw_tool_desc *d);
 int mpmw_set_tool_pose(mpmw_solver *s, const mpmw_tool_pose *p);
+/* ToolSDF::build (ToolSDF.cu:212-280) with computeSDFCPU (:379-532): a
+ * voxel SDF of the triangle mesh (verts: nverts xyz in the tool's rest
+ * frame, tris: ntris vertex-index triples, normals: ntris outward unit
+ * normals), bounding box grown by `padding`, voxel = longest extent /
+ * voxels_per_axis, at most 256 voxels per axis.  Built on the device (the
+ * flood fill of voxels beyond the scatter reach on the host); the values
+ * equal the reference's single-precision host build.  The tool becomes
+ * MPMW_TOOL_VOXEL with the contact law of d (band, friction, grad_eps; its
+ * kind is ignored) and is sampled through the pose (setToolTransform,
+ * :569-576) with the clamp-addressed trilinear filter of the texture
+ * (:534-567): no contact outside the texel range [0, n - 1]. */
+int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double *verts, int32_t nverts,
+                           const int32_t *tris, const double *normals, int32_t ntris, double padding,
+                           int32_t voxels_per_axis);
+/* The voxel SDF in use: dims, origin and voxel size always (when non-NULL);
+ * the dims[0]*dims[1]*dims[2] values (x fastest) when grid is non-NULL.
+ * MPMW_E_STATE when no SDF was built.  No counterpart in the reference
+ * (its grid lives only in the texture). */
+int mpmw_get_tool_sdf(mpmw_solver *s, float *grid, int32_t dims[3], double origin[3], double *voxel);
 /* setToolTipPosition, MPMWorkpieceSolver.cuh:133-135 */
 int mpmw_set_tool_tip(mpmw_solver *s, double x, double y, double z);
 /* applyExternalForce, MPMWorkpieceSolver.cu:812-818 */

@@ -23,6 +23,7 @@
  */
 #include "mpm_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -43,6 +44,7 @@ struct orc_solver {
     int ntool;
     orc_tool_desc tool;
     orc_tool_pose pose;
+    orc_sdf sdf;          /* voxel tool (kind ORC_TOOL_VOXEL) */
     double tip[3];
     double time;
     int64_t step;
@@ -172,6 +174,277 @@ static double phi_drill(const orc_tool_desc *d, const double q[3]) {
     return fmax(body, -gmin);
 }
 
+/* ---------------------------------------------------------- voxel SDF --- */
+
+/* pointTriDistanceSqHost, ToolSDF.cu:92-162: the closest-point-on-triangle
+ * regions of Ericson, Real-Time Collision Detection 5.1.5, in the
+ * reference's float arithmetic, including its interior case, which forms the
+ * point as a + u (b - a) + v (c - a) with u = 1 - v - w (:148-155). */
+float orc_point_tri_d2(const float p[3], const float a[3], const float b[3], const float c[3], float q[3]) {
+    float ab[3], ac[3], ap[3], bp[3], cp[3];
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = b[k] - a[k];
+        ac[k] = c[k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    float d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+    float d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        for (int k = 0; k < 3; ++k) q[k] = a[k];
+        goto out;
+    }
+    for (int k = 0; k < 3; ++k) bp[k] = p[k] - b[k];
+    float d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+    float d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+    if (d3 >= 0.0f && d4 <= d3) {
+        for (int k = 0; k < 3; ++k) q[k] = b[k];
+        goto out;
+    }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float v = d1 / (d1 - d3);
+        for (int k = 0; k < 3; ++k) q[k] = a[k] + v * ab[k];
+        goto out;
+    }
+    for (int k = 0; k < 3; ++k) cp[k] = p[k] - c[k];
+    float d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+    float d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+    if (d6 >= 0.0f && d5 <= d6) {
+        for (int k = 0; k < 3; ++k) q[k] = c[k];
+        goto out;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float w = d2 / (d2 - d6);
+        for (int k = 0; k < 3; ++k) q[k] = a[k] + w * ac[k];
+        goto out;
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int k = 0; k < 3; ++k) q[k] = b[k] + w * (c[k] - b[k]);
+        goto out;
+    }
+    {
+        float den = 1.0f / (va + vb + vc);
+        float v = vb * den, w = vc * den, u = 1.0f - v - w;
+        for (int k = 0; k < 3; ++k) q[k] = a[k] + u * (b[k] - a[k]) + v * (c[k] - a[k]);
+    }
+out:;
+    float dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+typedef struct {
+    float a[3], b[3], c[3], n[3], lo[3], hi[3];
+} sdf_tri;
+
+int orc_sdf_build(const double *verts, int nverts, const int32_t *tris, const double *normals, int ntris,
+                  double padding, int voxels_per_axis, orc_sdf *out) {
+    memset(out, 0, sizeof(*out));
+    if (nverts <= 0 || ntris <= 0 || voxels_per_axis <= 0) return -1;
+    /* build, :233-266 */
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], (float)verts[3 * i + k]);
+            hi[k] = fmaxf(hi[k], (float)verts[3 * i + k]);
+        }
+    const float pad = (float)padding;
+    for (int k = 0; k < 3; ++k) { lo[k] -= pad; hi[k] += pad; }
+    const float ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    const float vox = fmaxf(ext[0], fmaxf(ext[1], ext[2])) / (float)voxels_per_axis;
+    int dims[3];
+    for (int k = 0; k < 3; ++k) {
+        dims[k] = (int)(ext[k] / vox) + 2;
+        if (dims[k] > 256) dims[k] = 256;
+    }
+    const int nx = dims[0], ny = dims[1], nz = dims[2];
+    const long total = (long)nx * ny * nz;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || total <= 0) return -1;
+    out->nx = nx; out->ny = ny; out->nz = nz;
+    out->ox = lo[0]; out->oy = lo[1]; out->oz = lo[2];
+    out->voxel = vox;
+    /* buildFromVerts, :348-378 */
+    sdf_tri *T = (sdf_tri *)malloc(sizeof(sdf_tri) * (size_t)ntris);
+    for (int t = 0; t < ntris; ++t) {
+        for (int k = 0; k < 3; ++k) {
+            T[t].a[k] = (float)verts[3 * tris[3 * t + 0] + k];
+            T[t].b[k] = (float)verts[3 * tris[3 * t + 1] + k];
+            T[t].c[k] = (float)verts[3 * tris[3 * t + 2] + k];
+            T[t].n[k] = (float)normals[3 * t + k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            T[t].lo[k] = fminf(T[t].a[k], fminf(T[t].b[k], T[t].c[k]));
+            T[t].hi[k] = fmaxf(T[t].a[k], fmaxf(T[t].b[k], T[t].c[k]));
+        }
+    }
+    float *G = (float *)malloc(sizeof(float) * (size_t)total);
+    for (long i = 0; i < total; ++i) G[i] = FLT_MAX;
+    const float org[3] = {out->ox, out->oy, out->oz};
+    const float inv = 1.0f / vox;
+#define SDF_CENTER(ix, iy, iz, c) do { (c)[0] = org[0] + ((float)(ix) + 0.5f) * vox; \
+                                       (c)[1] = org[1] + ((float)(iy) + 0.5f) * vox; \
+                                       (c)[2] = org[2] + ((float)(iz) + 0.5f) * vox; } while (0)
+    /* squared distance to each triangle over its voxel box grown by one, :394-420 */
+    for (int t = 0; t < ntris; ++t) {
+        int i0[3], i1[3];
+        for (int k = 0; k < 3; ++k) {
+            i0[k] = (int)((T[t].lo[k] - org[k]) * inv) - 1;
+            if (i0[k] < 0) i0[k] = 0;
+            i1[k] = (int)((T[t].hi[k] - org[k]) * inv) + 2;
+            if (i1[k] > dims[k] - 1) i1[k] = dims[k] - 1;
+        }
+        for (int iz = i0[2]; iz <= i1[2]; ++iz)
+            for (int iy = i0[1]; iy <= i1[1]; ++iy)
+                for (int ix = i0[0]; ix <= i1[0]; ++ix) {
+                    float c[3], q[3];
+                    SDF_CENTER(ix, iy, iz, c);
+                    const float d2 = orc_point_tri_d2(c, T[t].a, T[t].b, T[t].c, q);
+                    const long idx = ix + (long)nx * (iy + (long)ny * iz);
+                    if (d2 < G[idx]) G[idx] = d2;
+                }
+    }
+    /* distances, :424-428 */
+    for (long i = 0; i < total; ++i)
+        if (G[i] < FLT_MAX) G[i] = sqrtf(G[i]);
+    /* sign within two voxels of the surface from the nearest triangle's
+     * normal (first strictly nearest among triangles whose box grown by one
+     * voxel holds the centre), :442-483 */
+    const float band = vox * 2.0f;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+                const long idx = ix + (long)nx * (iy + (long)ny * iz);
+                const float dist = G[idx];
+                if (dist > band) continue;
+                float c[3];
+                SDF_CENTER(ix, iy, iz, c);
+                float best = FLT_MAX, bn[3] = {0, 0, 0}, bq[3] = {0, 0, 0};
+                for (int t = 0; t < ntris; ++t) {
+                    if (c[0] < T[t].lo[0] - vox || c[0] > T[t].hi[0] + vox || c[1] < T[t].lo[1] - vox ||
+                        c[1] > T[t].hi[1] + vox || c[2] < T[t].lo[2] - vox || c[2] > T[t].hi[2] + vox)
+                        continue;
+                    float q[3];
+                    const float d2 = orc_point_tri_d2(c, T[t].a, T[t].b, T[t].c, q);
+                    if (d2 < best) {
+                        best = d2;
+                        for (int k = 0; k < 3; ++k) { bn[k] = T[t].n[k]; bq[k] = q[k]; }
+                    }
+                }
+                if (best < FLT_MAX) {
+                    const float dot = (c[0] - bq[0]) * bn[0] + (c[1] - bq[1]) * bn[1] + (c[2] - bq[2]) * bn[2];
+                    if (dot < 0.0f) G[idx] = -dist;
+                }
+            }
+    /* unknown voxels take the smallest |value| of their 26 neighbours plus
+     * half a voxel, three in-place passes over the interior, :489-520 */
+    for (int pass = 0; pass < 3; ++pass)
+        for (int iz = 1; iz < nz - 1; ++iz)
+            for (int iy = 1; iy < ny - 1; ++iy)
+                for (int ix = 1; ix < nx - 1; ++ix) {
+                    const long idx = ix + (long)nx * (iy + (long)ny * iz);
+                    if (G[idx] < FLT_MAX) continue;
+                    float near = FLT_MAX;
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                if (dx == 0 && dy == 0 && dz == 0) continue;
+                                const long ni = (ix + dx) + (long)nx * ((iy + dy) + (long)ny * (iz + dz));
+                                const float v = fabsf(G[ni]);
+                                if (v < near) near = v;
+                            }
+                    if (near < FLT_MAX) G[idx] = near + vox * 0.5f;
+                }
+    /* the rest, :523-527 */
+    const int mx = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+    for (long i = 0; i < total; ++i)
+        if (G[i] >= FLT_MAX) G[i] = vox * (float)mx;
+#undef SDF_CENTER
+    free(T);
+    out->grid = G;
+    return 0;
+}
+
+void orc_sdf_free(orc_sdf *sdf) {
+    if (!sdf) return;
+    free(sdf->grid);
+    memset(sdf, 0, sizeof(*sdf));
+}
+
+int orc_sdf_in_range(const orc_sdf *sdf, const double q[3]) {
+    const double org[3] = {sdf->ox, sdf->oy, sdf->oz};
+    const int dims[3] = {sdf->nx, sdf->ny, sdf->nz};
+    for (int k = 0; k < 3; ++k) {
+        const double t = (q[k] - org[k]) / (double)sdf->voxel;
+        if (!(t >= 0.0) || t > (double)(dims[k] - 1)) return 0;
+    }
+    return 1;
+}
+
+double orc_sdf_sample(const orc_sdf *sdf, const double q[3]) {
+    const double org[3] = {sdf->ox, sdf->oy, sdf->oz};
+    const int dims[3] = {sdf->nx, sdf->ny, sdf->nz};
+    int i0[3], i1[3];
+    double f[3];
+    for (int k = 0; k < 3; ++k) {
+        const double u = (q[k] - org[k]) / (double)sdf->voxel - 0.5;
+        const double fl = floor(u);
+        f[k] = u - fl;
+        int a = fl < -1.0 ? -1 : (fl > (double)dims[k] ? dims[k] : (int)fl);
+        i0[k] = a < 0 ? 0 : (a > dims[k] - 1 ? dims[k] - 1 : a);
+        i1[k] = a + 1 < 0 ? 0 : (a + 1 > dims[k] - 1 ? dims[k] - 1 : a + 1);
+    }
+    double v = 0.0;
+    for (int n = 0; n < 8; ++n) {
+        const int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
+        const double w = (dx ? f[0] : 1.0 - f[0]) * (dy ? f[1] : 1.0 - f[1]) * (dz ? f[2] : 1.0 - f[2]);
+        const long idx = (dx ? i1[0] : i0[0]) + (long)sdf->nx * ((dy ? i1[1] : i0[1]) + (long)sdf->ny * (dz ? i1[2] : i0[2]));
+        v += w * (double)sdf->grid[idx];
+    }
+    return v;
+}
+
+void orc_set_tool_sdf(orc_solver *s, const orc_sdf *sdf, const orc_tool_desc *d) {
+    orc_sdf_free(&s->sdf);
+    s->sdf = *sdf;
+    const size_t n = (size_t)sdf->nx * sdf->ny * sdf->nz;
+    s->sdf.grid = (float *)malloc(sizeof(float) * n);
+    memcpy(s->sdf.grid, sdf->grid, sizeof(float) * n);
+    s->tool = *d;
+    s->tool.kind = ORC_TOOL_VOXEL;
+}
+
+/* phi of the voxel tool at world point (x, y, z): the sampled value inside
+ * the texture range, no contact (a large value) outside it. */
+static double phi_voxel_world(const orc_solver *s, double x, double y, double z) {
+    double q[3];
+    to_tool_frame(&s->pose, x, y, z, q);
+    if (!orc_sdf_in_range(&s->sdf, q)) return 1e300;
+    return orc_sdf_sample(&s->sdf, q);
+}
+
+/* Normal of the voxel tool: central differences of the clamped sampler
+ * along the world axes (the reference differences the texture along its
+ * rest-frame axes, ContactSolver.cu:848-859; along the world axes the
+ * gradient is the same vector rotated into the world frame). */
+static void normal_voxel_world(const orc_solver *s, double x, double y, double z, double n[3]) {
+    const double e = s->tool.grad_eps;
+    double qp[3], qm[3], g[3];
+    const double dd[3][3] = {{e, 0, 0}, {0, e, 0}, {0, 0, e}};
+    for (int k = 0; k < 3; ++k) {
+        to_tool_frame(&s->pose, x + dd[k][0], y + dd[k][1], z + dd[k][2], qp);
+        to_tool_frame(&s->pose, x - dd[k][0], y - dd[k][1], z - dd[k][2], qm);
+        g[k] = (orc_sdf_sample(&s->sdf, qp) - orc_sdf_sample(&s->sdf, qm)) / (2.0 * e);
+    }
+    const double len = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (len > 1e-12) {
+        n[0] = g[0] / len; n[1] = g[1] / len; n[2] = g[2] / len;
+    } else {
+        n[0] = n[1] = n[2] = 0.0;
+    }
+}
+
 static double phi_local(const orc_tool_desc *d, const double q[3]) {
     if (d->kind == ORC_TOOL_WEDGE) return phi_wedge(d, q);
     if (d->kind == ORC_TOOL_DRILL) return phi_drill(d, q);
@@ -245,6 +518,7 @@ orc_solver *orc_create(const orc_params *p) {
 void orc_destroy(orc_solver *s) {
     if (!s) return;
     free(s->p); free(s->grid); free(s->snap); free(s->tm_new); free(s->tool_xyz);
+    orc_sdf_free(&s->sdf);
     free(s);
 }
 
@@ -555,11 +829,14 @@ static void grid_update(orc_solver *s, double fixture, double dt) {
                     node->tm += dT * node->mass;
                 }
             }
-        } else if (s->tool.kind == ORC_TOOL_WEDGE || s->tool.kind == ORC_TOOL_DRILL) {
-            double phi = orc_tool_phi(&s->tool, &s->pose, px, py, pz);
+        } else if (s->tool.kind == ORC_TOOL_WEDGE || s->tool.kind == ORC_TOOL_DRILL ||
+                   s->tool.kind == ORC_TOOL_VOXEL) {
+            const int vox = s->tool.kind == ORC_TOOL_VOXEL;
+            double phi = vox ? phi_voxel_world(s, px, py, pz) : orc_tool_phi(&s->tool, &s->pose, px, py, pz);
             if (phi < s->tool.band) {
                 double n[3];
-                orc_tool_normal(&s->tool, &s->pose, px, py, pz, n);
+                if (vox) normal_voxel_world(s, px, py, pz, n);
+                else orc_tool_normal(&s->tool, &s->pose, px, py, pz, n);
                 if (n[0] != 0.0 || n[1] != 0.0 || n[2] != 0.0) {
                     const orc_tool_pose *P = &s->pose;
                     double tvx = P->vel[0] - P->omega * (py - P->center[1]);

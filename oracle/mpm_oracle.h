@@ -23,7 +23,7 @@ extern "C" {
 enum { ORC_ACTIVE = 0, ORC_INACTIVE = 1, ORC_CHIP = 2, ORC_BOUNDARY = 3, ORC_FIXED_BOUNDARY = 4 };
 
 /* Tool kinds (none / reference point samples / analytic wedge / analytic drill) */
-enum { ORC_TOOL_NONE = 0, ORC_TOOL_POINTS = 1, ORC_TOOL_WEDGE = 2, ORC_TOOL_DRILL = 3 };
+enum { ORC_TOOL_NONE = 0, ORC_TOOL_POINTS = 1, ORC_TOOL_WEDGE = 2, ORC_TOOL_DRILL = 3, ORC_TOOL_VOXEL = 4 };
 
 /* MPMParticleGPU, include/MPMWorkpieceSolver.cuh:18-45 (field order kept) */
 typedef struct {
@@ -160,6 +160,35 @@ void orc_tool_normal(const orc_tool_desc *d, const orc_tool_pose *p, double x, d
 /* One G2P constitutive update of a single particle from a prescribed
  * velocity gradient (row = velocity component, column = d/dx_b). */
 void orc_constitutive(const orc_params *p, orc_particle *pt, const double L[9], double dt);
+/* Voxel signed distance field of a tool mesh at rest pose: ToolSDF::build
+ * (ToolSDF.cu:212-266, buildFromVerts :348-378, computeSDFCPU :380-532),
+ * restated in single precision as the reference computes it.  Voxel
+ * (i, j, k) holds the value at origin + (i + 0.5, j + 0.5, k + 0.5) * voxel,
+ * index i + nx * (j + ny * k). */
+typedef struct {
+    int32_t nx, ny, nz, pad_;
+    float ox, oy, oz, voxel;
+    float *grid;
+} orc_sdf;
+
+/* pointTriDistanceSqHost, ToolSDF.cu:92-162: squared distance from p to the
+ * point the reference takes as closest on triangle abc (written to closest). */
+float orc_point_tri_d2(const float p[3], const float a[3], const float b[3], const float c[3], float closest[3]);
+/* Build from nverts xyz vertices and ntris triangles (3 vertex indices and a
+ * normal each); returns 0 on success (grid allocated), -1 on bad input. */
+int orc_sdf_build(const double *verts, int nverts, const int32_t *tris, const double *normals, int ntris,
+                  double padding, int voxels_per_axis, orc_sdf *out);
+void orc_sdf_free(orc_sdf *sdf);
+/* Trilinear value at rest-frame point q over voxel centres, edge voxels
+ * repeated outside the grid (the texture's clamp addressing, :544-553). */
+double orc_sdf_sample(const orc_sdf *sdf, const double q[3]);
+/* The sampler's range test (ContactSolver.cu:837-840): 1 when the texel
+ * coordinate (q - origin) / voxel lies in [0, n - 1] on every axis. */
+int orc_sdf_in_range(const orc_sdf *sdf, const double q[3]);
+/* Use the SDF (copied) as the tool's phi, with the contact law of d
+ * (band, friction, grad_eps); the pose comes from orc_set_tool_pose. */
+void orc_set_tool_sdf(orc_solver *s, const orc_sdf *sdf, const orc_tool_desc *d);
+
 /* Jittered slab generator (config inputs), float-exact positions. */
 void orc_gen_jittered_slab(uint64_t seed, int cx, int cy, int cz, double h, const double origin[3],
                            double density, double fixture, double *pos, double *mass,

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "liborc_mpm.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
-TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL = 0, 1, 2, 3
+TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -69,6 +69,12 @@ class Particle(C.Structure):
 
 
 _lib = None
+
+
+class Sdf(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad_", C.c_int32),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("voxel", C.c_float),
+                ("grid", C.POINTER(C.c_float))]
 
 
 def lib():
@@ -127,6 +133,15 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_gen_jittered_slab.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _dp,
                                             C.c_double, C.c_double, _dp, _dp, _dp, _ip, _ip]
+        _fp = C.POINTER(C.c_float)
+        L.orc_point_tri_d2.argtypes = [_fp, _fp, _fp, _fp, _fp]
+        L.orc_point_tri_d2.restype = C.c_float
+        L.orc_sdf_build.argtypes = [_dp, C.c_int, _ip, _dp, C.c_int, C.c_double, C.c_int, C.POINTER(Sdf)]
+        L.orc_sdf_free.argtypes = [C.POINTER(Sdf)]
+        L.orc_sdf_sample.argtypes = [C.POINTER(Sdf), _dp]
+        L.orc_sdf_sample.restype = C.c_double
+        L.orc_sdf_in_range.argtypes = [C.POINTER(Sdf), _dp]
+        L.orc_set_tool_sdf.argtypes = [C.c_void_p, C.POINTER(Sdf), C.POINTER(ToolDesc)]
         _lib = L
     return _lib
 
@@ -218,6 +233,10 @@ class Oracle:
 
     def set_tool_analytic(self, desc):
         lib().orc_set_tool_analytic(self.h, C.byref(desc))
+
+    def set_tool_sdf(self, sdf, desc):
+        """Voxel tool: `sdf` from build_sdf (its grid is copied)."""
+        lib().orc_set_tool_sdf(self.h, C.byref(sdf.raw), C.byref(desc))
 
     def set_tool_pose(self, pose):
         lib().orc_set_tool_pose(self.h, C.byref(pose))
@@ -331,3 +350,59 @@ def constitutive(params, particle, L, dt):
 
 def set_threads(n):
     lib().orc_set_threads(n)
+
+
+def point_tri_d2(p, a, b, c):
+    """The reference's squared point-triangle distance and closest point (float)."""
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    p, a, b, c = f(p), f(a), f(b), f(c)
+    q = np.zeros(3, dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    d2 = lib().orc_point_tri_d2(p.ctypes.data_as(fp), a.ctypes.data_as(fp), b.ctypes.data_as(fp),
+                                c.ctypes.data_as(fp), q.ctypes.data_as(fp))
+    return float(d2), q
+
+
+class SdfGrid:
+    """A voxel SDF built by the oracle (owns its grid)."""
+
+    def __init__(self, verts, tris, normals, padding=0.002, voxels_per_axis=128):
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.int32)
+        n = np.ascontiguousarray(normals, dtype=np.float64)
+        self.raw = Sdf()
+        rc = lib().orc_sdf_build(_p(v), len(v), _p(t, _ip), _p(n), len(t), padding, voxels_per_axis,
+                                 C.byref(self.raw))
+        if rc != 0:
+            raise ValueError("orc_sdf_build: bad mesh")
+
+    @property
+    def dims(self):
+        return (self.raw.nx, self.raw.ny, self.raw.nz)
+
+    @property
+    def origin(self):
+        return (self.raw.ox, self.raw.oy, self.raw.oz)
+
+    @property
+    def voxel(self):
+        return self.raw.voxel
+
+    def grid(self):
+        """Values as an (nz, ny, nx) float32 array."""
+        nx, ny, nz = self.dims
+        return np.ctypeslib.as_array(self.raw.grid, shape=(nz * ny * nx,)).reshape(nz, ny, nx).copy()
+
+    def sample(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        return lib().orc_sdf_sample(C.byref(self.raw), _p(q))
+
+    def in_range(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        return bool(lib().orc_sdf_in_range(C.byref(self.raw), _p(q)))
+
+    def __del__(self):
+        try:
+            lib().orc_sdf_free(C.byref(self.raw))
+        except Exception:
+            pass

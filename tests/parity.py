@@ -52,11 +52,15 @@ def build_oracle(cfg):
     kind = cfg["tool"][0]
     if kind == "points":
         o.set_tool_samples(cfg["tool"][1], cfg["tool"][2])
-    elif kind in ("drill", "wedge"):
+    elif kind in ("drill", "wedge", "voxel"):
         desc = oracle.ToolDesc()
         for k, v in cfg["tool"][1].items():
             setattr(desc, k, v)
-        o.set_tool_analytic(desc)
+        if kind == "voxel":
+            verts, tris, normals = cfg["tool"][3]
+            o.set_tool_sdf(oracle.SdfGrid(verts, tris, normals, cfg["tool"][4], cfg["tool"][5]), desc)
+        else:
+            o.set_tool_analytic(desc)
         pose = oracle.ToolPose()
         pd = cfg["tool"][2]
         pose.center[:] = pd["center"]
