@@ -1,0 +1,443 @@
+// mpmw_apic.hpp -- the APIC transfer variant of the particle pass.
+//
+// Quadratic B-spline weights over 27 nodes with base = floor(x/h - 0.5)
+// (getWeightsAndGradients, MPMSolver.cu:20-34), affine momentum
+// m w (v + C (x_i - x_p)) in P2G (p2gKernel :370-421) and the affine matrix
+// C = 0.99 * 3/h^2 * sum w v_i (x_i - x_p)^T from G2P (g2pKernel :554-592);
+// the rest of the step (forces, grid update, contact, diffusion,
+// constitutive update, chip pass) is the workpiece solver's.
+//
+// Same tiles and binning as the trilinear pass.  A particle of cell c has
+// its base at c - 1 or c, so the node window of a tile of TX x TY x TZ cells
+// spans one node before the tile and two after it: QX x QY x QZ nodes.  The
+// 27 x 8 contributions of a particle go into the LDS window with LDS
+// atomics (a register accumulation per cell does not fit 27 nodes); the
+// window is then stored, with plain stores, into the tile's slot of the
+// per-tile window buffer, and the grid update sums the windows holding each
+// node in tile order.  Particles whose stencil leaves their tile's window
+// go to the node accumulators with device atomics.
+#pragma once
+
+#include "mpmw_tile.hpp"
+
+namespace mpmw {
+
+__device__ __forceinline__ int qnode(int x, int y, int z) { return (z * QY + y) * QX + x; }
+
+// The per-axis quadratic weights of one particle.  d0 = r - base in
+// [0.5, 1.5); node offsets x_i - x_p = (i - d0) h.
+template <typename R> struct QFrac {
+    int b[3];
+    R w[3][3], dw[3][3], d0[3];
+};
+
+// getWeightsAndGradients (MPMSolver.cu:20-34) on the three axes; r is
+// formed in FP64 as the reference forms x_frac, (x - origin) * invDx.
+template <typename R>
+__device__ __forceinline__ QFrac<R> quad_frac(const GridView<R> &g, double x, double y, double z) {
+    QFrac<R> q;
+    const double d[3] = {x - g.ox, y - g.oy, z - g.oz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double r = d[a] * g.inv_hd;
+        const double fl = floor(r - 0.5);
+        q.b[a] = (int)fl;
+        const R d0 = (R)(r - fl), d1 = d0 - R(1), d2 = d0 - R(2);
+        q.d0[a] = d0;
+        q.w[a][0] = R(0.5) * (R(1.5) - d0) * (R(1.5) - d0);
+        q.w[a][1] = R(0.75) - d1 * d1;
+        q.w[a][2] = R(0.5) * (R(1.5) + d2) * (R(1.5) + d2);
+        q.dw[a][0] = -(R(1.5) - d0) * g.inv_h;
+        q.dw[a][1] = R(-2) * d1 * g.inv_h;
+        q.dw[a][2] = (R(1.5) + d2) * g.inv_h;
+    }
+    return q;
+}
+
+// Node (i, j, k) of the stencil: weight, gradient and offset x_i - x_p.
+template <typename R>
+__device__ __forceinline__ void quad_node(const QFrac<R> &q, int i, int j, int k, R h, R &w, R &gx, R &gy, R &gz,
+                                          R &dx, R &dy, R &dz) {
+    w = q.w[0][i] * q.w[1][j] * q.w[2][k];
+    gx = q.dw[0][i] * q.w[1][j] * q.w[2][k];
+    gy = q.w[0][i] * q.dw[1][j] * q.w[2][k];
+    gz = q.w[0][i] * q.w[1][j] * q.dw[2][k];
+    dx = (R(i) - q.d0[0]) * h;
+    dy = (R(j) - q.d0[1]) * h;
+    dz = (R(k) - q.d0[2]) * h;
+}
+
+// One live node's share of the G2P sums and of B.
+template <typename R>
+__device__ __forceinline__ void quad_gather_add(const vec4_t<R> &nd, R w, R gx, R gy, R gz, R dx, R dy, R dz,
+                                                GatherSums<R> &s, R B[9]) {
+    s.v0 += w * nd.x; s.v1 += w * nd.y; s.v2 += w * nd.z;
+    s.temp += nd.w * w;
+    s.wsum += w;
+    const R wx = w * nd.x, wy = w * nd.y, wz = w * nd.z;
+    B[0] += wx * dx; B[1] += wx * dy; B[2] += wx * dz;
+    B[3] += wy * dx; B[4] += wy * dy; B[5] += wy * dz;
+    B[6] += wz * dx; B[7] += wz * dy; B[8] += wz * dz;
+    s.L[0] += nd.x * gx; s.L[1] += nd.x * gy; s.L[2] += nd.x * gz;
+    s.L[3] += nd.y * gx; s.L[4] += nd.y * gy; s.L[5] += nd.y * gz;
+    s.L[6] += nd.z * gx; s.L[7] += nd.z * gy; s.L[8] += nd.z * gz;
+}
+
+// G2P gather over the 27 nodes from the LDS window (base node at window
+// index wb; window nodes outside the grid hold the empty NaN record) or
+// from device memory (nodes outside the grid skipped).  Empty nodes are
+// skipped (g2pKernel :538, with the workpiece grid's emptiness test).
+template <typename R>
+__device__ __forceinline__ void quad_gather_window(const vec4_t<R> *s_node, int wb, const QFrac<R> &q, R h,
+                                                   GatherSums<R> &s, R B[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const vec4_t<R> nd = s_node[wb + qnode(i, j, k)];
+                if (nd.w != nd.w) continue;
+                R w, gx, gy, gz, dx, dy, dz;
+                quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
+                quad_gather_add(nd, w, gx, gy, gz, dx, dy, dz, s, B);
+            }
+}
+
+template <typename R>
+__device__ __forceinline__ void quad_gather_global(const GridView<R> &g, const vec4_t<R> *node, const QFrac<R> &q,
+                                                   R h, GatherSums<R> &s, R B[9]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int x = q.b[0] + i, y = q.b[1] + j, z = q.b[2] + k;
+                if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                const vec4_t<R> nd = node[((int64_t)z * g.ny + y) * g.nx + x];
+                if (nd.w != nd.w) continue;
+                R w, gx, gy, gz, dx, dy, dz;
+                quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
+                quad_gather_add(nd, w, gx, gy, gz, dx, dy, dz, s, B);
+            }
+}
+
+// What one particle adds in the APIC P2G.
+template <typename R> struct QP2G {
+    R mass, vol, vx, vy, vz, T;
+    R s[6];
+    R C[9];
+    R fex, fey, fez;
+    bool momentum;   // not FIXED_BOUNDARY (p2gKernel :393-398)
+    bool stress;     // ACTIVE or FIXED_BOUNDARY: stress force (mpmP2GKernel :121-127)
+    bool ext;        // stress or CHIP: external force (:121-132)
+};
+
+// The 8 channels of one node: mass, momentum, force, T * m.
+template <typename R>
+__device__ __forceinline__ void quad_channels(const QP2G<R> &a, R w, R gx, R gy, R gz, R dx, R dy, R dz, R c[8]) {
+    const R m = a.mass * w;
+    c[0] = m;
+    const R ax = a.C[0] * dx + a.C[1] * dy + a.C[2] * dz;
+    const R ay = a.C[3] * dx + a.C[4] * dy + a.C[5] * dz;
+    const R az = a.C[6] * dx + a.C[7] * dy + a.C[8] * dz;
+    c[1] = m * (a.vx + ax);
+    c[2] = m * (a.vy + ay);
+    c[3] = m * (a.vz + az);
+    const R fX = -(a.s[0] * gx + a.s[3] * gy + a.s[4] * gz) * a.vol;
+    const R fY = -(a.s[3] * gx + a.s[1] * gy + a.s[5] * gz) * a.vol;
+    const R fZ = -(a.s[4] * gx + a.s[5] * gy + a.s[2] * gz) * a.vol;
+    c[4] = fX + a.fex * w;
+    c[5] = fY + a.fey * w;
+    c[6] = fZ + a.fez * w;
+    c[7] = a.T * m;
+}
+
+// Scatter into the LDS window (window index wb of the base node, channel
+// stride QN) with LDS atomics.
+template <typename R>
+__device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFrac<R> &q, R h, const QP2G<R> &a) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R w, gx, gy, gz, dx, dy, dz, c[8];
+                quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
+                quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
+                R *p = s_acc + wb + qnode(i, j, k);
+                atomicAdd(p, c[0]);
+                if (a.momentum) {
+                    atomicAdd(p + QN, c[1]);
+                    atomicAdd(p + 2 * QN, c[2]);
+                    atomicAdd(p + 3 * QN, c[3]);
+                }
+                if (a.ext) {
+                    atomicAdd(p + 4 * QN, c[4]);
+                    atomicAdd(p + 5 * QN, c[5]);
+                    atomicAdd(p + 6 * QN, c[6]);
+                }
+                atomicAdd(p + 7 * QN, c[7]);
+            }
+}
+
+// Scatter into the node accumulators with device atomics (nodes outside the
+// grid skipped, as p2gKernel :371-378 skips them).
+template <typename R>
+__device__ __forceinline__ void quad_scatter_global(const GridView<R> &g, R *acc, const QFrac<R> &q, R h,
+                                                    const QP2G<R> &a) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int x = q.b[0] + i, y = q.b[1] + j, z = q.b[2] + k;
+                if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
+                R w, gx, gy, gz, dx, dy, dz, c[8];
+                quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
+                quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
+                R *p = acc + ((int64_t)z * g.ny + y) * g.nx + x;
+                unsafeAtomicAdd(p, c[0]);
+                if (a.momentum) {
+                    unsafeAtomicAdd(p + g.nn, c[1]);
+                    unsafeAtomicAdd(p + 2 * g.nn, c[2]);
+                    unsafeAtomicAdd(p + 3 * g.nn, c[3]);
+                }
+                if (a.ext) {
+                    unsafeAtomicAdd(p + 4 * g.nn, c[4]);
+                    unsafeAtomicAdd(p + 5 * g.nn, c[5]);
+                    unsafeAtomicAdd(p + 6 * g.nn, c[6]);
+                }
+                unsafeAtomicAdd(p + 7 * g.nn, c[7]);
+            }
+}
+
+template <typename R> __device__ __forceinline__ void load_affine(const Particles<R> &P, int i, R C[9]) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) C[c] = P.aff[c][i];
+}
+
+// The APIC P2G inputs of a particle from its state p (after its G2P, or as
+// loaded for a P2G-only pass) and its affine matrix.
+template <typename R>
+__device__ __forceinline__ QP2G<R> quad_inputs(const PState<R> &p, const R C[9], R mass, R vol, R fex, R fey,
+                                               R fez) {
+    QP2G<R> a;
+    a.mass = mass; a.vol = vol;
+    a.vx = p.vx; a.vy = p.vy; a.vz = p.vz;
+    a.T = p.T;
+    a.stress = p.status == kActive || p.status == kFixed;
+    a.ext = a.stress || p.status == kChip;
+    a.momentum = p.status != kFixed;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a.s[c] = a.stress ? p.s[c] : R(0);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) a.C[c] = C[c];
+    a.fex = a.ext ? fex : R(0);
+    a.fey = a.ext ? fey : R(0);
+    a.fez = a.ext ? fez : R(0);
+    return a;
+}
+
+// G2P of one particle with the quadratic gather, then its affine matrix:
+// C = B * 3/h^2, C *= 0.99 (g2pKernel :577-591) unless the particle was CHIP
+// (:570-573) or FIXED_BOUNDARY (:516-519) at entry.  C is left in C[] for a
+// following P2G.
+template <typename R, bool HAS_BACK, typename GatherFn>
+__device__ __forceinline__ void apic_g2p(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
+                                         const Material<R> &M, R dt, double dtd, bool chip_step, bool need_stress,
+                                         double zcut, R cfac, R C[9], GatherFn gather_fn) {
+    R B[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) B[c] = R(0);
+    g2p_particle<R, HAS_BACK>(P, i, in, p, M, dt, dtd, chip_step, need_stress, zcut,
+                              [&](GatherSums<R> &s) { gather_fn(s, B); });
+    if (in.status != kChip && in.status != kFixed) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            C[c] = B[c] * cfac;
+            C[c] = C[c] * R(0.99);
+            P.aff[c][i] = C[c];
+        }
+    } else {
+        load_affine(P, i, C);
+    }
+}
+
+// One workgroup per tile: the APIC G2P and / or P2G of the tile's particles,
+// thread tid taking particles b0 + tid, b0 + tid + kBlock, ... (slot-major
+// order: consecutive particles sit in different cells).  The window goes to
+// tpart[t] (8 channels x QN nodes).  counters[0] counts particles whose
+// stencil left the window, counters[2] is set when one was scattered into
+// the node accumulators.
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+__device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const GridView<R> &g, const Grid<R> &G,
+                                                const Material<R> &M, R dt, double dtd, double zcut,
+                                                G2PFlags flags, double ref_temp, double *part,
+                                                const int32_t *cell_start, const TileGeom &tg, int32_t *counters,
+                                                R *tpart, int t) {
+    constexpr int kNodeBytes = DO_G2P ? QN * (int)sizeof(vec4_t<R>) : 0;
+    constexpr int kAccBytes = DO_P2G ? 8 * QN * (int)sizeof(R) : 0;
+    __shared__ __align__(32) unsigned char smem[kNodeBytes + kAccBytes + 32];
+    vec4_t<R> *s_node = reinterpret_cast<vec4_t<R> *>(smem);
+    R *s_acc = DO_P2G ? reinterpret_cast<R *>(smem + kNodeBytes) : nullptr;
+    const bool stats = DO_G2P && !DO_P2G && flags.stats != 0;
+    const int tid = threadIdx.x, lane = tid % kWave;
+    const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
+    // window node 0 is the node before the tile's first cell
+    const int qx0 = tg.ox + tx * TX - 1, qy0 = tg.oy + ty * TY - 1, qz0 = tg.oz + tz * TZ - 1;
+    const int b0 = cell_start[t * kTileCells];
+    const int b1 = cell_start[(t + 1) * kTileCells];
+    const R h = (R)g.hd;
+    const R cfac = (R)(3.0 * g.inv_hd * g.inv_hd);
+    double st[kRed];
+    stats_identity(st, ref_temp);
+    if (b0 < b1) {  // uniform over the workgroup
+        if (DO_G2P) {
+            for (int w = tid; w < QN; w += kBlock) {
+                const int i = qx0 + w % QX, j = qy0 + (w / QX) % QY, k = qz0 + w / (QX * QY);
+                vec4_t<R> nd;
+                if (i >= 0 && j >= 0 && k >= 0 && i < g.nx && j < g.ny && k < g.nz) {
+                    nd = G.node[((int64_t)k * g.ny + j) * g.nx + i];
+                } else {
+                    nd.x = nd.y = nd.z = R(0);
+                    nd.w = R(__builtin_nan(""));
+                }
+                s_node[w] = nd;
+            }
+        }
+        if (DO_P2G)
+            for (int w = tid; w < 8 * QN; w += kBlock) s_acc[w] = R(0);
+        __syncthreads();
+        int my_oow = 0, my_acc = 0;
+        for (int i = b0 + tid; i < b1; i += kBlock) {
+            PState<R> p;
+            p.status = P.status[i];
+            if (p.status == kInactive) continue;
+            const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
+            R fex = R(0), fey = R(0), fez = R(0);
+            R C[9];
+            if (DO_G2P) {
+                G2PIn<R> in;
+                load_g2p_in(P, i, in);
+                const QFrac<R> q = quad_frac(g, in.x, in.y, in.z);
+                const int lx = q.b[0] - qx0, ly = q.b[1] - qy0, lz = q.b[2] - qz0;
+                const bool inw = (unsigned)lx <= (unsigned)TX && (unsigned)ly <= (unsigned)TY &&
+                                 (unsigned)lz <= (unsigned)TZ;
+                const int wb = qnode(lx, ly, lz);
+                my_oow += inw ? 0 : 1;
+                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, stats, zcut, cfac, C,
+                                      [&](GatherSums<R> &s, R B[9]) {
+                                          if (inw) quad_gather_window(s_node, wb, q, h, s, B);
+                                          else quad_gather_global(g, G.node, q, h, s, B);
+                                      });
+                if (stats) {
+                    const double vx = p.vx, vy = p.vy, vz = p.vz;
+                    accumulate_stats(st, p.x, p.y, p.z,
+                                     (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]),
+                                     (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
+                }
+            } else {
+                p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+                load_p2g_state(P, i, p, fex, fey, fez);
+                load_affine(P, i, C);
+            }
+            if (DO_P2G) {
+                // after G2P the external force has been consumed (:399)
+                const QFrac<R> q = quad_frac(g, p.x, p.y, p.z);
+                const QP2G<R> a = quad_inputs(p, C, mass, vol, fex, fey, fez);
+                const int lx = q.b[0] - qx0, ly = q.b[1] - qy0, lz = q.b[2] - qz0;
+                if ((unsigned)lx <= (unsigned)TX && (unsigned)ly <= (unsigned)TY && (unsigned)lz <= (unsigned)TZ) {
+                    quad_scatter_window(s_acc, qnode(lx, ly, lz), q, h, a);
+                } else {
+                    ++my_oow;
+                    my_acc = 1;
+                    quad_scatter_global(g, G.acc, q, h, a);
+                }
+            }
+        }
+        if (DO_P2G) {
+            __syncthreads();  // every LDS atomic is in
+            R *dst = tpart + (int64_t)t * (8 * QN);
+            for (int w = tid; w < 8 * QN; w += kBlock) dst[w] = s_acc[w];
+        }
+        int cntv = my_oow, cnta = my_acc;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            cntv += __shfl_xor(cntv, o, kWave);
+            cnta |= __shfl_xor(cnta, o, kWave);
+        }
+        if (lane == 0 && cntv) atomicAdd(counters, cntv);
+        if (lane == 0 && cnta) atomicOr(counters + 2, 1);
+    }
+    if (stats) block_reduce_stats(st, part, t);
+}
+
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+__global__ void __launch_bounds__(kBlock) k_apic_tile_batch(const TileWork<R> *__restrict__ items, int nitems) {
+    const int w = find_item(items, nitems, (int)blockIdx.x);
+    const TileWork<R> &W = items[w];
+    apic_tile_block<R, DO_G2P, DO_P2G, HAS_BACK>(W.P, W.g, W.G, W.M, W.dt, W.dtd, W.zcut, W.flags, W.ref_temp,
+                                                 W.part, W.cell_start, W.tg, W.counters, W.tpart,
+                                                 (int)blockIdx.x - W.block0);
+}
+
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+__global__ void __launch_bounds__(kBlock)
+    k_apic_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt, double dtd, double zcut,
+                G2PFlags flags, double ref_temp, double *part, const int32_t *cell_start, TileGeom tg,
+                int32_t *counters, R *tpart) {
+    apic_tile_block<R, DO_G2P, DO_P2G, HAS_BACK>(P, g, G, M, dt, dtd, zcut, flags, ref_temp, part, cell_start, tg,
+                                                 counters, tpart, (int)blockIdx.x);
+}
+
+// The loose particles of the APIC pass, one thread each, nodes read from
+// and accumulated into device memory (sets acc_flag).  Statistics partials
+// go to part[blockIdx.x].
+template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
+__global__ void __launch_bounds__(kBlock)
+    k_apic_loose(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt, double dtd, double zcut,
+                 G2PFlags flags, double ref_temp, double *part, int32_t begin, int32_t end, int32_t *acc_flag) {
+    double st[kRed];
+    stats_identity(st, ref_temp);
+    const int i = begin + (int)blockIdx.x * kBlock + (int)threadIdx.x;
+    const R h = (R)g.hd;
+    const R cfac = (R)(3.0 * g.inv_hd * g.inv_hd);
+    if (i < end) {
+        PState<R> p;
+        p.status = P.status[i];
+        if (p.status != kInactive) {
+            const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
+            R fex = R(0), fey = R(0), fez = R(0);
+            R C[9];
+            if (DO_G2P) {
+                G2PIn<R> in;
+                load_g2p_in(P, i, in);
+                const QFrac<R> q = quad_frac(g, in.x, in.y, in.z);
+                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut, cfac, C,
+                                      [&](GatherSums<R> &s, R B[9]) { quad_gather_global(g, G.node, q, h, s, B); });
+                if (flags.stats) {
+                    const double vx = p.vx, vy = p.vy, vz = p.vz;
+                    accumulate_stats(st, p.x, p.y, p.z,
+                                     (double)von_mises<R>(p.s[0], p.s[1], p.s[2], p.s[3], p.s[4], p.s[5]),
+                                     (double)p.T, 0.5 * (double)mass * (vx * vx + vy * vy + vz * vz));
+                }
+            } else {
+                p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+                load_p2g_state(P, i, p, fex, fey, fez);
+                load_affine(P, i, C);
+            }
+            if (DO_P2G) {
+                const QFrac<R> q = quad_frac(g, p.x, p.y, p.z);
+                quad_scatter_global(g, G.acc, q, h, quad_inputs(p, C, mass, vol, fex, fey, fez));
+                atomicOr(acc_flag, 1);
+            }
+        }
+    }
+    if (DO_G2P && flags.stats) block_reduce_stats(st, part, (int)blockIdx.x);
+}
+
+}  // namespace mpmw

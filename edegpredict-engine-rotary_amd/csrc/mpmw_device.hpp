@@ -75,6 +75,7 @@ template <typename R> struct Particles {
     R *a0, *a1, *a2, *a3, *a4, *a5;
     R *fx, *fy, *fz;
     R *eps, *rate, *dmg, *sres, *T, *m, *V;
+    R *aff[9];                 // APIC affine matrix, row-major (transfer variant only)
     R m0, V0;                  // mass and volume of every particle when m == nullptr
     int32_t *status, *id;
     int32_t n;

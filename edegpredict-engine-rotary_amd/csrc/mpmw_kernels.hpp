@@ -133,6 +133,7 @@ template <typename R> struct TileParts {
     const int32_t *tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
     const int32_t *acc_flag;
     TileGeom tg;
+    int32_t quad;                // windows of the quadratic stencil (QX x QY x QZ)
 };
 
 // The tiles whose window holds node coordinate i along one axis (tiles start
@@ -148,14 +149,29 @@ __device__ __forceinline__ int window_tiles(int i, int off, int nt, int T, int t
     return k;
 }
 
+// The same for the quadratic windows, which start one node before the tile
+// and end two after it: node i is local l = i - off + 1 - q * T of tile q
+// for l in [0, T + 2], so of at most two tiles when T >= 3.
+__device__ __forceinline__ int window_tiles_q(int i, int off, int nt, int T, int tile[2], int local[2]) {
+    const int r1 = i - off + 1;
+    if (r1 < 0) return 0;
+    const int q = r1 / T, l = r1 - q * T;
+    int k = 0;
+    if (l <= 2 && q >= 1 && q - 1 < nt) { tile[k] = q - 1; local[k] = l + T; ++k; }
+    if (q < nt) { tile[k] = q; local[k] = l; ++k; }
+    return k;
+}
+
 // This step's 8 P2G channel sums of node (i, j, k), in a fixed order:
 // tiles by increasing z, then y, then x tile index, then the accumulators.
 template <typename R>
 __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, int j, int k, R c[8]) {
     int tx[2], lx[2], ty[2], ly[2], tz[2], lz[2];
-    const int nxs = window_tiles(i, tp.tg.ox, tp.tg.ntx, TX, tx, lx);
-    const int nys = window_tiles(j, tp.tg.oy, tp.tg.nty, TY, ty, ly);
-    const int nzs = window_tiles(k, tp.tg.oz, tp.tg.ntz, TZ, tz, lz);
+    const bool q = tp.quad != 0;
+    const int nxs = q ? window_tiles_q(i, tp.tg.ox, tp.tg.ntx, TX, tx, lx) : window_tiles(i, tp.tg.ox, tp.tg.ntx, TX, tx, lx);
+    const int nys = q ? window_tiles_q(j, tp.tg.oy, tp.tg.nty, TY, ty, ly) : window_tiles(j, tp.tg.oy, tp.tg.nty, TY, ty, ly);
+    const int nzs = q ? window_tiles_q(k, tp.tg.oz, tp.tg.ntz, TZ, tz, lz) : window_tiles(k, tp.tg.oz, tp.tg.ntz, TZ, tz, lz);
+    const int ex = q ? QX : WX, ey = q ? QY : WY, en = q ? QN : WN;
 #pragma unroll
     for (int ch = 0; ch < 8; ++ch) c[ch] = R(0);
     for (int a = 0; a < nzs; ++a)
@@ -163,9 +179,9 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
             for (int d = 0; d < nxs; ++d) {
                 const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
                 if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
-                const R *src = tp.part + (int64_t)t * (8 * WN) + (lz[a] * WY + ly[b]) * WX + lx[d];
+                const R *src = tp.part + (int64_t)t * (8 * en) + (lz[a] * ey + ly[b]) * ex + lx[d];
 #pragma unroll
-                for (int ch = 0; ch < 8; ++ch) c[ch] += src[ch * WN];
+                for (int ch = 0; ch < 8; ++ch) c[ch] += src[ch * en];
             }
 }
 

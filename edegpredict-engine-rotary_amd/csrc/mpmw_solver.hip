@@ -29,7 +29,7 @@
 
 #include "mpmw.h"
 #include "mpmw_sdf.hpp"
-#include "mpmw_tile.hpp"
+#include "mpmw_apic.hpp"
 
 using namespace mpmw;
 
@@ -56,6 +56,7 @@ enum Field {
     F_EPS, F_RATE, F_DMG, F_SRES, F_T, F_M, F_V, F_DENS,
     F_A0, F_A1, F_A2, F_A3, F_A4, F_A5,
     F_FX, F_FY, F_FZ,
+    F_C0, F_C1, F_C2, F_C3, F_C4, F_C5, F_C6, F_C7, F_C8,
     F_STATUS, F_ID,
     NF
 };
@@ -92,6 +93,7 @@ struct mpmw_solver {
     void *alt[NF] = {};    // second buffer set for permutations
     int32_t alt_cap = 0;
     bool has_back = false, has_fext = false;
+    bool apic = false;     // APIC transfer variant (affine fields F_C0..F_C8 live)
 
     // grid
     double minx = 0, miny = 0, minz = 0, h = 0;
@@ -298,6 +300,7 @@ size_t field_bytes(const mpmw_solver *s, int fld, int64_t count) {
 bool field_live(const mpmw_solver *s, int fld) {
     if (fld >= F_A0 && fld <= F_A5) return s->has_back;
     if (fld >= F_FX && fld <= F_FZ) return s->has_fext;
+    if (fld >= F_C0 && fld <= F_C8) return s->apic;
     return true;
 }
 
@@ -383,6 +386,7 @@ template <typename R> Particles<R> pview(mpmw_solver *s) {
     P.fx = s->has_fext ? F(F_FX) : nullptr; P.fy = F(F_FY); P.fz = F(F_FZ);
     P.eps = F(F_EPS); P.rate = F(F_RATE); P.dmg = F(F_DMG); P.sres = F(F_SRES);
     P.T = F(F_T);
+    for (int c = 0; c < 9; ++c) P.aff[c] = s->apic ? F(F_C0 + c) : nullptr;
     // uniform mass and volume are passed as scalars (the arrays stay for export)
     P.m = s->uniform_mv ? nullptr : F(F_M);
     P.V = F(F_V);
@@ -713,7 +717,7 @@ template <typename R> int sort_particles(mpmw_solver *s) {
         hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)3, 0);
     }
     {
-        const int64_t bytes = (int64_t)tg.ntiles * 8 * WN * (int64_t)s->rsz;
+        const int64_t bytes = (int64_t)tg.ntiles * 8 * (s->apic ? QN : WN) * (int64_t)s->rsz;
         if (bytes > s->tpart_cap) {
             const int64_t c = std::max<int64_t>(bytes, s->tpart_cap * 5 / 4);
             dfree(s, s->tpart, (size_t)s->tpart_cap);
@@ -729,7 +733,7 @@ template <typename R> int sort_particles(mpmw_solver *s) {
         hipLaunchKernelGGL(k_fill<int32_t>, dim3(std::min(blocks_for(tg.nkeys + 2), 4096)), dim3(kBlock), 0,
                            s->stream, S.counts, (int64_t)tg.nkeys + 2, 0);
         hipLaunchKernelGGL(k_bin_keys<R>, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream, pview<R>(s),
-                           gview<R>(s), tg, S.keys, S.idx, S.counts);
+                           gview<R>(s), tg, s->apic ? 1 : 0, S.keys, S.idx, S.counts);
         HIP_TRY(hipGetLastError());
         HIP_TRY(radix_sort_pairs(S.temp, S.keys, S.keys_sorted, S.idx, S.idx_sorted, n, nbits, s->stream));
         HIP_TRY(exclusive_scan(S.temp2, S.counts, S.cell_start, tg.nkeys + 2, s->stream));
@@ -810,6 +814,13 @@ int loose_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     const Grid<R> G = grid_arrays<R>(s);
     const Material<R> M = material<R>(s);
     SortState &S = s->sort;
+    if (s->apic) {
+        hipLaunchKernelGGL((k_apic_loose<R, DO_G2P, DO_P2G, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream, P, g,
+                           G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)S.geom.ntiles * kRed,
+                           S.loose_begin, S.loose_end, S.oow + 2);
+        HIP_TRY(hipGetLastError());
+        return MPMW_OK;
+    }
     hipLaunchKernelGGL((k_loose<R, DO_G2P, DO_P2G && !DET, HAS_BACK>), dim3(nbl), dim3(kBlock), 0, s->stream, P, g,
                        G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part + (int64_t)S.geom.ntiles * kRed,
                        S.loose_begin, S.loose_end, S.oow + 2);
@@ -831,9 +842,14 @@ int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     // counters [1] (deterministic: particles off their cell) and [2] (node
     // accumulators written) start at zero for every P2G pass
     if (DO_P2G) hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow + 1, (int64_t)2, 0);
-    hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, HAS_BACK, DET>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream, P,
-                       g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
-                       (R *)s->tpart);
+    if (s->apic)
+        hipLaunchKernelGGL((k_apic_tile<R, DO_G2P, DO_P2G, HAS_BACK>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream,
+                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
+                           (R *)s->tpart);
+    else
+        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, HAS_BACK, DET>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream,
+                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
+                           (R *)s->tpart);
     HIP_TRY(hipGetLastError());
     return loose_pass_t<R, DO_G2P, DO_P2G, HAS_BACK, DET>(s, dt, zcut, fl);
 }
@@ -929,6 +945,7 @@ template <typename R> TileParts<R> tile_parts(const mpmw_solver *s) {
     tp.tile_start = s->sort.tile_start;
     tp.acc_flag = s->sort.oow + 2;
     tp.tg = s->sort.geom;
+    tp.quad = s->apic ? 1 : 0;
     return tp;
 }
 
@@ -1098,6 +1115,10 @@ template <typename R> int set_particles_impl(mpmw_solver *s, const mpmw_particle
         {F_A0, a->backstress, 6, 0, 0}, {F_A1, a->backstress, 6, 1, 0}, {F_A2, a->backstress, 6, 2, 0},
         {F_A3, a->backstress, 6, 3, 0}, {F_A4, a->backstress, 6, 4, 0}, {F_A5, a->backstress, 6, 5, 0},
         {F_FX, a->fext, 3, 0, 0}, {F_FY, a->fext, 3, 1, 0}, {F_FZ, a->fext, 3, 2, 0},
+        // new particles start with a zero affine matrix
+        {F_C0, nullptr, 1, 0, 0}, {F_C1, nullptr, 1, 0, 0}, {F_C2, nullptr, 1, 0, 0},
+        {F_C3, nullptr, 1, 0, 0}, {F_C4, nullptr, 1, 0, 0}, {F_C5, nullptr, 1, 0, 0},
+        {F_C6, nullptr, 1, 0, 0}, {F_C7, nullptr, 1, 0, 0}, {F_C8, nullptr, 1, 0, 0},
     };
     for (auto &e : map) {
         if (!field_live(s, e.fld)) continue;
@@ -1238,6 +1259,10 @@ template <typename R> int init_slab_impl(mpmw_solver *s, uint64_t seed, int32_t 
                        n, (R)s->m0);
     hipLaunchKernelGGL(k_fill<R>, dim3(std::min(blocks_for(n), 4096)), dim3(kBlock), 0, s->stream, (R *)s->f[F_V],
                        n, (R)s->V0);
+    if (s->apic)
+        for (int c = F_C0; c <= F_C8; ++c)
+            hipLaunchKernelGGL(k_fill<R>, dim3(std::min(blocks_for(n), 4096)), dim3(kBlock), 0, s->stream,
+                               (R *)s->f[c], n, R(0));
     HIP_TRY(hipGetLastError());
     s->p2g_ready = false;
     s->stats_fresh = false;
@@ -1898,9 +1923,66 @@ int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t coun
 
 int mpmw_set_deterministic(mpmw_solver *s, int32_t on) {
     CHECK_S(s);
+    if (on && s->apic) return fail(MPMW_E_STATE, "the deterministic mode is not available with the APIC transfer");
     s->deterministic = on != 0;
     s->p2g_ready = false;
     s->sort.valid = false;
+    return MPMW_OK;
+}
+
+int mpmw_set_transfer(mpmw_solver *s, int32_t mode) {
+    CHECK_S(s);
+    if (mode != MPMW_TRANSFER_PIC && mode != MPMW_TRANSFER_APIC) return fail(MPMW_E_INVAL, "unknown transfer");
+    const bool apic = mode == MPMW_TRANSFER_APIC;
+    if (apic && s->deterministic)
+        return fail(MPMW_E_STATE, "the deterministic mode is not available with the APIC transfer");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->apic = apic;
+    for (int c = F_C0; c <= F_C8; ++c) {
+        if (apic) {
+            int rc = ensure_field(s, c);  // zero-filled
+            if (rc) return rc;
+            if (s->f[c] && s->cap > 0)
+                hipLaunchKernelGGL(k_fill<float>, dim3(std::min(blocks_for((int64_t)s->cap * s->rsz / 4), 4096)),
+                                   dim3(kBlock), 0, s->stream, (float *)s->f[c], (int64_t)s->cap * s->rsz / 4, 0.0f);
+        } else {
+            dfree(s, s->f[c], field_bytes(s, c, s->cap));
+            dfree(s, s->alt[c], field_bytes(s, c, s->alt_cap));
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    // the binning's loose rule and the window layout depend on the stencil
+    s->p2g_ready = false;
+    s->sort.valid = false;
+    return MPMW_OK;
+}
+
+int mpmw_get_affine(mpmw_solver *s, double *C, int32_t n) {
+    CHECK_S(s);
+    if (!s->apic) return fail(MPMW_E_STATE, "get_affine: the transfer is not APIC");
+    if (!C || n < 0) return fail(MPMW_E_INVAL, "get_affine: bad arguments");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const int32_t m = std::min(n, s->n);
+    for (int c = 0; c < 9; ++c) {
+        int rc = s->fp64 ? download_field<double>(s, F_C0 + c, C, m, 9, c)
+                         : download_field<float>(s, F_C0 + c, C, m, 9, c);
+        if (rc) return rc;
+    }
+    return MPMW_OK;
+}
+
+int mpmw_set_affine(mpmw_solver *s, const double *C, int32_t n) {
+    CHECK_S(s);
+    if (!s->apic) return fail(MPMW_E_STATE, "set_affine: the transfer is not APIC");
+    if (!C || n != s->n) return fail(MPMW_E_INVAL, "set_affine: n must equal the particle count");
+    if (!finite_array(C, 9 * (int64_t)n)) return fail(MPMW_E_INVAL, "set_affine: non-finite value");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int c = 0; c < 9; ++c) {
+        int rc = s->fp64 ? upload_field<double>(s, F_C0 + c, C, n, 9, c, 0.0)
+                         : upload_field<float>(s, F_C0 + c, C, n, 9, c, 0.0);
+        if (rc) return rc;
+    }
+    s->p2g_ready = false;   // the next P2G uses the new matrices
     return MPMW_OK;
 }
 
@@ -2042,14 +2124,21 @@ template <typename R> void launch_tile_batch(hipStream_t stream, int variant, co
     case 2: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, true>), grid, blk, 0, stream, items, n); break;
     case 3: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, true>), grid, blk, 0, stream, items, n); break;
     case 4: hipLaunchKernelGGL((k_tile_batch<R, true, false, false, false>), grid, blk, 0, stream, items, n); break;
-    default: hipLaunchKernelGGL((k_tile_batch<R, true, false, true, false>), grid, blk, 0, stream, items, n); break;
+    case 5: hipLaunchKernelGGL((k_tile_batch<R, true, false, true, false>), grid, blk, 0, stream, items, n); break;
+    case 6: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, false>), grid, blk, 0, stream, items, n); break;
+    case 7: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, true>), grid, blk, 0, stream, items, n); break;
+    case 8: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, false>), grid, blk, 0, stream, items, n); break;
+    default: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, true>), grid, blk, 0, stream, items, n); break;
     }
 }
 
 // Tile-pass variant of a member: fused (0-3: backstress, deterministic) or
-// G2P-only before a rebuild (4-5: backstress).
+// G2P-only before a rebuild (4-5: backstress); the APIC transfer's fused
+// (6-7) and G2P-only (8-9) passes.
+constexpr int kTileVariants = 10;
 int tile_variant(const mpmw_solver *s, const StepPlan &sp) {
     const int back = s->has_back ? 1 : 0;
+    if (s->apic) return (sp.next_rebuild ? 8 : 6) + back;
     if (sp.next_rebuild) return 4 + back;
     return back + (s->deterministic ? 2 : 0);
 }
@@ -2097,8 +2186,8 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     std::vector<GridWork<R>> gw(nr);
     std::vector<SumWork> sw(nr);
     std::vector<int32_t *> cnt;
-    std::vector<TileWork<R>> tw[6];
-    int gblocks = 0, tblocks[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<TileWork<R>> tw[kTileVariants];
+    int gblocks = 0, tblocks[kTileVariants] = {};
     for (int r = 0; r < nr; ++r) {
         mpmw_solver *s = members[run[r]];
         const StepPlan &sp = plans[run[r]];
@@ -2146,9 +2235,9 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     // one blob: grid items, sum items, counter pointers, tile items per variant
     size_t off_sw = align256(sizeof(GridWork<R>) * nr);
     size_t off_cnt = off_sw + align256(sizeof(SumWork) * nr);
-    size_t off_tw[6];
+    size_t off_tw[kTileVariants];
     size_t end = off_cnt + align256(sizeof(int32_t *) * std::max<size_t>(cnt.size(), 1));
-    for (int v = 0; v < 6; ++v) {
+    for (int v = 0; v < kTileVariants; ++v) {
         off_tw[v] = end;
         end += align256(sizeof(TileWork<R>) * tw[v].size());
     }
@@ -2182,7 +2271,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
     std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
     if (!cnt.empty()) std::memcpy(h + off_cnt, cnt.data(), sizeof(int32_t *) * cnt.size());
-    for (int v = 0; v < 6; ++v)
+    for (int v = 0; v < kTileVariants; ++v)
         if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
     HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipEventRecord(A.copied, stream));
@@ -2210,7 +2299,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
                            (int32_t *const *)(d + off_cnt), (int)cnt.size());
     {
         Timed t(one, MPMW_K_G2P_P2G);
-        for (int v = 0; v < 6; ++v)
+        for (int v = 0; v < kTileVariants; ++v)
             if (!tw[v].empty())
                 launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
                                      tblocks[v]);

@@ -24,6 +24,10 @@ constexpr int TX = 8, TY = 8, TZ = 4;                 // cells per tile
 constexpr int kTileCells = TX * TY * TZ;              // 256 = one cell per thread
 constexpr int WX = TX + 1, WY = TY + 1, WZ = TZ + 1;  // node window of a tile
 constexpr int WN = WX * WY * WZ;                      // 405 nodes
+// node window of a tile for the quadratic (APIC) stencil: one node before
+// the tile's first cell, two after its last
+constexpr int QX = TX + 3, QY = TY + 3, QZ = TZ + 3;
+constexpr int QN = QX * QY * QZ;                      // 847 nodes
 
 struct ScratchBuf {
     void *ptr = nullptr;

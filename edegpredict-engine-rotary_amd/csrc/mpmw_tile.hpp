@@ -690,10 +690,11 @@ __global__ void __launch_bounds__(kWave) k_loose_p2g_ordered(Particles<R> P, Gri
 
 // Tile-major cell key of every particle on the current grid.  A particle is
 // loose (key nkeys) when its cell (the floor used by the P2G stencil) lies
-// outside the tiles or has a corner node outside the grid; INACTIVE
-// particles get key nkeys + 1 and sort to the end.
+// outside the tiles or has a stencil node outside the grid (trilinear: the
+// cell's corners; quadratic, `quad`: one node before the cell to two after
+// it); INACTIVE particles get key nkeys + 1 and sort to the end.
 template <typename R>
-__global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, uint32_t *keys, int32_t *idx,
+__global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, int quad, uint32_t *keys, int32_t *idx,
                            int32_t *counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
@@ -706,11 +707,13 @@ __global__ void k_bin_keys(Particles<R> P, GridView<R> g, TileGeom tg, uint32_t 
         const int off[3] = {tg.ox, tg.oy, tg.oz};
         int c[3];
         bool tiled = true;
+        const double lo = quad ? 1.0 : 0.0, hi = quad ? 2.0 : 1.0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double fl = floor(r[a]);
-            // in a tile, and both stencil nodes inside the grid (NaN fails too)
-            tiled = tiled && fl >= (double)off[a] && fl < (double)(off[a] + nt[a]) && fl + 1.0 < (double)nodes[a];
+            // in a tile, and the stencil nodes inside the grid (NaN fails too)
+            tiled = tiled && fl >= (double)off[a] && fl < (double)(off[a] + nt[a]) && fl - lo >= 0.0 &&
+                    fl + hi < (double)nodes[a];
             c[a] = tiled ? (int)fl - off[a] : 0;
         }
         if (tiled) {

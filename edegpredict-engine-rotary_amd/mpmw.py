@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmpmw_hip.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
 FP32, FP64 = 0, 1
+TRANSFER_PIC, TRANSFER_APIC = 0, 1
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
 K_G2P_P2G, K_P2G, K_GRID, K_DIFFUSE, K_SORT, K_REDUCE, K_OTHER = range(7)
 
@@ -33,6 +34,7 @@ EXPORTS = (
     "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes", "mpmw_get_diagnostics",
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
+    "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine",
 )
 
 
@@ -129,6 +131,9 @@ def lib():
         "mpmw_set_tool_sdf_mesh": (C.c_int, [vp, C.POINTER(ToolDesc), _dp, C.c_int32, _ip, _dp, C.c_int32,
                                              C.c_double, C.c_int32]),
         "mpmw_get_tool_sdf": (C.c_int, [vp, C.POINTER(C.c_float), _ip, _dp, _dp]),
+        "mpmw_set_transfer": (C.c_int, [vp, C.c_int32]),
+        "mpmw_get_affine": (C.c_int, [vp, _dp, C.c_int32]),
+        "mpmw_set_affine": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
         "mpmw_step": (C.c_int, [vp, C.c_double]),
@@ -309,6 +314,20 @@ class Solver:
         _check(lib().mpmw_get_tool_sdf(self.h, g.ctypes.data_as(C.POINTER(C.c_float)), dims, org, C.byref(vox)),
                "get_tool_sdf")
         return g, tuple(org), vox.value
+
+    def set_transfer(self, mode):
+        """TRANSFER_PIC (trilinear) or TRANSFER_APIC (quadratic; every C set to zero)."""
+        _check(lib().mpmw_set_transfer(self.h, mode), "set_transfer")
+
+    def get_affine(self):
+        n = self.count()
+        C9 = np.zeros((n, 9))
+        _check(lib().mpmw_get_affine(self.h, _p(C9), n), "get_affine")
+        return C9
+
+    def set_affine(self, C9):
+        C9 = np.ascontiguousarray(C9, dtype=np.float64).reshape(-1, 9)
+        _check(lib().mpmw_set_affine(self.h, _p(C9), len(C9)), "set_affine")
 
     def set_tool_pose(self, pose):
         _check(lib().mpmw_set_tool_pose(self.h, C.byref(pose)), "set_tool_pose")

@@ -239,3 +239,5 @@ def apply_to_solver(mpmw, solver, cfg):
         p.omega = pd["omega"]
         solver.set_tool_pose(p)
     solver.set_tool_tip(*cfg["tip"])
+    if cfg.get("transfer") == "apic":
+        solver.set_transfer(mpmw.TRANSFER_APIC)

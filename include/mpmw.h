@@ -45,6 +45,11 @@ enum mpmw_particle_status {
 
 enum mpmw_precision { MPMW_FP32 = 0, MPMW_FP64 = 1 };
 
+/* Particle-grid transfer: the workpiece solver's PIC with trilinear weights
+ * (MPMWorkpieceSolver.cu:52-90), or the APIC variant with quadratic
+ * B-splines (MPMSolver.cu:20-34, 370-421, 554-592). */
+enum mpmw_transfer { MPMW_TRANSFER_PIC = 0, MPMW_TRANSFER_APIC = 1 };
+
 enum mpmw_tool_kind {
     MPMW_TOOL_NONE = 0,
     MPMW_TOOL_POINTS = 1,  /* reference point samples (setToolMesh) */
@@ -229,6 +234,21 @@ int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t coun
  * input agree in every bit.  Particles are re-binned whenever one has changed
  * cell since the last binning.  Takes effect at the next step. */
 int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
+
+/* --- APIC transfer variant: quadratic B-spline weights over 27 nodes with
+ *     base = floor(x/h - 0.5) (getWeightsAndGradients, MPMSolver.cu:20-34),
+ *     P2G momentum m w (v + C (x_i - x_p)) with none from FIXED_BOUNDARY
+ *     particles (p2gKernel :370-421), and from G2P
+ *     C = 0.99 * 3/h^2 * sum w v_i (x_i - x_p)^T, unchanged for CHIP and
+ *     FIXED_BOUNDARY particles (g2pKernel :554-592).  Everything else in the
+ *     step is the workpiece solver's.  Selecting a transfer sets every
+ *     particle's C to zero; new particles start at zero.  The deterministic
+ *     mode is not available with APIC (MPMW_E_STATE either way round). */
+int mpmw_set_transfer(mpmw_solver *s, int32_t mode);
+/* C of the first n particles, 9 values each (row-major); APIC only. */
+int mpmw_get_affine(mpmw_solver *s, double *C, int32_t n);
+/* Replace C of all n particles (n = particle count); APIC only. */
+int mpmw_set_affine(mpmw_solver *s, const double *C, int32_t n);
 
 /* --- inspection (parity / measurement) */
 int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);

@@ -50,6 +50,7 @@ struct orc_solver {
     int64_t step;
     double max_stress, max_temp, ke, contact_force;
     int compactions;
+    int transfer;         /* ORC_TRANSFER_PIC / ORC_TRANSFER_APIC */
 };
 
 static int g_threads = 1;
@@ -103,6 +104,55 @@ void orc_weights(const orc_grid *g, double x, double y, double z,
                 grad[3 * n] = dwx * wy * wz;
                 grad[3 * n + 1] = wx * dwy * wz;
                 grad[3 * n + 2] = wx * wy * dwz;
+            }
+}
+
+/* getWeightsAndGradients, MPMSolver.cu:20-34, along one axis: r is the
+ * position in cells, base = floor(r - 0.5), d0 = r - base in [0.5, 1.5). */
+static void quad_axis(double r, double inv, int *base, double w[3], double dw[3]) {
+    int b = (int)floor(r - 0.5);
+    double d0 = r - b, d1 = d0 - 1.0, d2 = d0 - 2.0;
+    w[0] = 0.5 * (1.5 - d0) * (1.5 - d0);
+    w[1] = 0.75 - d1 * d1;
+    w[2] = 0.5 * (1.5 + d2) * (1.5 + d2);
+    dw[0] = -(1.5 - d0) * inv;
+    dw[1] = -2.0 * d1 * inv;
+    dw[2] = (1.5 + d2) * inv;
+    *base = b;
+}
+
+/* The 27-node stencil (p2gKernel :370-400 / g2pKernel :526-556 loop order:
+ * x outer, z inner), positions relative to the grid origin scaled by
+ * invDx = 1 / h as the reference forms x_frac (:21). */
+void orc_weights_quadratic(const orc_grid *g, double x, double y, double z, int32_t idx[27], double w[27],
+                           double grad[81], double dpos[81]) {
+    const double inv = 1.0 / g->cell;
+    int b[3];
+    double wa[3][3], dwa[3][3];
+    quad_axis((x - g->minx) * inv, inv, &b[0], wa[0], dwa[0]);
+    quad_axis((y - g->miny) * inv, inv, &b[1], wa[1], dwa[1]);
+    quad_axis((z - g->minz) * inv, inv, &b[2], wa[2], dwa[2]);
+    int n = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k, ++n) {
+                const int gx = b[0] + i, gy = b[1] + j, gz = b[2] + k;
+                double *gr = &grad[3 * n], *dp = &dpos[3 * n];
+                if (gx < 0 || gy < 0 || gz < 0 || gx >= g->nx || gy >= g->ny || gz >= g->nz) {
+                    idx[n] = -1;
+                    w[n] = 0.0;
+                    gr[0] = gr[1] = gr[2] = dp[0] = dp[1] = dp[2] = 0.0;
+                    continue;
+                }
+                idx[n] = (gz * g->ny + gy) * g->nx + gx;
+                w[n] = wa[0][i] * wa[1][j] * wa[2][k];
+                gr[0] = dwa[0][i] * wa[1][j] * wa[2][k];
+                gr[1] = wa[0][i] * dwa[1][j] * wa[2][k];
+                gr[2] = wa[0][i] * wa[1][j] * dwa[2][k];
+                /* node position minus particle position (:385-387) */
+                dp[0] = (gx * g->cell + g->minx) - x;
+                dp[1] = (gy * g->cell + g->miny) - y;
+                dp[2] = (gz * g->cell + g->minz) - z;
             }
 }
 
@@ -704,6 +754,80 @@ void orc_apply_external_force(orc_solver *s, const double c[3], double radius, c
 
 /* ---------------------------------------------------------------- step --- */
 
+/* One particle of mpmP2GKernel, MPMWorkpieceSolver.cu:103-133. */
+static void p2g_particle_pic(const orc_grid *g, const orc_particle *p, orc_node *G) {
+    int32_t ids[8];
+    double ws[8], gr[24];
+    orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
+    for (int n = 0; n < 8; ++n) {
+        int gid = ids[n];
+        if (gid < 0 || ws[n] <= 0.0) continue;
+        orc_node *nd = &G[gid];
+        double m = p->mass * ws[n];
+        nd->mass += m;
+        nd->vx += p->vx * m;
+        nd->vy += p->vy * m;
+        nd->vz += p->vz * m;
+        nd->tm += p->temperature * m;
+        if (p->status == ORC_ACTIVE || p->status == ORC_FIXED_BOUNDARY) {
+            double gx = gr[3 * n], gy = gr[3 * n + 1], gz = gr[3 * n + 2];
+            double fX = -(p->s[0] * gx + p->s[3] * gy + p->s[4] * gz) * p->volume;
+            double fY = -(p->s[3] * gx + p->s[1] * gy + p->s[5] * gz) * p->volume;
+            double fZ = -(p->s[4] * gx + p->s[5] * gy + p->s[2] * gz) * p->volume;
+            nd->fx += fX + p->fx * ws[n];
+            nd->fy += fY + p->fy * ws[n];
+            nd->fz += fZ + p->fz * ws[n];
+        } else if (p->status == ORC_CHIP) {
+            nd->fx += p->fx * ws[n];
+            nd->fy += p->fy * ws[n];
+            nd->fz += p->fz * ws[n];
+        }
+    }
+}
+
+/* One particle of the APIC transfer (p2gKernel, MPMSolver.cu:370-421) with
+ * the workpiece solver's channels and force rules: mass m w; momentum
+ * m w (v + C (x_i - x_p)), none from FIXED_BOUNDARY particles (:393-398);
+ * T m w; the stress force of ACTIVE and FIXED_BOUNDARY particles and the
+ * external force as mpmP2GKernel forms them (MPMWorkpieceSolver.cu:121-132),
+ * with the quadratic weight gradients. */
+static void p2g_particle_apic(const orc_grid *g, const orc_particle *p, orc_node *G) {
+    int32_t ids[27];
+    double ws[27], gr[81], dp[81];
+    orc_weights_quadratic(g, p->x, p->y, p->z, ids, ws, gr, dp);
+    const int fixed = p->status == ORC_FIXED_BOUNDARY;
+    const int stress = p->status == ORC_ACTIVE || fixed;
+    for (int n = 0; n < 27; ++n) {
+        if (ids[n] < 0) continue;
+        orc_node *nd = &G[ids[n]];
+        const double m = p->mass * ws[n];
+        nd->mass += m;
+        if (!fixed) {
+            const double *d = &dp[3 * n];
+            const double ax = p->C[0] * d[0] + p->C[1] * d[1] + p->C[2] * d[2];
+            const double ay = p->C[3] * d[0] + p->C[4] * d[1] + p->C[5] * d[2];
+            const double az = p->C[6] * d[0] + p->C[7] * d[1] + p->C[8] * d[2];
+            nd->vx += m * (p->vx + ax);
+            nd->vy += m * (p->vy + ay);
+            nd->vz += m * (p->vz + az);
+        }
+        nd->tm += p->temperature * m;
+        if (stress) {
+            const double gx = gr[3 * n], gy = gr[3 * n + 1], gz = gr[3 * n + 2];
+            const double fX = -(p->s[0] * gx + p->s[3] * gy + p->s[4] * gz) * p->volume;
+            const double fY = -(p->s[3] * gx + p->s[1] * gy + p->s[5] * gz) * p->volume;
+            const double fZ = -(p->s[4] * gx + p->s[5] * gy + p->s[2] * gz) * p->volume;
+            nd->fx += fX + p->fx * ws[n];
+            nd->fy += fY + p->fy * ws[n];
+            nd->fz += fZ + p->fz * ws[n];
+        } else if (p->status == ORC_CHIP) {
+            nd->fx += p->fx * ws[n];
+            nd->fy += p->fy * ws[n];
+            nd->fz += p->fz * ws[n];
+        }
+    }
+}
+
 /* mpmP2GKernel, MPMWorkpieceSolver.cu:98-134 (serial: atomics become plain adds;
  * with several threads each thread fills a private grid that is summed after). */
 static void p2g(orc_solver *s) {
@@ -726,33 +850,8 @@ static void p2g(orc_solver *s) {
         for (int pi = 0; pi < s->n; ++pi) {
             const orc_particle *p = &s->p[pi];
             if (p->status == ORC_INACTIVE) continue;
-            int32_t ids[8];
-            double ws[8], gr[24];
-            orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
-            for (int n = 0; n < 8; ++n) {
-                int gid = ids[n];
-                if (gid < 0 || ws[n] <= 0.0) continue;
-                orc_node *nd = &G[gid];
-                double m = p->mass * ws[n];
-                nd->mass += m;
-                nd->vx += p->vx * m;
-                nd->vy += p->vy * m;
-                nd->vz += p->vz * m;
-                nd->tm += p->temperature * m;
-                if (p->status == ORC_ACTIVE || p->status == ORC_FIXED_BOUNDARY) {
-                    double gx = gr[3 * n], gy = gr[3 * n + 1], gz = gr[3 * n + 2];
-                    double fX = -(p->s[0] * gx + p->s[3] * gy + p->s[4] * gz) * p->volume;
-                    double fY = -(p->s[3] * gx + p->s[1] * gy + p->s[5] * gz) * p->volume;
-                    double fZ = -(p->s[4] * gx + p->s[5] * gy + p->s[2] * gz) * p->volume;
-                    nd->fx += fX + p->fx * ws[n];
-                    nd->fy += fY + p->fy * ws[n];
-                    nd->fz += fZ + p->fz * ws[n];
-                } else if (p->status == ORC_CHIP) {
-                    nd->fx += p->fx * ws[n];
-                    nd->fy += p->fy * ws[n];
-                    nd->fz += p->fz * ws[n];
-                }
-            }
+            if (s->transfer == ORC_TRANSFER_APIC) p2g_particle_apic(g, p, G);
+            else p2g_particle_pic(g, p, G);
         }
     }
     if (priv) {
@@ -970,10 +1069,99 @@ void orc_constitutive(const orc_params *p, orc_particle *pt, const double L[9], 
     chip_transition(p, pt);
 }
 
-/* mpmG2PKernel, MPMWorkpieceSolver.cu:264-400 */
-static void g2p(orc_solver *s, double dt) {
+/* One particle of mpmG2PKernel, MPMWorkpieceSolver.cu:264-400. */
+static void g2p_particle_pic(const orc_solver *s, orc_particle *p, double dt) {
     const orc_grid *g = &s->g;
     const orc_params *mat = &s->prm;
+    int32_t ids[8];
+    double ws[8], gr[24];
+    orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
+    double v[3] = {0, 0, 0}, temp = 0.0, wsum = 0.0;
+    double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n = 0; n < 8; ++n) {
+        int gid = ids[n];
+        if (gid < 0 || ws[n] <= 0.0) continue;
+        const orc_node *nd = &s->grid[gid];
+        if (nd->mass <= 1e-18) continue;
+        double nv[3] = {nd->vx, nd->vy, nd->vz};
+        v[0] += nv[0] * ws[n]; v[1] += nv[1] * ws[n]; v[2] += nv[2] * ws[n];
+        temp += (nd->tm / nd->mass) * ws[n];
+        wsum += ws[n];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) L[3 * a + b] += nv[a] * gr[3 * n + b];
+    }
+    if (p->status == ORC_FIXED_BOUNDARY) {
+        p->vx = p->vy = p->vz = 0.0;
+        p->fx = p->fy = p->fz = 0.0;
+        return;
+    }
+    p->vx = v[0]; p->vy = v[1]; p->vz = v[2];
+    p->x += p->vx * dt;
+    p->y += p->vy * dt;
+    p->z += p->vz * dt;
+    if (wsum > 1e-8) p->temperature = fmin(fmax(temp / wsum, mat->ref_temp), mat->melt_temp);
+    if (p->status == ORC_ACTIVE) constitutive_active(mat, p, L, dt);
+    chip_transition(mat, p);
+    p->fx = p->fy = p->fz = 0.0;
+}
+
+/* One particle of the APIC transfer's G2P (g2pKernel, MPMSolver.cu:506-592)
+ * followed by the workpiece solver's update (mpmG2PKernel,
+ * MPMWorkpieceSolver.cu:296-399): the same gather over the 27 quadratic
+ * nodes (nodes of mass <= 1e-18 skipped, as the workpiece grid marks them
+ * empty), plus B = sum w v_i (x_i - x_p)^T; then C = B * 3 / h^2 (:577-578,
+ * the reference's factor; the quadratic B-spline's D^-1 is 4 / h^2) and
+ * C *= 0.99 (:591), except for particles that were CHIP at entry, whose C
+ * the reference leaves unchanged (:570-573). */
+static void g2p_particle_apic(const orc_solver *s, orc_particle *p, double dt) {
+    const orc_grid *g = &s->g;
+    const orc_params *mat = &s->prm;
+    int32_t ids[27];
+    double ws[27], gr[81], dp[81];
+    orc_weights_quadratic(g, p->x, p->y, p->z, ids, ws, gr, dp);
+    double v[3] = {0, 0, 0}, temp = 0.0, wsum = 0.0;
+    double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int n = 0; n < 27; ++n) {
+        if (ids[n] < 0) continue;
+        const orc_node *nd = &s->grid[ids[n]];
+        if (nd->mass <= 1e-18) continue;
+        const double nv[3] = {nd->vx, nd->vy, nd->vz};
+        const double w = ws[n];
+        v[0] += w * nv[0]; v[1] += w * nv[1]; v[2] += w * nv[2];
+        temp += (nd->tm / nd->mass) * w;
+        wsum += w;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                B[3 * a + b] += w * nv[a] * dp[3 * n + b];
+                L[3 * a + b] += nv[a] * gr[3 * n + b];
+            }
+    }
+    if (p->status == ORC_FIXED_BOUNDARY) {
+        p->vx = p->vy = p->vz = 0.0;
+        p->fx = p->fy = p->fz = 0.0;
+        return;
+    }
+    const int chip0 = p->status == ORC_CHIP;
+    p->vx = v[0]; p->vy = v[1]; p->vz = v[2];
+    p->x += p->vx * dt;
+    p->y += p->vy * dt;
+    p->z += p->vz * dt;
+    if (wsum > 1e-8) p->temperature = fmin(fmax(temp / wsum, mat->ref_temp), mat->melt_temp);
+    if (p->status == ORC_ACTIVE) constitutive_active(mat, p, L, dt);
+    chip_transition(mat, p);
+    if (!chip0) {
+        const double inv = 1.0 / g->cell;
+        const double dinv = 3.0 * inv * inv;
+        for (int c = 0; c < 9; ++c) {
+            p->C[c] = B[c] * dinv;
+            p->C[c] *= 0.99;
+        }
+    }
+    p->fx = p->fy = p->fz = 0.0;
+}
+
+/* mpmG2PKernel, MPMWorkpieceSolver.cu:264-400 */
+static void g2p(orc_solver *s, double dt) {
     const int nt = g_threads;
 #ifdef _OPENMP
 #pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
@@ -981,36 +1169,8 @@ static void g2p(orc_solver *s, double dt) {
     for (int pi = 0; pi < s->n; ++pi) {
         orc_particle *p = &s->p[pi];
         if (p->status == ORC_INACTIVE) continue;
-        int32_t ids[8];
-        double ws[8], gr[24];
-        orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
-        double v[3] = {0, 0, 0}, temp = 0.0, wsum = 0.0;
-        double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int n = 0; n < 8; ++n) {
-            int gid = ids[n];
-            if (gid < 0 || ws[n] <= 0.0) continue;
-            const orc_node *nd = &s->grid[gid];
-            if (nd->mass <= 1e-18) continue;
-            double nv[3] = {nd->vx, nd->vy, nd->vz};
-            v[0] += nv[0] * ws[n]; v[1] += nv[1] * ws[n]; v[2] += nv[2] * ws[n];
-            temp += (nd->tm / nd->mass) * ws[n];
-            wsum += ws[n];
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) L[3 * a + b] += nv[a] * gr[3 * n + b];
-        }
-        if (p->status == ORC_FIXED_BOUNDARY) {
-            p->vx = p->vy = p->vz = 0.0;
-            p->fx = p->fy = p->fz = 0.0;
-            continue;
-        }
-        p->vx = v[0]; p->vy = v[1]; p->vz = v[2];
-        p->x += p->vx * dt;
-        p->y += p->vy * dt;
-        p->z += p->vz * dt;
-        if (wsum > 1e-8) p->temperature = fmin(fmax(temp / wsum, mat->ref_temp), mat->melt_temp);
-        if (p->status == ORC_ACTIVE) constitutive_active(mat, p, L, dt);
-        chip_transition(mat, p);
-        p->fx = p->fy = p->fz = 0.0;
+        if (s->transfer == ORC_TRANSFER_APIC) g2p_particle_apic(s, p, dt);
+        else g2p_particle_pic(s, p, dt);
     }
 }
 
@@ -1073,6 +1233,20 @@ void orc_compact(orc_solver *s) {
     if (k <= 0) return;
     s->n = k;
     s->compactions++;
+}
+
+void orc_set_transfer(orc_solver *s, int mode) {
+    s->transfer = mode == ORC_TRANSFER_APIC ? ORC_TRANSFER_APIC : ORC_TRANSFER_PIC;
+    for (int i = 0; i < s->n; ++i) memset(s->p[i].C, 0, sizeof(s->p[i].C));
+}
+
+int orc_get_affine(const orc_solver *s, double *C) {
+    for (int i = 0; i < s->n; ++i) memcpy(C + 9 * (size_t)i, s->p[i].C, sizeof(s->p[i].C));
+    return s->n;
+}
+
+void orc_set_affine(orc_solver *s, const double *C) {
+    for (int i = 0; i < s->n; ++i) memcpy(s->p[i].C, C + 9 * (size_t)i, sizeof(s->p[i].C));
 }
 
 /* MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791 */

@@ -36,7 +36,13 @@ typedef struct {
     double a[6];   /* backstress xx yy zz xy xz yz */
     int32_t id;
     int32_t status;
+    double C[9];   /* APIC affine matrix, row-major (MPMSolver.cuh MPMParticle::C) */
 } orc_particle;
+
+/* Particle-grid transfer: the workpiece solver's PIC with trilinear weights,
+ * or the APIC variant with quadratic B-splines (MPMSolver.cu:20-34,370-421,
+ * 554-592). */
+enum { ORC_TRANSFER_PIC = 0, ORC_TRANSFER_APIC = 1 };
 
 /* MPMGridNodeGPU, include/MPMWorkpieceSolver.cuh:47-57 */
 typedef struct {
@@ -128,6 +134,12 @@ void orc_set_tool_pose(orc_solver *s, const orc_tool_pose *p);
 void orc_set_tool_tip(orc_solver *s, double x, double y, double z);
 void orc_apply_external_force(orc_solver *s, const double c[3], double radius, const double f[3]);
 
+/* Select the transfer; the affine matrices start at zero. */
+void orc_set_transfer(orc_solver *s, int mode);
+/* Affine matrices of all particles, 9 per particle (row-major). */
+int orc_get_affine(const orc_solver *s, double *C);
+void orc_set_affine(orc_solver *s, const double *C);
+
 void orc_step(orc_solver *s, double dt);
 void orc_run(orc_solver *s, int nsteps, double dt);
 double orc_stable_dt(const orc_solver *s);
@@ -150,6 +162,12 @@ int orc_get_grid_state(const orc_solver *s, double *mass, double *vel, double *t
 /* Single-function entry points for known-answer tests. */
 void orc_weights(const orc_grid *g, double x, double y, double z,
                  int32_t idx[8], double w[8], double grad[24]);
+/* getWeightsAndGradients (MPMSolver.cu:20-34) on all three axes: the 27
+ * nodes of the quadratic stencil, x outermost and z innermost as the APIC
+ * loops visit them (:370-378), with weight, gradient and node offset
+ * x_i - x_p; nodes outside the grid get index -1. */
+void orc_weights_quadratic(const orc_grid *g, double x, double y, double z, int32_t idx[27], double w[27],
+                           double grad[81], double dpos[81]);
 double orc_equivalent_stress(double xx, double yy, double zz, double xy, double xz, double yz);
 double orc_jc_flow(const orc_params *p, double eps_p, double eps_rate, double temp);
 void orc_rebuild_params(double cell, const double mn[3], const double mx[3], orc_grid *g);

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liborc_mpm.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
+TRANSFER_PIC, TRANSFER_APIC = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -65,7 +66,8 @@ class Particle(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "volume", "density",
         "temperature", "plasticStrain", "strainRate", "damage", "residualStress")] + [
-        ("s", C.c_double * 6), ("a", C.c_double * 6), ("id", C.c_int32), ("status", C.c_int32)]
+        ("s", C.c_double * 6), ("a", C.c_double * 6), ("id", C.c_int32), ("status", C.c_int32),
+        ("C", C.c_double * 9)]
 
 
 _lib = None
@@ -142,6 +144,12 @@ def lib():
         L.orc_sdf_sample.restype = C.c_double
         L.orc_sdf_in_range.argtypes = [C.POINTER(Sdf), _dp]
         L.orc_set_tool_sdf.argtypes = [C.c_void_p, C.POINTER(Sdf), C.POINTER(ToolDesc)]
+        L.orc_set_transfer.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_affine.argtypes = [C.c_void_p, _dp]
+        L.orc_get_affine.restype = C.c_int
+        L.orc_set_affine.argtypes = [C.c_void_p, _dp]
+        L.orc_weights_quadratic.argtypes = [C.POINTER(Grid), C.c_double, C.c_double, C.c_double, _ip, _dp, _dp,
+                                            _dp]
         _lib = L
     return _lib
 
@@ -241,6 +249,19 @@ class Oracle:
     def set_tool_pose(self, pose):
         lib().orc_set_tool_pose(self.h, C.byref(pose))
 
+    def set_transfer(self, mode):
+        """TRANSFER_PIC (trilinear) or TRANSFER_APIC (quadratic, affine C = 0)."""
+        lib().orc_set_transfer(self.h, mode)
+
+    def get_affine(self):
+        C9 = np.zeros((self.count(), 9))
+        lib().orc_get_affine(self.h, _p(C9))
+        return C9
+
+    def set_affine(self, C9):
+        C9 = np.ascontiguousarray(C9, dtype=np.float64).reshape(self.count(), 9)
+        lib().orc_set_affine(self.h, _p(C9))
+
     def set_tool_tip(self, x, y, z):
         lib().orc_set_tool_tip(self.h, x, y, z)
 
@@ -310,6 +331,16 @@ def weights(grid, x, y, z):
     g = np.zeros(24)
     lib().orc_weights(C.byref(grid), x, y, z, _p(idx, _ip), _p(w), _p(g))
     return idx, w, g.reshape(8, 3)
+
+
+def weights_quadratic(grid, x, y, z):
+    """The 27-node quadratic stencil: (idx, w, grad[27, 3], dpos[27, 3])."""
+    idx = np.zeros(27, dtype=np.int32)
+    w = np.zeros(27)
+    g = np.zeros(81)
+    d = np.zeros(81)
+    lib().orc_weights_quadratic(C.byref(grid), x, y, z, _p(idx, _ip), _p(w), _p(g), _p(d))
+    return idx, w, g.reshape(27, 3), d.reshape(27, 3)
 
 
 def equivalent_stress(*s):

@@ -49,6 +49,8 @@ def build_oracle(cfg):
     if cfg.get("mutate"):
         cfg["mutate"](d)
     o.set_particles(d)
+    if cfg.get("transfer") == "apic":
+        o.set_transfer(oracle.TRANSFER_APIC)
     kind = cfg["tool"][0]
     if kind == "points":
         o.set_tool_samples(cfg["tool"][1], cfg["tool"][2])
