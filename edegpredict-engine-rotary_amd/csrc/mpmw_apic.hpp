@@ -105,7 +105,7 @@ __device__ __forceinline__ void quad_gather_window(const vec4_t<R> *s_node, int 
 }
 
 template <typename R>
-__device__ __forceinline__ void quad_gather_global(const GridView<R> &g, const vec4_t<R> *node, const QFrac<R> &q,
+__device__ __forceinline__ void quad_gather_global(const GridView<R> &g, gp<const vec4_t<R>> node, const QFrac<R> &q,
                                                    R h, GatherSums<R> &s, R B[9]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -186,7 +186,7 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
 // Scatter into the node accumulators with device atomics (nodes outside the
 // grid skipped, as p2gKernel :371-378 skips them).
 template <typename R>
-__device__ __forceinline__ void quad_scatter_global(const GridView<R> &g, R *acc, const QFrac<R> &q, R h,
+__device__ __forceinline__ void quad_scatter_global(const GridView<R> &g, gp<R> acc, const QFrac<R> &q, R h,
                                                     const QP2G<R> &a) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -16,6 +16,27 @@ namespace mpmw {
 enum : int32_t { kActive = 0, kInactive = 1, kChip = 2, kBoundary = 3, kFixed = 4 };
 enum : int32_t { kToolNone = 0, kToolPoints = 1, kToolWedge = 2, kToolDrill = 3, kToolVoxel = 4 };
 
+// A pointer into device memory whose element accesses are global-memory
+// instructions.  A plain pointer read from a work item in memory (a batch's
+// per-solver records) is generic to the compiler, which then emits flat
+// loads and stores: they count in both the vector-memory and the LDS /
+// scalar wait counters, so every wait for an LDS result or a scalar load
+// also waits for the outstanding particle loads.  Indexing through gp casts
+// to the global address space, which gives global instructions with a
+// scalar base and a 32-bit vector offset.
+template <typename T> struct gp {
+    T *p = nullptr;
+    gp() = default;
+    __host__ __device__ gp(T *q) : p(q) {}
+    template <typename U> __host__ __device__ gp(const gp<U> &o) : p(o.p) {}
+    __host__ __device__ operator T *() const { return p; }
+#if defined(__HIP_DEVICE_COMPILE__)
+    __device__ __forceinline__ __attribute__((address_space(1))) T &operator[](int64_t i) const {
+        return ((__attribute__((address_space(1))) T *)p)[i];
+    }
+#endif
+};
+
 template <typename R> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<double> { using type = double4; };
@@ -69,15 +90,15 @@ template <typename R> struct GridView {
 // coordinate, and the reference's near-empty free-surface nodes turn weight
 // errors of that size into large velocity errors.
 template <typename R> struct Particles {
-    double *x, *y, *z;
-    R *vx, *vy, *vz;
-    R *s0, *s1, *s2, *s3, *s4, *s5;
-    R *a0, *a1, *a2, *a3, *a4, *a5;
-    R *fx, *fy, *fz;
-    R *eps, *rate, *dmg, *sres, *T, *m, *V;
-    R *aff[9];                 // APIC affine matrix, row-major (transfer variant only)
+    gp<double> x, y, z;
+    gp<R> vx, vy, vz;
+    gp<R> s0, s1, s2, s3, s4, s5;
+    gp<R> a0, a1, a2, a3, a4, a5;
+    gp<R> fx, fy, fz;
+    gp<R> eps, rate, dmg, sres, T, m, V;
+    gp<R> aff[9];              // APIC affine matrix, row-major (transfer variant only)
     R m0, V0;                  // mass and volume of every particle when m == nullptr
-    int32_t *status, *id;
+    gp<int32_t> status, id;
     int32_t n;
 };
 
@@ -86,17 +107,17 @@ template <typename R> struct Particles {
 // node of mass <= 1e-18 (skipped by G2P, MPMWorkpieceSolver.cu:286).
 // gm/gtm: post-update mass and T*m read by the diffusion stencil.
 template <typename R> struct Grid {
-    R *acc;
-    vec4_t<R> *node;
-    R *gm, *gtm;
-    R *snap;  // optional copy of acc (8 channels) for parity inspection
+    gp<R> acc;
+    gp<vec4_t<R>> node;
+    gp<R> gm, gtm;
+    gp<R> snap;  // optional copy of acc (8 channels) for parity inspection
 };
 
 // Tool state for the grid update.  Tool geometry (node positions, sample
 // distances, SDF and its gradient) is evaluated in FP64 in every mode.
 struct Tool {
     int32_t kind, nsamples;
-    const double *sx, *sy, *sz;       // point samples (SoA)
+    gp<const double> sx, sy, sz;      // point samples (SoA)
     double lo[3], hi[3];              // sample AABB grown by r_c
     double rc2;
     // analytic
@@ -107,7 +128,7 @@ struct Tool {
     double radius, tan_helix, sin_half_point, cos_half_point, flute_offset, flute_radius, flute_length;
     // voxel SDF (rest frame): value of voxel (i, j, k) at origin + (i, j, k
     // + 0.5) * voxel, index i + nx * (j + ny * k)
-    const float *sdf;
+    gp<const float> sdf;
     int32_t snx, sny, snz, pad_;
     double sox, soy, soz, svox;
 };
@@ -117,11 +138,19 @@ struct Tool {
 template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return fmax(a, b); }
 template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return fmin(a, b); }
 
+// Square root and quotient of the particle update.  FP32: the hardware
+// square root and reciprocal (within 1 ulp each; the IEEE-rounded forms cost
+// a correction sequence of ~15 instructions apiece); FP64: IEEE.
+template <typename R> __device__ __forceinline__ R vsqrt(R x) { return sqrt(x); }
+template <> __device__ __forceinline__ float vsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename R> __device__ __forceinline__ R vdiv(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float vdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // equivalentStress, MPMWorkpieceSolver.cu:18-24
 template <typename R>
 __device__ __forceinline__ R von_mises(R xx, R yy, R zz, R xy, R xz, R yz) {
     R s1 = xx - yy, s2 = yy - zz, s3 = zz - xx;
-    return sqrt(R(0.5) * (s1 * s1 + s2 * s2 + s3 * s3 + R(6) * (xy * xy + xz * xz + yz * yz)));
+    return vsqrt(R(0.5) * (s1 * s1 + s2 * s2 + s3 * s3 + R(6) * (xy * xy + xz * xz + yz * yz)));
 }
 
 // johnsonCookStress, MPMWorkpieceSolver.cu:43-50
@@ -331,7 +360,7 @@ __device__ __forceinline__ void constitutive_active(const Material<R> &M, PState
     R tr = Dxx + Dyy + Dzz;
     const R tr3 = divc(tr, R(3), R(1) / R(3));
     R devXX = Dxx - tr3, devYY = Dyy - tr3, devZZ = Dzz - tr3;
-    R eqRate = sqrt(R(2) / R(3) *
+    R eqRate = vsqrt(R(2) / R(3) *
                     (devXX * devXX + devYY * devYY + devZZ * devZZ + R(2) * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz)));
     p.rate = eqRate;
     R pressureInc = M.bulk * tr * dt;
@@ -389,7 +418,7 @@ template <typename R> __device__ __forceinline__ void chip_transition(const Mate
 // chipFractureAndSeparationKernel, MPMWorkpieceSolver.cu:667-725.  The
 // particle id (seed of the fracture kick) is read only when a kick happens.
 template <typename R>
-__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, const int32_t *ids, int i,
+__device__ __forceinline__ void chip_pass(const Material<R> &M, PState<R> &p, gp<const int32_t> ids, int i,
                                           double zcut) {
     if (p.status == kActive) {
         if (p.z < zcut && p.eps > R(1e-6)) {

@@ -129,9 +129,9 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 // particles, and in the atomic mode particles scattered outside their tile's
 // window).
 template <typename R> struct TileParts {
-    const R *part;
-    const int32_t *tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
-    const int32_t *acc_flag;
+    gp<const R> part;
+    gp<const int32_t> tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
+    gp<const int32_t> acc_flag;
     TileGeom tg;
     int32_t quad;                // windows of the quadratic stencil (QX x QY x QZ)
 };
@@ -179,7 +179,7 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
             for (int d = 0; d < nxs; ++d) {
                 const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
                 if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
-                const R *src = tp.part + (int64_t)t * (8 * en) + (lz[a] * ey + ly[b]) * ex + lx[d];
+                const gp<const R> src = tp.part + (int64_t)t * (8 * en) + (lz[a] * ey + ly[b]) * ex + lx[d];
 #pragma unroll
                 for (int ch = 0; ch < 8; ++ch) c[ch] += src[ch * en];
             }

@@ -40,7 +40,7 @@ template <typename R> struct GatherSums {
 // Gather over the 8 corners from device memory; corners outside the grid,
 // with w <= 0 or with an empty node (NaN record) are skipped (:280-286).
 template <typename R>
-__device__ __forceinline__ void gather_global(const GridView<R> &g, const vec4_t<R> *node, const Frac<R> &f,
+__device__ __forceinline__ void gather_global(const GridView<R> &g, gp<const vec4_t<R>> node, const Frac<R> &f,
                                               GatherSums<R> &q) {
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
@@ -60,12 +60,104 @@ __device__ __forceinline__ void gather_global(const GridView<R> &g, const vec4_t
 
 __device__ __forceinline__ int wnode(int x, int y, int z) { return (z * WY + y) * WX + x; }
 
+// The LDS window holds node records with the empty node (mass <= 1e-18,
+// or outside the grid) as (0, 0, 0, -1): a live node's temperature is
+// never negative, so liveness is T >= 0 and an empty node adds zeros to
+// every gather sum without a select per component.
+template <typename R> __device__ __forceinline__ vec4_t<R> window_record(const vec4_t<R> &nd) {
+    vec4_t<R> r = nd;
+    if (!(nd.w == nd.w)) { r.x = r.y = r.z = R(0); r.w = R(-1); }
+    return r;
+}
+
+template <typename R> __device__ __forceinline__ vec4_t<R> empty_record() {
+    vec4_t<R> r;
+    r.x = r.y = r.z = R(0);
+    r.w = R(-1);
+    return r;
+}
+
+// Products of two axis weights: pyz[dy][dz] = wy * wz, pxz[dx][dz],
+// pxy[dx][dy].  Corner (dx, dy, dz) has weight wx[dx] * pyz[dy][dz] and
+// weight gradient (sx pyz, sy pxz, sz pxy) / h with s = -1 for the near
+// node and +1 for the far node of each axis.
+template <typename R> struct WeightPairs {
+    R wx[2], yz[2][2], xz[2][2], xy[2][2];
+};
+
+template <typename R> __device__ __forceinline__ WeightPairs<R> weight_pairs(const Frac<R> &f) {
+    WeightPairs<R> P;
+    const R wx[2] = {f.w0[0], f.w1[0]}, wy[2] = {f.w0[1], f.w1[1]}, wz[2] = {f.w0[2], f.w1[2]};
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        P.wx[a] = wx[a];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            P.yz[a][b] = wy[a] * wz[b];
+            P.xz[a][b] = wx[a] * wz[b];
+            P.xy[a][b] = wx[a] * wy[b];
+        }
+    }
+    return P;
+}
+
+// Every corner has a positive weight (no axis fraction is zero): then no
+// corner is skipped and the factored forms below hold.
+template <typename R> __device__ __forceinline__ bool all_corners_live(const Frac<R> &f) {
+    return f.w1[0] > R(0) && f.w1[1] > R(0) && f.w1[2] > R(0) && f.w0[0] > R(0) && f.w0[1] > R(0) &&
+           f.w0[2] > R(0);
+}
+
 // The same gather from the tile's LDS window, base corner at window index
-// wb: every corner is inside the window, and window nodes outside the grid
-// hold the empty (NaN) record.  Branch-free: a skipped corner adds zeros.
+// wb: every corner is inside the window (window records as window_record
+// writes them).  When every corner weight is positive the velocity
+// gradient is formed along each axis from node differences,
+// L[a][x] = (1/h) sum over (dy, dz) of wy wz (v_a(1, dy, dz) - v_a(0, dy, dz))
+// and likewise for y and z: the same sum as sum_c v_c grad w_c, with an
+// empty node's zero velocity standing for the skipped corner, but without
+// the cancellation of large equal velocities (a translating body) that
+// the corner-by-corner sum suffers in FP32.  Otherwise the corners with
+// w <= 0 are skipped one by one (:280-286).
 template <typename R>
 __device__ __forceinline__ void gather_window(const vec4_t<R> *s_node, int wb, const Frac<R> &f, R inv_h,
                                               GatherSums<R> &q) {
+    if (all_corners_live(f)) {
+        const WeightPairs<R> P = weight_pairs(f);
+        vec4_t<R> nd[8];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) nd[n] = s_node[wb + wnode(n & 1, (n >> 1) & 1, (n >> 2) & 1)];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
+            const R w = P.wx[dx] * P.yz[dy][dz];
+            q.v0 = fma(nd[n].x, w, q.v0); q.v1 = fma(nd[n].y, w, q.v1); q.v2 = fma(nd[n].z, w, q.v2);
+            q.temp = fma(rmax(nd[n].w, R(0)), w, q.temp);
+            q.wsum += nd[n].w >= R(0) ? w : R(0);
+        }
+        R L[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) L[c] = R(0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                // x pairs (0, a, b) - (1, a, b); y pairs (a, 0, b) - (a, 1, b); z pairs (a, b, 0) - (a, b, 1)
+                const int x0 = 2 * a + 4 * b, y0 = a + 4 * b, z0 = a + 2 * b;
+                const R px = P.yz[a][b], py = P.xz[a][b], pz = P.xy[a][b];
+                L[0] = fma(nd[x0 + 1].x - nd[x0].x, px, L[0]);
+                L[3] = fma(nd[x0 + 1].y - nd[x0].y, px, L[3]);
+                L[6] = fma(nd[x0 + 1].z - nd[x0].z, px, L[6]);
+                L[1] = fma(nd[y0 + 2].x - nd[y0].x, py, L[1]);
+                L[4] = fma(nd[y0 + 2].y - nd[y0].y, py, L[4]);
+                L[7] = fma(nd[y0 + 2].z - nd[y0].z, py, L[7]);
+                L[2] = fma(nd[z0 + 4].x - nd[z0].x, pz, L[2]);
+                L[5] = fma(nd[z0 + 4].y - nd[z0].y, pz, L[5]);
+                L[8] = fma(nd[z0 + 4].z - nd[z0].z, pz, L[8]);
+            }
+#pragma unroll
+        for (int c = 0; c < 9; ++c) q.L[c] = fma(L[c], inv_h, q.L[c]);
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         const int dx = n & 1, dy = (n >> 1) & 1, dz = (n >> 2) & 1;
@@ -74,7 +166,7 @@ __device__ __forceinline__ void gather_window(const vec4_t<R> *s_node, int wb, c
         const R wz = dz ? f.w1[2] : f.w0[2];
         const R w = wx * wy * wz;
         const vec4_t<R> nd = s_node[wb + wnode(dx, dy, dz)];
-        const bool ok = (w > R(0)) && (nd.w == nd.w);
+        const bool ok = (w > R(0)) && (nd.w >= R(0));
         const R nx = ok ? nd.x : R(0), ny = ok ? nd.y : R(0), nz = ok ? nd.z : R(0), nt = ok ? nd.w : R(0);
         const R gx = (dx ? inv_h : -inv_h) * wy * wz;
         const R gy = wx * (dy ? inv_h : -inv_h) * wz;
@@ -147,7 +239,7 @@ __device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const
         p.x += (double)p.vx * dtd;
         p.y += (double)p.vy * dtd;
         p.z += (double)p.vz * dtd;
-        if (q.wsum > R(1e-8)) p.T = rmin(rmax(q.temp / q.wsum, M.ref_temp), M.melt);
+        if (q.wsum > R(1e-8)) p.T = rmin(rmax(vdiv(q.temp, q.wsum), M.ref_temp), M.melt);
         if (act) constitutive_active<R, HAS_BACK>(M, p, q.L, dt);
         chip_transition(M, p);
     }
@@ -225,17 +317,49 @@ template <typename R> struct CellAcc {
 
 // Add one particle's P2G contributions (mpmP2GKernel :110-133) to the
 // accumulators of its base cell.  Corners with w <= 0 are skipped as the
-// reference skips them (:112).
+// reference skips them (:112).  With every corner weight positive the
+// stress term -V sigma grad w is formed from the weight pairs with
+// -V sigma / h scaled once per particle; `ext` adds the external force
+// (only when the solver has one).
 template <typename R>
 __device__ __forceinline__ void cell_accumulate(CellAcc<R> &A, const Frac<R> &f, R inv_h, R mass, R vol, R vx, R vy,
                                                 R vz, R T, const R s[6], R fex, R fey, R fez, bool stress_force,
-                                                bool ext_only) {
+                                                bool ext_only, bool ext) {
+    const bool fe = ext && (stress_force || ext_only);
+    const R ex = fe ? fex : R(0), ey = fe ? fey : R(0), ez = fe ? fez : R(0);
+    if (all_corners_live(f)) {
+        const WeightPairs<R> P = weight_pairs(f);
+        const R sc = stress_force ? -vol * inv_h : R(0);
+        const R qxx = s[0] * sc, qyy = s[1] * sc, qzz = s[2] * sc;
+        const R qxy = s[3] * sc, qxz = s[4] * sc, qyz = s[5] * sc;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+            const R w = P.wx[dx] * P.yz[dy][dz];
+            const R gx = dx ? P.yz[dy][dz] : -P.yz[dy][dz];
+            const R gy = dy ? P.xz[dx][dz] : -P.xz[dx][dz];
+            const R gz = dz ? P.xy[dx][dy] : -P.xy[dx][dy];
+            const R m = mass * w;
+            A.v[0][c] += m;
+            A.v[1][c] = fma(vx, m, A.v[1][c]);
+            A.v[2][c] = fma(vy, m, A.v[2][c]);
+            A.v[3][c] = fma(vz, m, A.v[3][c]);
+            A.v[7][c] = fma(T, m, A.v[7][c]);
+            A.v[4][c] = fma(qxz, gz, fma(qxy, gy, fma(qxx, gx, A.v[4][c])));
+            A.v[5][c] = fma(qyz, gz, fma(qyy, gy, fma(qxy, gx, A.v[5][c])));
+            A.v[6][c] = fma(qzz, gz, fma(qyz, gy, fma(qxz, gx, A.v[6][c])));
+            if (ext) {
+                A.v[4][c] = fma(ex, w, A.v[4][c]);
+                A.v[5][c] = fma(ey, w, A.v[5][c]);
+                A.v[6][c] = fma(ez, w, A.v[6][c]);
+            }
+        }
+        return;
+    }
     // branch-free form of p2g_corner: a corner with w <= 0 adds zeros, the
     // stress term is zero unless stress_force, the external force unless
     // stress_force or ext_only
     const R volm = stress_force ? vol : R(0);
-    const bool fe = stress_force || ext_only;
-    const R ex = fe ? fex : R(0), ey = fe ? fey : R(0), ez = fe ? fez : R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
@@ -264,7 +388,7 @@ __device__ __forceinline__ void cell_accumulate(CellAcc<R> &A, const Frac<R> &f,
 // atomics when the stencil lies in the tile window, else device atomics into
 // the node accumulators (then returns true).
 template <typename R>
-__device__ __forceinline__ bool scatter_exception(const GridView<R> &g, R *acc, R *s_acc, const Frac<R> &f,
+__device__ __forceinline__ bool scatter_exception(const GridView<R> &g, gp<R> acc, R *s_acc, const Frac<R> &f,
                                                   int wx0, int wy0, int wz0, R mass, R vol, R vx, R vy, R vz, R T,
                                                   const R s[6], R fex, R fey, R fez, bool stress_force,
                                                   bool ext_only) {
@@ -420,10 +544,9 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                 const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
                 vec4_t<R> nd;
                 if (i < g.nx && j < g.ny && k < g.nz) {
-                    nd = G.node[((int64_t)k * g.ny + j) * g.nx + i];
+                    nd = window_record<R>(G.node[((int64_t)k * g.ny + j) * g.nx + i]);
                 } else {
-                    nd.x = nd.y = nd.z = R(0);
-                    nd.w = R(__builtin_nan(""));
+                    nd = empty_record<R>();
                 }
                 s_node[w] = nd;
             }
@@ -525,7 +648,7 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                 const bool ext_only = (p.status == kChip);
                 if (home_ok && f.b[0] == hgx && f.b[1] == hgy && f.b[2] == hgz) {
                     cell_accumulate(A, f, g.inv_h, mass, vol, p.vx, p.vy, p.vz, p.T, p.s, fex, fey, fez,
-                                    stress_force, ext_only);
+                                    stress_force, ext_only, P.fx != nullptr);
                 } else if (DET) {
                     ++my_exc;
                 } else {
