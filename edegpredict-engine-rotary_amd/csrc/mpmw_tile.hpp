@@ -206,10 +206,16 @@ template <typename R> __device__ __forceinline__ void load_g2p_in(const Particle
     in.s[3] = P.s3[i]; in.s[4] = P.s4[i]; in.s[5] = P.s5[i];
 }
 
+// Which fields a particle's G2P changed (the stores g2p_store makes).
+enum : uint32_t { kStPos = 1, kStActive = 2, kStEps = 4, kStDmg = 8, kStT = 16, kStStatus = 32, kStFext = 64 };
+
+// The G2P update of one particle without its stores (except the rare
+// residual-stress update, a read-modify-write); returns the kSt* set of
+// fields to store.
 template <typename R, bool HAS_BACK, typename Gather>
-__device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
-                                             const Material<R> &M, R dt, double dtd, bool chip_step,
-                                             bool need_stress, double zcut, Gather gather) {
+__device__ __forceinline__ uint32_t g2p_compute(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
+                                                const Material<R> &M, R dt, double dtd, bool chip_step,
+                                                bool need_stress, double zcut, Gather gather) {
     const int32_t st0 = in.status;
     const bool act = st0 == kActive, fixed = st0 == kFixed;
     p.status = st0;
@@ -244,9 +250,28 @@ __device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const
         chip_transition(M, p);
     }
     if (chip_step) chip_pass(M, p, P.id, i, zcut);
-    if (!fixed) { P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z; }
+    uint32_t st = 0;
+    st |= fixed ? 0u : kStPos;
+    st |= act ? kStActive : 0u;
+    st |= p.eps != eps0 ? kStEps : 0u;
+    st |= p.dmg != dmg0 ? kStDmg : 0u;
+    st |= p.T != T0 ? kStT : 0u;
+    st |= p.status != st0 ? kStStatus : 0u;
+    st |= P.fx ? kStFext : 0u;
+    if (p.sres > -__builtin_huge_valf()) {
+        const R old = P.sres[i];
+        if (p.sres > old) P.sres[i] = p.sres;
+    }
+    return st;
+}
+
+// The stores of one particle's G2P: only fields that changed, velocity
+// always, the external force zeroed (:399).
+template <typename R, bool HAS_BACK>
+__device__ __forceinline__ void g2p_store(const Particles<R> &P, int i, const PState<R> &p, uint32_t st) {
+    if (st & kStPos) { P.x[i] = p.x; P.y[i] = p.y; P.z[i] = p.z; }
     P.vx[i] = p.vx; P.vy[i] = p.vy; P.vz[i] = p.vz;
-    if (act) {
+    if (st & kStActive) {
         P.s0[i] = p.s[0]; P.s1[i] = p.s[1]; P.s2[i] = p.s[2];
         P.s3[i] = p.s[3]; P.s4[i] = p.s[4]; P.s5[i] = p.s[5];
         if (HAS_BACK) {
@@ -255,15 +280,21 @@ __device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const
         }
         P.rate[i] = p.rate;
     }
-    if (p.eps != eps0) P.eps[i] = p.eps;
-    if (p.dmg != dmg0) P.dmg[i] = p.dmg;
-    if (p.T != T0) P.T[i] = p.T;
-    if (p.status != st0) P.status[i] = p.status;
-    if (p.sres > -__builtin_huge_valf()) {
-        const R old = P.sres[i];
-        if (p.sres > old) P.sres[i] = p.sres;
-    }
-    if (P.fx) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }   // :399
+    if (st & kStEps) P.eps[i] = p.eps;
+    if (st & kStDmg) P.dmg[i] = p.dmg;
+    if (st & kStT) P.T[i] = p.T;
+    if (st & kStStatus) P.status[i] = p.status;
+    if (st & kStFext) { P.fx[i] = R(0); P.fy[i] = R(0); P.fz[i] = R(0); }
+}
+
+// One particle through mpmG2PKernel (MPMWorkpieceSolver.cu:264-400) and, on
+// chip steps, chipFractureAndSeparationKernel (:667-725), stores included.
+template <typename R, bool HAS_BACK, typename Gather>
+__device__ __forceinline__ void g2p_particle(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
+                                             const Material<R> &M, R dt, double dtd, bool chip_step,
+                                             bool need_stress, double zcut, Gather gather) {
+    const uint32_t st = g2p_compute<R, HAS_BACK>(P, i, in, p, M, dt, dtd, chip_step, need_stress, zcut, gather);
+    g2p_store<R, HAS_BACK>(P, i, p, st);
 }
 
 __device__ __forceinline__ void accumulate_stats(double st[kRed], double x, double y, double z, double vm, double T,
@@ -539,20 +570,31 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
     stats_identity(st, ref_temp);
 
     if (b0 < b1) {  // uniform over the workgroup
+        // Prologue: the window's node records and the first slot's particle
+        // inputs are requested before anything waits on them (the first
+        // slot is b0 + tid when the tile turns out uniform, else unused).
+        static_assert(WN <= 2 * kBlock, "two window records per thread");
+        vec4_t<R> wrec[2];
         if (DO_G2P) {
-            for (int w = tid; w < WN; w += kBlock) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int w = tid + r * kBlock;
                 const int i = wx0 + w % WX, j = wy0 + (w / WX) % WY, k = wz0 + w / (WX * WY);
-                vec4_t<R> nd;
-                if (i < g.nx && j < g.ny && k < g.nz) {
-                    nd = window_record<R>(G.node[((int64_t)k * g.ny + j) * g.nx + i]);
-                } else {
-                    nd = empty_record<R>();
-                }
-                s_node[w] = nd;
+                if (w < WN && i < g.nx && j < g.ny && k < g.nz) wrec[r] = G.node[((int64_t)k * g.ny + j) * g.nx + i];
+                else wrec[r] = empty_record<R>();
             }
         }
+        G2PIn<R> nxt;
+        if (DO_G2P) load_g2p_in(P, cnt > 0 && b0 + tid < b1 ? b0 + tid : b0, nxt);
         if (DO_P2G) {
             for (int w = tid; w < 8 * WN; w += kBlock) s_acc[w] = R(0);
+        }
+        if (DO_G2P) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int w = tid + r * kBlock;
+                if (w < WN) s_node[w] = window_record<R>(wrec[r]);
+            }
         }
         // largest and smallest particle count of the tile's cells
         int mx = cnt, mn = cnt;
@@ -580,43 +622,52 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
         int my_oow = 0, my_exc = 0, my_acc = 0;
         const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int base = 0;
-        // uniform tiles: slot s of this thread's cell is particle b0 + s * kBlock
-        // + tid; its G2P inputs are loaded one slot ahead
-        G2PIn<R> nxt;
-        if (DO_G2P && uniform && cnt > 0) load_g2p_in(P, b0 + tid, nxt);
-        for (int s = 0; s < maxc; ++s) {
-            const bool active = cnt > s;
-            G2PIn<R> cur;
-            if (DO_G2P && uniform) {
-                cur = nxt;
-                if (cnt > s + 1) load_g2p_in(P, b0 + (s + 1) * kBlock + tid, nxt);
+        // Particle index of this thread's cell in slot s: uniform tiles
+        // b0 + s * kBlock + tid; otherwise the slot-major position (cells
+        // with more than s particles, in key order), from a ballot and a
+        // prefix over the waves.
+        auto slot_index = [&](int sl) -> int {
+            if (uniform) return b0 + sl * kBlock + tid;
+            const unsigned long long mask = __ballot(cnt > sl);
+            if (lane == 0) s_cnt[sl & 1][wv] = __popcll(mask);
+            __syncthreads();
+            int before = 0, total = 0;
+            for (int w = 0; w < kBlock / kWave; ++w) {
+                const int c = s_cnt[sl & 1][w];
+                before += (w < wv) ? c : 0;
+                total += c;
             }
-            int i;
-            if (uniform) {
-                i = b0 + s * kBlock + tid;
-            } else {
-                // slot-major position: cells with more than s particles, in key order
-                const unsigned long long mask = __ballot(active);
-                if (lane == 0) s_cnt[s & 1][wv] = __popcll(mask);
-                __syncthreads();
-                int before = 0, total = 0;
-                for (int w = 0; w < kBlock / kWave; ++w) {
-                    const int c = s_cnt[s & 1][w];
-                    before += (w < wv) ? c : 0;
-                    total += c;
-                }
-                i = b0 + base + before + __popcll(mask & lt);
-                base += total;
+            const int idx = b0 + base + before + __popcll(mask & lt);
+            base += total;
+            return idx;
+        };
+        // A slot's G2P inputs are loaded one slot ahead (slot 0 in the
+        // prologue), unconditionally (a branch joining two versions of them
+        // would move the loaded registers, waiting for the loads on the
+        // spot); a slot past a cell's last particle loads particle b0 and
+        // does not use it.  A slot's G2P stores are made in the next slot,
+        // after that slot has taken its prefetched inputs (after the loop
+        // for the last): the wait for the inputs then covers loads and
+        // stores issued a slot earlier, not stores just issued (the vector-
+        // memory counter is in order and counts both).
+        PState<R> pend;
+        int pend_i = -1;
+        uint32_t pend_st = 0;
+        auto flush = [&]() {
+            if (DO_G2P && pend_i >= 0) {
+                g2p_store<R, HAS_BACK>(P, pend_i, pend, pend_st);
+                pend_i = -1;
             }
-            if (!active) continue;
+        };
+        auto process = [&](const G2PIn<R> &cur, int i, bool active) {
+            if (!active) return;
             PState<R> p;
             if (DO_G2P) {
-                if (!uniform) load_g2p_in(P, i, cur);
                 p.status = cur.status;
             } else {
                 p.status = P.status[i];
             }
-            if (p.status == kInactive) continue;
+            if (p.status == kInactive) return;
             const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
             R fex = R(0), fey = R(0), fez = R(0);
             if (DO_G2P) {
@@ -626,11 +677,13 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                                 (unsigned)lz < (unsigned)TZ;
                 const int wb = wnode(lx, ly, lz);
                 my_oow += in ? 0 : 1;
-                g2p_particle<R, HAS_BACK>(P, i, cur, p, M, dt, dtd, flags.chip != 0, stats, zcut,
-                                          [&](GatherSums<R> &q) {
-                                              if (in) gather_window(s_node, wb, f, g.inv_h, q);
-                                              else gather_global(g, G.node, f, q);
-                                          });
+                pend_st = g2p_compute<R, HAS_BACK>(P, i, cur, p, M, dt, dtd, flags.chip != 0, stats, zcut,
+                                                   [&](GatherSums<R> &q) {
+                                                       if (in) gather_window(s_node, wb, f, g.inv_h, q);
+                                                       else gather_global(g, G.node, f, q);
+                                                   });
+                pend = p;
+                pend_i = i;
                 if (stats) {
                     const double vx = p.vx, vy = p.vy, vz = p.vz;
                     accumulate_stats(st, p.x, p.y, p.z,
@@ -659,7 +712,28 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                                   : 0;
                 }
             }
+        };
+        int i_next = slot_index(0);
+        if (DO_G2P && !uniform) load_g2p_in(P, cnt > 0 ? i_next : b0, nxt);
+        for (int s = 0; s < maxc; ++s) {
+            G2PIn<R> cur;
+            if (DO_G2P) {
+                cur = nxt;
+                // the prefetched inputs are complete here, before this
+                // slot's loads and the previous slot's stores are issued
+                // (the asm uses them and keeps memory operations on its side)
+                asm volatile("" ::"v"(cur.x), "v"(cur.y), "v"(cur.z), "v"(cur.T), "v"(cur.eps), "v"(cur.dmg),
+                             "v"(cur.s[0]), "v"(cur.s[1]), "v"(cur.s[2]), "v"(cur.s[3]), "v"(cur.s[4]),
+                             "v"(cur.s[5]), "v"(cur.status)
+                             : "memory");
+            }
+            flush();
+            const int i = i_next;
+            i_next = slot_index(s + 1);
+            if (DO_G2P) load_g2p_in(P, cnt > s + 1 ? i_next : b0, nxt);
+            process(cur, i, cnt > s);
         }
+        flush();
         if (DO_P2G) {
             __syncthreads();  // exception atomics are in
             combine_layer(A, s_acc, hx, hy, hz, 0);
