@@ -174,9 +174,15 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
     const int ex = q ? QX : WX, ey = q ? QY : WY, en = q ? QN : WN;
 #pragma unroll
     for (int ch = 0; ch < 8; ++ch) c[ch] = R(0);
-    for (int a = 0; a < nzs; ++a)
-        for (int b = 0; b < nys; ++b)
-            for (int d = 0; d < nxs; ++d) {
+    // at most two tiles per axis; unrolled so that the loads of every
+    // window are in flight together
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                if (a >= nzs || b >= nys || d >= nxs) continue;
                 const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
                 if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
                 const gp<const R> src = tp.part + (int64_t)t * (8 * en) + (lz[a] * ey + ly[b]) * ex + lx[d];
@@ -216,13 +222,13 @@ __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> 
         R tm = c[7];
         bool live = mass > R(1e-18);
         if (live) {
-            vx /= mass; vy /= mass; vz /= mass;
+            vx = vdiv(vx, mass); vy = vdiv(vy, mass); vz = vdiv(vz, mass);
             if (k <= g.kfix) {
                 vx = vy = vz = R(0);
             } else {
-                vx += (fxn / mass) * dt;
-                vy += (fyn / mass) * dt;
-                vz += (fzn / mass) * dt;
+                vx += vdiv(fxn, mass) * dt;
+                vy += vdiv(fyn, mass) * dt;
+                vz += vdiv(fzn, mass) * dt;
                 // node position nodePosition (:37-41), in FP64
                 const double px = g.ox + i * g.hd, py = g.oy + j * g.hd, pz = g.oz + k * g.hd;
                 if (tool.kind == kToolPoints) {
@@ -298,7 +304,7 @@ __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> 
         }
         vec4_t<R> nd;
         nd.x = vx; nd.y = vy; nd.z = vz;
-        nd.w = live ? (direct_T ? tm / mass : R(0)) : R(__builtin_nan(""));
+        nd.w = live ? (direct_T ? vdiv(tm, mass) : R(0)) : R(__builtin_nan(""));
         G.node[idx] = nd;
         G.gm[idx] = mass;
         G.gtm[idx] = tm;
@@ -314,11 +320,13 @@ __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Gr
                                                   const Material<R> &M, const Tool &tool, R dt, int direct_T,
                                                   double *force_part, int blk) {
     double cf = 0.0;
-    const int64_t idx = (int64_t)blk * kBlock + threadIdx.x;
+    // node counts are below 2^31 (the grid's index type is int32 per axis
+    // and the binning keys are 32-bit)
+    const int idx = blk * kBlock + (int)threadIdx.x;
     if (idx < g.nn) {
-        const int i = (int)(idx % g.nx);
-        const int64_t row = idx / g.nx;
-        const int j = (int)(row % g.ny), k = (int)(row / g.ny);
+        const int row = idx / g.nx;
+        const int i = idx - row * g.nx;
+        const int k = row / g.ny, j = row - k * g.ny;
         cf = grid_node(g, G, tp, M, tool, dt, direct_T, idx, i, j, k);
     }
     // fixed-order block sum of the contact force
@@ -343,29 +351,29 @@ __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G
 // reads the post-update field (gm, gtm), writes the gather temperature.
 template <typename R>
 __device__ __forceinline__ void diffuse_block(const GridView<R> &g, const Grid<R> &G, R alpha, R dt, int blk) {
-    const int64_t idx = (int64_t)blk * kBlock + threadIdx.x;
+    const int idx = blk * kBlock + (int)threadIdx.x;
     if (idx >= g.nn) return;
     const R mass = G.gm[idx];
     if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
     R tm = G.gtm[idx];
     if (mass >= R(1e-15)) {
-        const R T = tm / mass;
+        const R T = vdiv(tm, mass);
         const R inv_h2 = (R)(1.0 / (g.hd * g.hd));
-        const int i = (int)(idx % g.nx);
-        const int64_t t = idx / g.nx;
-        const int j = (int)(t % g.ny), k = (int)(t / g.ny);
-        const int64_t nxy = (int64_t)g.nx * g.ny;
+        const int t = idx / g.nx;
+        const int i = idx - t * g.nx;
+        const int k = t / g.ny, j = t - k * g.ny;
+        const int nxy = g.nx * g.ny;
         R lap = R(0);
-        if (i > 0) lap += (G.gtm[idx - 1] / rmax(G.gm[idx - 1], R(1e-30)) - T);
-        if (i < g.nx - 1) lap += (G.gtm[idx + 1] / rmax(G.gm[idx + 1], R(1e-30)) - T);
-        if (j > 0) lap += (G.gtm[idx - g.nx] / rmax(G.gm[idx - g.nx], R(1e-30)) - T);
-        if (j < g.ny - 1) lap += (G.gtm[idx + g.nx] / rmax(G.gm[idx + g.nx], R(1e-30)) - T);
-        if (k > 0) lap += (G.gtm[idx - nxy] / rmax(G.gm[idx - nxy], R(1e-30)) - T);
-        if (k < g.nz - 1) lap += (G.gtm[idx + nxy] / rmax(G.gm[idx + nxy], R(1e-30)) - T);
+        if (i > 0) lap += (vdiv(G.gtm[idx - 1], rmax(G.gm[idx - 1], R(1e-30))) - T);
+        if (i < g.nx - 1) lap += (vdiv(G.gtm[idx + 1], rmax(G.gm[idx + 1], R(1e-30))) - T);
+        if (j > 0) lap += (vdiv(G.gtm[idx - g.nx], rmax(G.gm[idx - g.nx], R(1e-30))) - T);
+        if (j < g.ny - 1) lap += (vdiv(G.gtm[idx + g.nx], rmax(G.gm[idx + g.nx], R(1e-30))) - T);
+        if (k > 0) lap += (vdiv(G.gtm[idx - nxy], rmax(G.gm[idx - nxy], R(1e-30))) - T);
+        if (k < g.nz - 1) lap += (vdiv(G.gtm[idx + nxy], rmax(G.gm[idx + nxy], R(1e-30))) - T);
         const R T_new = T + alpha * dt * lap * inv_h2;
         tm = rmax(T_new, R(0)) * mass;
     }
-    G.node[idx].w = tm / mass;
+    G.node[idx].w = vdiv(tm, mass);
 }
 
 template <typename R>
@@ -411,30 +419,36 @@ __global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> 
     grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, (int)blockIdx.x - W.block0);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_diffuse_batch(const GridWork<R> *__restrict__ items, int nitems) {
-    const int w = find_item(items, nitems, (int)blockIdx.x);
-    const GridWork<R> &W = items[w];
-    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, (int)blockIdx.x - W.block0);
-}
-
-// One block per item: the fixed-order sum of its contact-force partials.
+// The fixed-order sum of an item's contact-force partials and, when
+// `counters` is set, the reset of its P2G-pass flags (counters[1] and [2],
+// read by the grid update, written by the next particle pass).
 struct SumWork {
     const double *part;
     int32_t nparts, pad_;
     const int32_t *modes;
     double *out;
+    int32_t *counters;
 };
 
-__global__ void __launch_bounds__(kBlock) k_finalize_batch(const SumWork *__restrict__ items) {
-    const SumWork &W = items[blockIdx.x];
+__device__ __forceinline__ void sum_block(const SumWork &W) {
     finalize_block(W.part, W.nparts, 1, W.modes, W.out);
+    if (W.counters && threadIdx.x == 0) { W.counters[1] = 0; W.counters[2] = 0; }
 }
 
-// Zero counters[1] and counters[2] of every item (the P2G-pass flags).
-__global__ void k_clear_flags_batch(int32_t *const *counters, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) { counters[i][1] = 0; counters[i][2] = 0; }
+// The pass after the grid update, one launch: blocks [0, gblocks) diffuse
+// the members' grids (items with diffusion), blocks gblocks + r sum member
+// r's contact force and reset its flags.
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_post_batch(const GridWork<R> *__restrict__ items, int nitems,
+                                                       const SumWork *__restrict__ sums, int gblocks) {
+    const int b = (int)blockIdx.x;
+    if (b >= gblocks) {
+        sum_block(sums[b - gblocks]);
+        return;
+    }
+    const int w = find_item(items, nitems, b);
+    const GridWork<R> &W = items[w];
+    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, b - W.block0);
 }
 
 // ------------------------------------------------------------ small ops ---

@@ -609,7 +609,9 @@ int rebuild_grid_from_stats(mpmw_solver *s) {
     if (!dim(mn[0], mx[0], s->nx) || !dim(mn[1], mx[1], s->ny) || !dim(mn[2], mx[2], s->nz))
         return fail(MPMW_E_RANGE, "particle bounds give a grid beyond 1e9 nodes per axis");
     const int64_t nn = (int64_t)s->nx * s->ny * s->nz;
-    if (nn > (int64_t)1 << 33) return fail(MPMW_E_RANGE, "grid of " + std::to_string(nn) + " nodes is too large");
+    // node indices are 32-bit in the grid kernels
+    if (nn >= (int64_t)1 << 31)
+        return fail(MPMW_E_RANGE, "grid of " + std::to_string(nn) + " nodes is too large (limit 2^31 - 1)");
     // fixture: nodes with pos.z <= minZ + fixture + h are clamped (:153)
     const double fixture = s->prm.has_config ? s->prm.fixture_thickness : 0.002;
     s->kfix = -1;
@@ -2185,7 +2187,6 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     // work items
     std::vector<GridWork<R>> gw(nr);
     std::vector<SumWork> sw(nr);
-    std::vector<int32_t *> cnt;
     std::vector<TileWork<R>> tw[kTileVariants];
     int gblocks = 0, tblocks[kTileVariants] = {};
     for (int r = 0; r < nr; ++r) {
@@ -2210,7 +2211,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         sw[r].nparts = g.blocks;
         sw[r].modes = s->d_modes + kRed;
         sw[r].out = s->d_cf;
-        if (!sp.next_rebuild) cnt.push_back(s->sort.oow);
+        sw[r].counters = sp.next_rebuild ? nullptr : s->sort.oow;
         const int v = tile_variant(s, sp);
         TileWork<R> t;
         t.P = pview<R>(s);
@@ -2232,77 +2233,70 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         tblocks[v] += t.blocks;
         tw[v].push_back(t);
     }
-    // one blob: grid items, sum items, counter pointers, tile items per variant
-    size_t off_sw = align256(sizeof(GridWork<R>) * nr);
-    size_t off_cnt = off_sw + align256(sizeof(SumWork) * nr);
-    size_t off_tw[kTileVariants];
-    size_t end = off_cnt + align256(sizeof(int32_t *) * std::max<size_t>(cnt.size(), 1));
-    for (int v = 0; v < kTileVariants; ++v) {
-        off_tw[v] = end;
-        end += align256(sizeof(TileWork<R>) * tw[v].size());
-    }
-    StepStaging::Area &A = st.area[st.parity];
-    st.parity ^= 1;
-    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
-    if (A.pending) {
-        HIP_TRY(hipEventSynchronize(A.copied));
-        A.pending = false;
-    }
-    if (end > A.cap) {
-        pool::put(A.h);
-        A.h = nullptr;
-        A.cap = 0;
-        int dev = 0;
-        HIP_TRY(hipGetDevice(&dev));
-        HIP_TRY(pool::get(dev, true, end * 2, &A.h));
-        A.cap = end * 2;
-    }
-    if (end > st.d_cap) {
-        HIP_TRY(hipStreamSynchronize(stream));
-        pool::put(st.d_blob);
-        st.d_blob = nullptr;
-        st.d_cap = 0;
-        int dev = 0;
-        HIP_TRY(hipGetDevice(&dev));
-        HIP_TRY(pool::get(dev, false, end * 2, &st.d_blob));
-        st.d_cap = end * 2;
-    }
-    char *h = (char *)A.h;
-    std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
-    std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
-    if (!cnt.empty()) std::memcpy(h + off_cnt, cnt.data(), sizeof(int32_t *) * cnt.size());
-    for (int v = 0; v < kTileVariants; ++v)
-        if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
-    HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipEventRecord(A.copied, stream));
-    A.pending = true;
-    char *d = (char *)st.d_blob;
-    const GridWork<R> *d_gw = (const GridWork<R> *)d;
     // a batch of one keeps the per-kernel timing of its solver
     mpmw_solver *one = nr == 1 ? members[run[0]] : nullptr;
-    bool any_diffusion = false;
-    for (int r = 0; r < nr; ++r) any_diffusion = any_diffusion || gw[r].diffuse;
     {
-        Timed t(one, MPMW_K_GRID);
-        hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
-    }
-    {
-        Timed t(one, MPMW_K_REDUCE);
-        hipLaunchKernelGGL(k_finalize_batch, dim3(nr), dim3(kBlock), 0, stream, (const SumWork *)(d + off_sw));
-    }
-    if (any_diffusion) {
-        Timed t(one, MPMW_K_DIFFUSE);
-        hipLaunchKernelGGL(k_diffuse_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
-    }
-    if (!cnt.empty())
-        hipLaunchKernelGGL(k_clear_flags_batch, dim3((int)(cnt.size() + 63) / 64), dim3(64), 0, stream,
-                           (int32_t *const *)(d + off_cnt), (int)cnt.size());
-    {
-        Timed t(one, MPMW_K_G2P_P2G);
+        // one blob: grid items, sum items, tile items per variant
+        size_t off_sw = align256(sizeof(GridWork<R>) * nr);
+        size_t off_tw[kTileVariants];
+        size_t end = off_sw + align256(sizeof(SumWork) * nr);
+        for (int v = 0; v < kTileVariants; ++v) {
+            off_tw[v] = end;
+            end += align256(sizeof(TileWork<R>) * tw[v].size());
+        }
+        StepStaging::Area &A = st.area[st.parity];
+        st.parity ^= 1;
+        if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
+        if (A.pending) {
+            HIP_TRY(hipEventSynchronize(A.copied));
+            A.pending = false;
+        }
+        if (end > A.cap) {
+            pool::put(A.h);
+            A.h = nullptr;
+            A.cap = 0;
+            int dev = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(pool::get(dev, true, end * 2, &A.h));
+            A.cap = end * 2;
+        }
+        if (end > st.d_cap) {
+            HIP_TRY(hipStreamSynchronize(stream));
+            pool::put(st.d_blob);
+            st.d_blob = nullptr;
+            st.d_cap = 0;
+            int dev = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(pool::get(dev, false, end * 2, &st.d_blob));
+            st.d_cap = end * 2;
+        }
+        char *h = (char *)A.h;
+        std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
+        std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
         for (int v = 0; v < kTileVariants; ++v)
-            if (!tw[v].empty())
-                launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
-                                     tblocks[v]);
+            if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
+        HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipEventRecord(A.copied, stream));
+        A.pending = true;
+        char *d = (char *)st.d_blob;
+        const GridWork<R> *d_gw = (const GridWork<R> *)d;
+        {
+            Timed t(one, MPMW_K_GRID);
+            hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
+        }
+        {
+            // diffusion, contact-force sums and flag resets in one launch
+            Timed t(one, MPMW_K_DIFFUSE);
+            hipLaunchKernelGGL(k_post_batch<R>, dim3(gblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
+                               (const SumWork *)(d + off_sw), gblocks);
+        }
+        {
+            Timed t(one, MPMW_K_G2P_P2G);
+            for (int v = 0; v < kTileVariants; ++v)
+                if (!tw[v].empty())
+                    launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
+                                         tblocks[v]);
+        }
     }
     HIP_TRY(hipGetLastError());
     for (int r = 0; r < nr; ++r)
