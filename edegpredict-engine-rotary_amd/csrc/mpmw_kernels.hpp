@@ -353,23 +353,29 @@ template <typename R>
 __device__ __forceinline__ void diffuse_block(const GridView<R> &g, const Grid<R> &G, R alpha, R dt, int blk) {
     const int idx = blk * kBlock + (int)threadIdx.x;
     if (idx >= g.nn) return;
+    const int t = idx / g.nx;
+    const int i = idx - t * g.nx;
+    const int k = t / g.ny, j = t - k * g.ny;
+    const int nxy = g.nx * g.ny;
+    // the six neighbours' (T m, m), read together with the node's own (a
+    // missing neighbour reads the node itself and is left out)
+    const int nb[6] = {i > 0 ? idx - 1 : idx, i < g.nx - 1 ? idx + 1 : idx,
+                       j > 0 ? idx - g.nx : idx, j < g.ny - 1 ? idx + g.nx : idx,
+                       k > 0 ? idx - nxy : idx, k < g.nz - 1 ? idx + nxy : idx};
+    const bool has[6] = {i > 0, i < g.nx - 1, j > 0, j < g.ny - 1, k > 0, k < g.nz - 1};
     const R mass = G.gm[idx];
-    if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
     R tm = G.gtm[idx];
+    R ntm[6], nm[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) { ntm[q] = G.gtm[nb[q]]; nm[q] = G.gm[nb[q]]; }
+    if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
     if (mass >= R(1e-15)) {
         const R T = vdiv(tm, mass);
         const R inv_h2 = (R)(1.0 / (g.hd * g.hd));
-        const int t = idx / g.nx;
-        const int i = idx - t * g.nx;
-        const int k = t / g.ny, j = t - k * g.ny;
-        const int nxy = g.nx * g.ny;
         R lap = R(0);
-        if (i > 0) lap += (vdiv(G.gtm[idx - 1], rmax(G.gm[idx - 1], R(1e-30))) - T);
-        if (i < g.nx - 1) lap += (vdiv(G.gtm[idx + 1], rmax(G.gm[idx + 1], R(1e-30))) - T);
-        if (j > 0) lap += (vdiv(G.gtm[idx - g.nx], rmax(G.gm[idx - g.nx], R(1e-30))) - T);
-        if (j < g.ny - 1) lap += (vdiv(G.gtm[idx + g.nx], rmax(G.gm[idx + g.nx], R(1e-30))) - T);
-        if (k > 0) lap += (vdiv(G.gtm[idx - nxy], rmax(G.gm[idx - nxy], R(1e-30))) - T);
-        if (k < g.nz - 1) lap += (vdiv(G.gtm[idx + nxy], rmax(G.gm[idx + nxy], R(1e-30))) - T);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (has[q]) lap += (vdiv(ntm[q], rmax(nm[q], R(1e-30))) - T);
         const R T_new = T + alpha * dt * lap * inv_h2;
         tm = rmax(T_new, R(0)) * mass;
     }
