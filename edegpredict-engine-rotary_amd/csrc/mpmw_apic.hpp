@@ -154,10 +154,48 @@ __device__ __forceinline__ void quad_channels(const QP2G<R> &a, R w, R gx, R gy,
     c[7] = a.T * m;
 }
 
+// Adds the live channels of c[8] (bit ch of live) to p[ch * QN] in LDS.
+// FP64 uses the LDS atomic add.  FP32 uses compare-and-swap: gfx950 runs
+// ds_add_f32 one lane at a time (0.33 lane-adds per clock per CU measured,
+// against 4.6 for the compare-and-swap loop and 7.8 for ds_add_f64).  Each
+// pass reads the eight words, then swaps in old + c and keeps the lanes
+// whose word changed in between; every add is one FP32 add of the current
+// sum, as the atomic add is.
+__device__ __forceinline__ void lds_add8(double *p, const double c[8], unsigned live) {
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch)
+        if (live & (1u << ch)) atomicAdd(p + ch * QN, c[ch]);
+}
+
+__device__ __forceinline__ void lds_add8(float *p, const float c[8], unsigned live) {
+    unsigned *q = reinterpret_cast<unsigned *>(p);
+    unsigned old[8];
+#pragma unroll
+    for (int ch = 0; ch < 8; ++ch)
+        old[ch] = (live & (1u << ch)) ? __hip_atomic_load(q + ch * QN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                      : 0u;
+    while (live) {
+        unsigned again = 0;
+#pragma unroll
+        for (int ch = 0; ch < 8; ++ch) {
+            if (!(live & (1u << ch))) continue;
+            unsigned expect = old[ch];
+            const unsigned want = __float_as_uint(__uint_as_float(expect) + c[ch]);
+            if (!__hip_atomic_compare_exchange_strong(q + ch * QN, &expect, want, __ATOMIC_RELAXED,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                old[ch] = expect;
+                again |= 1u << ch;
+            }
+        }
+        live = again;
+    }
+}
+
 // Scatter into the LDS window (window index wb of the base node, channel
-// stride QN) with LDS atomics.
+// stride QN).
 template <typename R>
 __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFrac<R> &q, R h, const QP2G<R> &a) {
+    const unsigned live = 0x81u | (a.momentum ? 0x0eu : 0u) | (a.ext ? 0x70u : 0u);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -167,19 +205,7 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
                 R w, gx, gy, gz, dx, dy, dz, c[8];
                 quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
                 quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
-                R *p = s_acc + wb + qnode(i, j, k);
-                atomicAdd(p, c[0]);
-                if (a.momentum) {
-                    atomicAdd(p + QN, c[1]);
-                    atomicAdd(p + 2 * QN, c[2]);
-                    atomicAdd(p + 3 * QN, c[3]);
-                }
-                if (a.ext) {
-                    atomicAdd(p + 4 * QN, c[4]);
-                    atomicAdd(p + 5 * QN, c[5]);
-                    atomicAdd(p + 6 * QN, c[6]);
-                }
-                atomicAdd(p + 7 * QN, c[7]);
+                lds_add8(s_acc + wb + qnode(i, j, k), c, live);
             }
 }
 
