@@ -10,8 +10,8 @@
 // Same tiles and binning as the trilinear pass.  A particle of cell c has
 // its base at c - 1 or c, so the node window of a tile of TX x TY x TZ cells
 // spans one node before the tile and two after it: QX x QY x QZ nodes.  The
-// 27 x 8 contributions of a particle go into the LDS window with LDS
-// atomics (a register accumulation per cell does not fit 27 nodes); the
+// 27 x 8 contributions of a particle go into the LDS window with LDS adds
+// (lds_add8; a register accumulation per cell does not fit 27 nodes); the
 // window is then stored, with plain stores, into the tile's slot of the
 // per-tile window buffer, and the grid update sums the windows holding each
 // node in tile order.  Particles whose stencil leaves their tile's window
@@ -154,43 +154,6 @@ __device__ __forceinline__ void quad_channels(const QP2G<R> &a, R w, R gx, R gy,
     c[7] = a.T * m;
 }
 
-// Adds the live channels of c[8] (bit ch of live) to p[ch * QN] in LDS.
-// FP64 uses the LDS atomic add.  FP32 uses compare-and-swap: gfx950 runs
-// ds_add_f32 one lane at a time (0.33 lane-adds per clock per CU measured,
-// against 4.6 for the compare-and-swap loop and 7.8 for ds_add_f64).  Each
-// pass reads the eight words, then swaps in old + c and keeps the lanes
-// whose word changed in between; every add is one FP32 add of the current
-// sum, as the atomic add is.
-__device__ __forceinline__ void lds_add8(double *p, const double c[8], unsigned live) {
-#pragma unroll
-    for (int ch = 0; ch < 8; ++ch)
-        if (live & (1u << ch)) atomicAdd(p + ch * QN, c[ch]);
-}
-
-__device__ __forceinline__ void lds_add8(float *p, const float c[8], unsigned live) {
-    unsigned *q = reinterpret_cast<unsigned *>(p);
-    unsigned old[8];
-#pragma unroll
-    for (int ch = 0; ch < 8; ++ch)
-        old[ch] = (live & (1u << ch)) ? __hip_atomic_load(q + ch * QN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
-                                      : 0u;
-    while (live) {
-        unsigned again = 0;
-#pragma unroll
-        for (int ch = 0; ch < 8; ++ch) {
-            if (!(live & (1u << ch))) continue;
-            unsigned expect = old[ch];
-            const unsigned want = __float_as_uint(__uint_as_float(expect) + c[ch]);
-            if (!__hip_atomic_compare_exchange_strong(q + ch * QN, &expect, want, __ATOMIC_RELAXED,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                old[ch] = expect;
-                again |= 1u << ch;
-            }
-        }
-        live = again;
-    }
-}
-
 // Scatter into the LDS window (window index wb of the base node, channel
 // stride QN).
 template <typename R>
@@ -205,7 +168,7 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
                 R w, gx, gy, gz, dx, dy, dz, c[8];
                 quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
                 quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
-                lds_add8(s_acc + wb + qnode(i, j, k), c, live);
+                lds_add8(s_acc + wb + qnode(i, j, k), QN, c, live);
             }
 }
 
@@ -296,7 +259,7 @@ __device__ __forceinline__ void apic_g2p(const Particles<R> &P, int i, const G2P
 // One workgroup per tile: the APIC G2P and / or P2G of the tile's particles,
 // thread tid taking particles b0 + tid, b0 + tid + kBlock, ... (slot-major
 // order: consecutive particles sit in different cells).  The window goes to
-// tpart[t] (8 channels x QN nodes).  counters[0] counts particles whose
+// tpart[t] (QN node records of 8 channels).  counters[0] counts particles whose
 // stencil left the window, counters[2] is set when one was scattered into
 // the node accumulators.
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
@@ -387,8 +350,7 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
         }
         if (DO_P2G) {
             __syncthreads();  // every LDS atomic is in
-            R *dst = tpart + (int64_t)t * (8 * QN);
-            for (int w = tid; w < 8 * QN; w += kBlock) dst[w] = s_acc[w];
+            store_window<R, QN>(tpart + (int64_t)t * (8 * QN), s_acc, tid);
         }
         int cntv = my_oow, cnta = my_acc;
 #pragma unroll

@@ -17,6 +17,20 @@ namespace mpmw {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kRed = 9;  // AABB min xyz, -max xyz, max von Mises, max T, kinetic energy
+constexpr int kXcds = 8;  // accelerator dies of the MI355X, each with its own L2
+
+// The block's index in a launch read as kXcds contiguous ranges, one per
+// XCD.  Workgroups go to the XCDs round-robin (workgroup b to XCD b % 8), so
+// with blockIdx.x neighbouring blocks run on different dies and every cache
+// line two of them share is fetched into two L2s; with this index each die
+// works through one contiguous range of the launch and neighbours share its
+// L2.  A bijection of [0, gridDim.x): what a block computes is unchanged.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb / kXcds, r = nb % kXcds;
+    const int x = b % kXcds, pos = b / kXcds;
+    return x < r ? x * (q + 1) + pos : r * (q + 1) + (x - r) * q + pos;
+}
 
 // ------------------------------------------------------------ reductions --
 
@@ -185,9 +199,13 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
                 if (a >= nzs || b >= nys || d >= nxs) continue;
                 const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
                 if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
-                const gp<const R> src = tp.part + (int64_t)t * (8 * en) + (lz[a] * ey + ly[b]) * ex + lx[d];
-#pragma unroll
-                for (int ch = 0; ch < 8; ++ch) c[ch] += src[ch * en];
+                // node-major window records: 8 channels, 32 bytes per node
+                const gp<const vec4_t<R>> src =
+                    reinterpret_cast<const vec4_t<R> *>((const R *)tp.part + (int64_t)t * (8 * en)) +
+                    2 * ((lz[a] * ey + ly[b]) * ex + lx[d]);
+                const vec4_t<R> u = src[0], v = src[1];
+                c[0] += u.x; c[1] += u.y; c[2] += u.z; c[3] += u.w;
+                c[4] += v.x; c[5] += v.y; c[6] += v.z; c[7] += v.w;
             }
 }
 
@@ -344,7 +362,7 @@ __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Gr
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
                                                         Tool tool, R dt, int direct_T, double *force_part) {
-    grid_update_block(g, G, tp, M, tool, dt, direct_T, force_part, (int)blockIdx.x);
+    grid_update_block(g, G, tp, M, tool, dt, direct_T, force_part, xcd_block());
 }
 
 // mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, Jacobi form:
@@ -384,7 +402,7 @@ __device__ __forceinline__ void diffuse_block(const GridView<R> &g, const Grid<R
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R alpha, R dt) {
-    diffuse_block(g, G, alpha, dt, (int)blockIdx.x);
+    diffuse_block(g, G, alpha, dt, xcd_block());
 }
 
 // ------------------------------------------------------- batched launches --
@@ -420,9 +438,10 @@ template <typename R> struct GridWork {
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *__restrict__ items, int nitems) {
-    const int w = find_item(items, nitems, (int)blockIdx.x);
+    const int b = xcd_block();
+    const int w = find_item(items, nitems, b);
     const GridWork<R> &W = items[w];
-    grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, (int)blockIdx.x - W.block0);
+    grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, b - W.block0);
 }
 
 // The fixed-order sum of an item's contact-force partials and, when
@@ -447,7 +466,7 @@ __device__ __forceinline__ void sum_block(const SumWork &W) {
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_post_batch(const GridWork<R> *__restrict__ items, int nitems,
                                                        const SumWork *__restrict__ sums, int gblocks) {
-    const int b = (int)blockIdx.x;
+    const int b = xcd_block();
     if (b >= gblocks) {
         sum_block(sums[b - gblocks]);
         return;
