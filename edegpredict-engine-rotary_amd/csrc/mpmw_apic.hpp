@@ -360,6 +360,8 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
         }
         if (lane == 0 && cntv) atomicAdd(counters, cntv);
         if (lane == 0 && cnta) atomicOr(counters + 2, 1);
+    } else if (DO_P2G) {
+        zero_window<R, QN>(tpart + (int64_t)t * (8 * QN), tid);
     }
     if (stats) block_reduce_stats(st, part, t);
 }

@@ -18,6 +18,10 @@ constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kRed = 9;  // AABB min xyz, -max xyz, max von Mises, max T, kinetic energy
 constexpr int kXcds = 8;  // accelerator dies of the MI355X, each with its own L2
+// Nodes per thread of the diffusion pass (4: 28 -> 21 us at 2.65 M nodes;
+// the grid update measured fastest at one node per thread).
+constexpr int kDiffuseNodes = 4;
+constexpr int kDiffuseBlock = kBlock * kDiffuseNodes;
 
 // The block's index in a launch read as kXcds contiguous ranges, one per
 // XCD.  Workgroups go to the XCDs round-robin (workgroup b to XCD b % 8), so
@@ -87,7 +91,18 @@ __device__ __forceinline__ void finalize_block(const double *part, int nparts, i
     for (int c = 0; c < w; ++c) {
         const int mode = modes[c];
         double r = mode == 0 ? 1e300 : (mode == 1 ? -1e300 : 0.0);
-        for (int i = threadIdx.x; i < nparts; i += kBlock) {
+        // each thread combines parts tid, tid + kBlock, ... in that order;
+        // eight loads are issued before the first of them is combined (the
+        // partials come from every die, a memory round trip each)
+        int i = threadIdx.x;
+        for (; i + 7 * kBlock < nparts; i += 8 * kBlock) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)(i + u * kBlock) * w + c];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r = mode == 0 ? fmin(r, x[u]) : (mode == 1 ? fmax(r, x[u]) : r + x[u]);
+        }
+        for (; i < nparts; i += kBlock) {
             double x = part[(int64_t)i * w + c];
             r = mode == 0 ? fmin(r, x) : (mode == 1 ? fmax(r, x) : r + x);
         }
@@ -138,13 +153,13 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 // ---------------------------------------------------------- grid update ---
 
 // Where the grid update finds this step's P2G sums: in the tile windows,
-// part[(tile * 8 + channel) * WN + window node], written whole by every
-// non-empty tile, plus, when *acc_flag is set, the node accumulators (loose
+// part[(tile * WN + window node) * 8 + channel], written whole by every tile
+// (zeros when it has no particles), plus, when *acc_flag is set, the node
+// accumulators (loose
 // particles, and in the atomic mode particles scattered outside their tile's
 // window).
 template <typename R> struct TileParts {
     gp<const R> part;
-    gp<const int32_t> tile_start;   // tile t is empty when tile_start[t] == tile_start[t + 1]
     gp<const int32_t> acc_flag;
     TileGeom tg;
     int32_t quad;                // windows of the quadratic stencil (QX x QY x QZ)
@@ -197,8 +212,9 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
 #pragma unroll
             for (int d = 0; d < 2; ++d) {
                 if (a >= nzs || b >= nys || d >= nxs) continue;
+                // an empty tile's window holds zeros (the particle pass
+                // stores them), so no emptiness test waits on a load
                 const int t = (tz[a] * tp.tg.nty + ty[b]) * tp.tg.ntx + tx[d];
-                if (tp.tile_start[t] == tp.tile_start[t + 1]) continue;
                 // node-major window records: 8 channels, 32 bytes per node
                 const gp<const vec4_t<R>> src =
                     reinterpret_cast<const vec4_t<R> *>((const R *)tp.part + (int64_t)t * (8 * en)) +
@@ -211,18 +227,17 @@ __device__ __forceinline__ void gather_tile_sums(const TileParts<R> &tp, int i, 
 
 // mpmGridUpdateKernel, MPMWorkpieceSolver.cu:136-193 (point samples) and its
 // analytic-SDF form with tool velocity and Coulomb friction, for node
-// (i, j, k) = idx.  Takes the P2G sums from where `tp` says, zeroing the
-// accumulators it has read; writes node velocity, mass and T*m.  When
+// (i, j, k) = idx.  Takes the P2G sums c[] of the tile windows
+// (gather_tile_sums) and adds the accumulators when `tp` flags them,
+// zeroing what it has read; writes node velocity, mass and T*m.  When
 // `direct_T` is set (no diffusion pass follows) it also writes the gather
 // temperature.  Returns the node's contact-force contribution.
 template <typename R>
 __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> &G, const TileParts<R> &tp,
                                             const Material<R> &M, const Tool &tool, R dt, int direct_T, int64_t idx,
-                                            int i, int j, int k) {
+                                            int i, int j, int k, R c[8]) {
     double cf = 0.0;
     {
-        R c[8];
-        gather_tile_sums(tp, i, j, k, c);
         if (*tp.acc_flag) {
 #pragma unroll
             for (int ch = 0; ch < 8; ++ch) {
@@ -345,7 +360,9 @@ __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Gr
         const int row = idx / g.nx;
         const int i = idx - row * g.nx;
         const int k = row / g.ny, j = row - k * g.ny;
-        cf = grid_node(g, G, tp, M, tool, dt, direct_T, idx, i, j, k);
+        R c[8];
+        gather_tile_sums(tp, i, j, k, c);
+        cf = grid_node(g, G, tp, M, tool, dt, direct_T, idx, i, j, k, c);
     }
     // fixed-order block sum of the contact force
     __shared__ double sh[kBlock / kWave];
@@ -359,50 +376,54 @@ __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Gr
     }
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update(GridView<R> g, Grid<R> G, TileParts<R> tp, Material<R> M,
-                                                        Tool tool, R dt, int direct_T, double *force_part) {
-    grid_update_block(g, G, tp, M, tool, dt, direct_T, force_part, xcd_block());
-}
-
 // mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, Jacobi form:
 // reads the post-update field (gm, gtm), writes the gather temperature.
+// kDiffuseNodes nodes per thread (node blk * kDiffuseBlock + r * kBlock +
+// tid): every node's 14 loads are issued before any node's update.
 template <typename R>
 __device__ __forceinline__ void diffuse_block(const GridView<R> &g, const Grid<R> &G, R alpha, R dt, int blk) {
-    const int idx = blk * kBlock + (int)threadIdx.x;
-    if (idx >= g.nn) return;
-    const int t = idx / g.nx;
-    const int i = idx - t * g.nx;
-    const int k = t / g.ny, j = t - k * g.ny;
+    const int first = blk * kDiffuseBlock + (int)threadIdx.x;
     const int nxy = g.nx * g.ny;
-    // the six neighbours' (T m, m), read together with the node's own (a
-    // missing neighbour reads the node itself and is left out)
-    const int nb[6] = {i > 0 ? idx - 1 : idx, i < g.nx - 1 ? idx + 1 : idx,
-                       j > 0 ? idx - g.nx : idx, j < g.ny - 1 ? idx + g.nx : idx,
-                       k > 0 ? idx - nxy : idx, k < g.nz - 1 ? idx + nxy : idx};
-    const bool has[6] = {i > 0, i < g.nx - 1, j > 0, j < g.ny - 1, k > 0, k < g.nz - 1};
-    const R mass = G.gm[idx];
-    R tm = G.gtm[idx];
-    R ntm[6], nm[6];
+    R mass[kDiffuseNodes], tm[kDiffuseNodes], ntm[kDiffuseNodes][6], nm[kDiffuseNodes][6];
+    unsigned has[kDiffuseNodes];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) { ntm[q] = G.gtm[nb[q]]; nm[q] = G.gm[nb[q]]; }
-    if (!(mass > R(1e-18))) return;  // empty node keeps its NaN marker
-    if (mass >= R(1e-15)) {
-        const R T = vdiv(tm, mass);
-        const R inv_h2 = (R)(1.0 / (g.hd * g.hd));
-        R lap = R(0);
+    for (int r = 0; r < kDiffuseNodes; ++r) {
+        const int idx = first + r * kBlock;
+        has[r] = 0;
+        if (idx >= g.nn) continue;
+        const int t = idx / g.nx;
+        const int i = idx - t * g.nx;
+        const int k = t / g.ny, j = t - k * g.ny;
+        // the six neighbours' (T m, m), read together with the node's own (a
+        // missing neighbour reads the node itself and is left out)
+        const int nb[6] = {i > 0 ? idx - 1 : idx, i < g.nx - 1 ? idx + 1 : idx,
+                           j > 0 ? idx - g.nx : idx, j < g.ny - 1 ? idx + g.nx : idx,
+                           k > 0 ? idx - nxy : idx, k < g.nz - 1 ? idx + nxy : idx};
+        has[r] = (i > 0 ? 1u : 0u) | (i < g.nx - 1 ? 2u : 0u) | (j > 0 ? 4u : 0u) | (j < g.ny - 1 ? 8u : 0u) |
+                 (k > 0 ? 16u : 0u) | (k < g.nz - 1 ? 32u : 0u) | 64u;
+        mass[r] = G.gm[idx];
+        tm[r] = G.gtm[idx];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (has[q]) lap += (vdiv(ntm[q], rmax(nm[q], R(1e-30))) - T);
-        const R T_new = T + alpha * dt * lap * inv_h2;
-        tm = rmax(T_new, R(0)) * mass;
+        for (int q = 0; q < 6; ++q) { ntm[r][q] = G.gtm[nb[q]]; nm[r][q] = G.gm[nb[q]]; }
     }
-    G.node[idx].w = vdiv(tm, mass);
-}
-
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_diffuse(GridView<R> g, Grid<R> G, R alpha, R dt) {
-    diffuse_block(g, G, alpha, dt, xcd_block());
+    const R inv_h2 = (R)(1.0 / (g.hd * g.hd));
+#pragma unroll
+    for (int r = 0; r < kDiffuseNodes; ++r) {
+        const int idx = first + r * kBlock;
+        if (!(has[r] & 64u)) continue;
+        if (!(mass[r] > R(1e-18))) continue;  // empty node keeps its NaN marker
+        R t = tm[r];
+        if (mass[r] >= R(1e-15)) {
+            const R T = vdiv(t, mass[r]);
+            R lap = R(0);
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (has[r] & (1u << q)) lap += (vdiv(ntm[r][q], rmax(nm[r][q], R(1e-30))) - T);
+            const R T_new = T + alpha * dt * lap * inv_h2;
+            t = rmax(T_new, R(0)) * mass[r];
+        }
+        G.node[idx].w = vdiv(t, mass[r]);
+    }
 }
 
 // ------------------------------------------------------- batched launches --
@@ -424,6 +445,17 @@ template <typename Item> __device__ __forceinline__ int find_item(const Item *it
     return lo;
 }
 
+// The same over the diffusion blocks of grid work items (dblock0).
+template <typename Item> __device__ __forceinline__ int find_item_d(const Item *items, int nitems, int b) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].dblock0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <typename R> struct GridWork {
     GridView<R> g;
     Grid<R> G;
@@ -433,7 +465,8 @@ template <typename R> struct GridWork {
     R dt, alpha;
     int32_t direct_T, diffuse;
     double *force_part;
-    int32_t block0, blocks;
+    int32_t block0, blocks;    // grid-update blocks of the item
+    int32_t dblock0, dblocks;  // diffusion blocks of the item (post pass)
 };
 
 template <typename R>
@@ -460,20 +493,20 @@ __device__ __forceinline__ void sum_block(const SumWork &W) {
     if (W.counters && threadIdx.x == 0) { W.counters[1] = 0; W.counters[2] = 0; }
 }
 
-// The pass after the grid update, one launch: blocks [0, gblocks) diffuse
-// the members' grids (items with diffusion), blocks gblocks + r sum member
+// The pass after the grid update, one launch: blocks [0, dblocks) diffuse
+// the members' grids (items with diffusion), blocks dblocks + r sum member
 // r's contact force and reset its flags.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_post_batch(const GridWork<R> *__restrict__ items, int nitems,
-                                                       const SumWork *__restrict__ sums, int gblocks) {
+                                                       const SumWork *__restrict__ sums, int dblocks) {
     const int b = xcd_block();
-    if (b >= gblocks) {
-        sum_block(sums[b - gblocks]);
+    if (b >= dblocks) {
+        sum_block(sums[b - dblocks]);
         return;
     }
-    const int w = find_item(items, nitems, b);
+    const int w = find_item_d(items, nitems, b);
     const GridWork<R> &W = items[w];
-    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, b - W.block0);
+    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, b - W.dblock0);
 }
 
 // ------------------------------------------------------------ small ops ---

@@ -944,7 +944,6 @@ template <typename R> int step_begin(mpmw_solver *s, double dt, StepPlan &sp) {
 template <typename R> TileParts<R> tile_parts(const mpmw_solver *s) {
     TileParts<R> tp;
     tp.part = (const R *)s->tpart;
-    tp.tile_start = s->sort.tile_start;
     tp.acc_flag = s->sort.oow + 2;
     tp.tg = s->sort.geom;
     tp.quad = s->apic ? 1 : 0;
@@ -2188,7 +2187,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     std::vector<GridWork<R>> gw(nr);
     std::vector<SumWork> sw(nr);
     std::vector<TileWork<R>> tw[kTileVariants];
-    int gblocks = 0, tblocks[kTileVariants] = {};
+    int gblocks = 0, dblocks = 0, tblocks[kTileVariants] = {};
     for (int r = 0; r < nr; ++r) {
         mpmw_solver *s = members[run[r]];
         const StepPlan &sp = plans[run[r]];
@@ -2207,6 +2206,9 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         g.block0 = gblocks;
         g.blocks = blocks_for(s->nn);
         gblocks += g.blocks;
+        g.dblock0 = dblocks;
+        g.dblocks = blocks_for(s->nn, kDiffuseBlock);
+        dblocks += g.dblocks;
         sw[r].part = s->fpart;
         sw[r].nparts = g.blocks;
         sw[r].modes = s->d_modes + kRed;
@@ -2287,8 +2289,8 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         {
             // diffusion, contact-force sums and flag resets in one launch
             Timed t(one, MPMW_K_DIFFUSE);
-            hipLaunchKernelGGL(k_post_batch<R>, dim3(gblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
-                               (const SumWork *)(d + off_sw), gblocks);
+            hipLaunchKernelGGL(k_post_batch<R>, dim3(dblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
+                               (const SumWork *)(d + off_sw), dblocks);
         }
         {
             Timed t(one, MPMW_K_G2P_P2G);

@@ -567,6 +567,15 @@ template <typename R> struct TileWaves {
     static constexpr int value = sizeof(R) == 4 ? MPMW_TILE_WAVES_F32 : 1;
 };
 
+// Zeros as the window of a tile without particles: the grid update adds
+// every window that holds a node without testing the tile for particles.
+template <typename R, int N> __device__ __forceinline__ void zero_window(R *dst, int tid) {
+    vec4_t<R> *d = reinterpret_cast<vec4_t<R> *>(dst);
+    vec4_t<R> z;
+    z.x = z.y = z.z = z.w = R(0);
+    for (int w = tid; w < 2 * N; w += kBlock) d[w] = z;
+}
+
 // Stores a tile's LDS window (N nodes, channel stride N) into its slot of
 // the window buffer node-major: node w's eight channel sums are the 32-byte
 // record dst[8 w .. 8 w + 7], which the grid update reads with two vector
@@ -795,6 +804,8 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
         if (lane == 0 && cntv) atomicAdd(counters, cntv);
         if (DET && lane == 0 && cnte) atomicAdd(counters + 1, cnte);
         if (!DET && lane == 0 && cnta) atomicOr(counters + 2, 1);
+    } else if (DO_P2G) {
+        zero_window<R, WN>(tpart + (int64_t)t * (8 * WN), tid);
     }
     if (stats) block_reduce_stats(st, part, t);
 }
