@@ -694,7 +694,10 @@ void free_sort_state(mpmw_solver *s) {
 // Bin the particles into cell tiles of the current grid: stable radix sort
 // by tile-major cell key, slot-major order inside each tile, then one gather
 // of every particle field.  Only the order changes, never a value.
-template <typename R> int sort_particles(mpmw_solver *s) {
+// sort_launch enqueues the binning and the copy of the loose range to the
+// host; after the stream has been synchronised, sort_finish takes the range
+// and marks the binning valid.
+template <typename R> int sort_launch(mpmw_solver *s) {
     SortState &S = s->sort;
     const TileGeom tg = tile_geom(s);
     if ((int64_t)tg.ntiles * kTileCells >= INT32_MAX) return fail(MPMW_E_RANGE, "too many tiles for the binning");
@@ -749,19 +752,34 @@ template <typename R> int sort_particles(mpmw_solver *s) {
     // the loose range [cell_start[nkeys], cell_start[nkeys + 1]) for the host
     HIP_TRY(hipMemcpyAsync(S.h_oow + 2, S.cell_start + tg.nkeys, 2 * sizeof(int32_t), hipMemcpyDeviceToHost,
                            s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    S.geom = tg;
+    return MPMW_OK;
+}
+
+void sort_finish(mpmw_solver *s) {
+    SortState &S = s->sort;
     S.loose_begin = S.h_oow[2];
     S.loose_end = S.h_oow[3];
-    S.geom = tg;
     S.valid = true;
     S.origin[0] = s->minx; S.origin[1] = s->miny; S.origin[2] = s->minz;
     S.sorts++;
+}
+
+template <typename R> int sort_particles(mpmw_solver *s) {
+    int rc = sort_launch<R>(s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    sort_finish(s);
     return MPMW_OK;
 }
 
 // Recompute the AABB on the device, read it back, rebuild the grid and
-// re-bin the particles when the tiles no longer match it.
-template <typename R> int rebuild(mpmw_solver *s) {
+// re-bin the particles when the tiles no longer match it.  rebuild_launch
+// enqueues the reduction (unless the last pass left fresh statistics) and
+// the copies to the host; after the stream has been synchronised,
+// rebuild_finish rebuilds the grid and, when the binning must change,
+// enqueues it (sets `sorting`; sort_finish after the next synchronisation).
+template <typename R> int rebuild_launch(mpmw_solver *s) {
     int rc;
     if (!s->stats_fresh) {
         if ((rc = reduce_particles<R>(s))) return rc;
@@ -769,7 +787,13 @@ template <typename R> int rebuild(mpmw_solver *s) {
     }
     SortState &S = s->sort;
     if (S.oow) HIP_TRY(hipMemcpyAsync(S.h_oow, S.oow, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPMW_OK;
+}
+
+template <typename R> int rebuild_finish(mpmw_solver *s, bool &sorting) {
+    int rc;
+    sorting = false;
+    SortState &S = s->sort;
     if ((rc = rebuild_grid_from_stats(s))) return rc;
     bool need = !S.valid;
     if (S.valid) {
@@ -788,7 +812,10 @@ template <typename R> int rebuild(mpmw_solver *s) {
     }
     if (S.oow) hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow, (int64_t)1, 0);
     S.steps_since = 0;
-    if (need && (rc = sort_particles<R>(s))) return rc;
+    if (need) {
+        if ((rc = sort_launch<R>(s))) return rc;
+        sorting = true;
+    }
     return MPMW_OK;
 }
 
@@ -901,7 +928,7 @@ template <typename R> int det_settle(mpmw_solver *s, double dt, double zcut) {
 
 // Host plan of one step (MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:
 // 727-791), shared by the single-solver step and the batched step: what
-// step_begin prepared and what the launches and step_end need.
+// step_begin_many prepared and what the launches and step_end need.
 struct StepPlan {
     bool run = false;         // false: nothing to step (:728 silent return)
     bool diffusion = false;
@@ -909,37 +936,6 @@ struct StepPlan {
     double alpha = 0.0, zcut = 0.0;
     G2PFlags fl{};
 };
-
-// Everything before the grid update: the rebuild every 25 steps, binning,
-// the P2G when the sums are not ready (first step, after a rebuild or a
-// re-binning) and, in the deterministic mode, its check.
-template <typename R> int step_begin(mpmw_solver *s, double dt, StepPlan &sp) {
-    sp = StepPlan();
-    if (s->n <= 0 || s->nn <= 0) return MPMW_OK;  // :728 silent return
-    int rc;
-    if (s->step % 25 == 0) {
-        if ((rc = rebuild<R>(s))) return rc;
-    }
-    if (!s->sort.valid && (rc = sort_particles<R>(s))) return rc;
-    sp.zcut = s->tip[2] - s->h * 1.5;
-    if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
-    if (!s->p2g_ready) {
-        if ((rc = p2g_alone<R>(s, dt, sp.zcut))) return rc;
-        if (s->deterministic && (rc = det_settle<R>(s, dt, sp.zcut))) return rc;
-    }
-    // the binning may have changed in det_settle
-    if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
-    sp.diffusion = s->prm.specific_heat > 0.0 &&
-                   s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
-    sp.alpha = sp.diffusion ? s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) : 0.0;
-    // G2P + chip pass, fused with the next step's P2G unless the next step
-    // rebuilds the grid (then the AABB and metrics are reduced instead)
-    sp.next_rebuild = ((s->step + 1) % 25 == 0);
-    sp.fl.chip = (s->step % 2 == 0);
-    sp.fl.stats = sp.next_rebuild ? 1 : 0;
-    sp.run = true;
-    return MPMW_OK;
-}
 
 template <typename R> TileParts<R> tile_parts(const mpmw_solver *s) {
     TileParts<R> tp;
@@ -2129,14 +2125,20 @@ template <typename R> void launch_tile_batch(hipStream_t stream, int variant, co
     case 6: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, false>), grid, blk, 0, stream, items, n); break;
     case 7: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, true>), grid, blk, 0, stream, items, n); break;
     case 8: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, false>), grid, blk, 0, stream, items, n); break;
-    default: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, true>), grid, blk, 0, stream, items, n); break;
+    case 9: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, true>), grid, blk, 0, stream, items, n); break;
+    case 10: hipLaunchKernelGGL((k_tile_batch<R, false, true, false, false>), grid, blk, 0, stream, items, n); break;
+    case 11: hipLaunchKernelGGL((k_tile_batch<R, false, true, true, false>), grid, blk, 0, stream, items, n); break;
+    case 12: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, false>), grid, blk, 0, stream, items, n); break;
+    default: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, true>), grid, blk, 0, stream, items, n); break;
     }
 }
 
 // Tile-pass variant of a member: fused (0-3: backstress, deterministic) or
 // G2P-only before a rebuild (4-5: backstress); the APIC transfer's fused
-// (6-7) and G2P-only (8-9) passes.
-constexpr int kTileVariants = 10;
+// (6-7) and G2P-only (8-9) passes; the standalone P2G of the step's begin
+// (10-11 trilinear, 12-13 APIC: backstress).
+constexpr int kTileVariants = 14;
+int p2g_variant(const mpmw_solver *s) { return (s->apic ? 12 : 10) + (s->has_back ? 1 : 0); }
 int tile_variant(const mpmw_solver *s, const StepPlan &sp) {
     const int back = s->has_back ? 1 : 0;
     if (s->apic) return (sp.next_rebuild ? 8 : 6) + back;
@@ -2168,6 +2170,180 @@ void free_staging(StepStaging &st) {
     st.d_cap = 0;
 }
 
+// Stages `end` bytes, written by `fill` into one of the two pinned areas,
+// into the device blob on `stream` (kernels of the same stream read it);
+// returns the blob's device address in *dev.
+template <typename Fill> int stage_blob(StepStaging &st, hipStream_t stream, size_t end, Fill fill, char **dev) {
+    StepStaging::Area &A = st.area[st.parity];
+    st.parity ^= 1;
+    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
+    if (A.pending) {
+        HIP_TRY(hipEventSynchronize(A.copied));
+        A.pending = false;
+    }
+    if (end > A.cap) {
+        pool::put(A.h);
+        A.h = nullptr;
+        A.cap = 0;
+        int dev_id = 0;
+        HIP_TRY(hipGetDevice(&dev_id));
+        HIP_TRY(pool::get(dev_id, true, end * 2, &A.h));
+        A.cap = end * 2;
+    }
+    if (end > st.d_cap) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        pool::put(st.d_blob);
+        st.d_blob = nullptr;
+        st.d_cap = 0;
+        int dev_id = 0;
+        HIP_TRY(hipGetDevice(&dev_id));
+        HIP_TRY(pool::get(dev_id, false, end * 2, &st.d_blob));
+        st.d_cap = end * 2;
+    }
+    fill((char *)A.h);
+    HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(A.copied, stream));
+    A.pending = true;
+    *dev = (char *)st.d_blob;
+    return MPMW_OK;
+}
+
+// The tile-pass work item of a solver (blocks = its tiles; block0 is set by
+// the caller).
+template <typename R> TileWork<R> tile_work(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
+    TileWork<R> t;
+    t.P = pview<R>(s);
+    t.g = gview<R>(s);
+    t.G = grid_arrays<R>(s);
+    t.M = material<R>(s);
+    t.dt = (R)dt;
+    t.dtd = dt;
+    t.zcut = zcut;
+    t.ref_temp = s->prm.ref_temp;
+    t.flags = fl;
+    t.part = s->part;
+    t.cell_start = s->sort.cell_start;
+    t.tg = s->sort.geom;
+    t.counters = s->sort.oow;
+    t.tpart = (R *)s->tpart;
+    t.block0 = 0;
+    t.blocks = t.tg.ntiles;
+    return t;
+}
+
+// Everything before the grid update, for every member (MPMWorkpieceSolver
+// ::step :728-745): the rebuild every 25 steps, binning, and the P2G when
+// the sums are not ready (first step, after a rebuild or a re-binning).  The
+// host reads the members need (AABB, out-of-window counts, loose ranges)
+// come from two synchronisations for all of them, and their standalone P2G
+// passes are one launch, except for deterministic members, which check
+// theirs and may re-bin (det_settle) on their own.
+template <typename R> int step_begin_many(mpmw_solver *const *members, size_t nm, const double *dts,
+                                         hipStream_t stream, StepStaging &st, std::vector<StepPlan> &plans) {
+    int rc;
+    std::vector<char> live(nm, 0), sorting(nm, 0);
+    bool wait = false;
+    for (size_t m = 0; m < nm; ++m) {
+        mpmw_solver *s = members[m];
+        plans[m] = StepPlan();
+        if (s->n <= 0 || s->nn <= 0) continue;  // :728 silent return
+        live[m] = 1;
+        if (s->step % 25 == 0) {
+            if ((rc = rebuild_launch<R>(s))) return rc;
+            wait = true;
+        }
+    }
+    if (wait) HIP_TRY(hipStreamSynchronize(stream));
+    wait = false;
+    for (size_t m = 0; m < nm; ++m) {
+        if (!live[m]) continue;
+        mpmw_solver *s = members[m];
+        bool srt = false;
+        if (s->step % 25 == 0 && (rc = rebuild_finish<R>(s, srt))) return rc;
+        if (!srt && !s->sort.valid) {
+            if ((rc = sort_launch<R>(s))) return rc;
+            srt = true;
+        }
+        sorting[m] = srt ? 1 : 0;
+        wait = wait || srt;
+    }
+    if (wait) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (size_t m = 0; m < nm; ++m)
+            if (sorting[m]) sort_finish(members[m]);
+    }
+    std::vector<TileWork<R>> pw[kTileVariants];
+    std::vector<int> pm;
+    int pblocks[kTileVariants] = {};
+    for (size_t m = 0; m < nm; ++m) {
+        if (!live[m]) continue;
+        mpmw_solver *s = members[m];
+        StepPlan &sp = plans[m];
+        sp.zcut = s->tip[2] - s->h * 1.5;
+        if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
+        if (s->p2g_ready) continue;
+        if (s->deterministic) {
+            if ((rc = p2g_alone<R>(s, dts[m], sp.zcut))) return rc;
+            if ((rc = det_settle<R>(s, dts[m], sp.zcut))) return rc;
+            continue;
+        }
+        // P2G into empty sums: node accumulators and counters [1], [2] cleared
+        if ((rc = zero_acc<R>(s))) return rc;
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, stream, s->sort.oow + 1, (int64_t)2, 0);
+        const int v = p2g_variant(s);
+        TileWork<R> t = tile_work<R>(s, dts[m], sp.zcut, G2PFlags{});
+        t.block0 = pblocks[v];
+        pblocks[v] += t.blocks;
+        pw[v].push_back(t);
+        pm.push_back((int)m);
+    }
+    if (!pm.empty()) {
+        size_t off[kTileVariants], end = 0;
+        for (int v = 0; v < kTileVariants; ++v) {
+            off[v] = end;
+            end += align256(sizeof(TileWork<R>) * pw[v].size());
+        }
+        char *d = nullptr;
+        if ((rc = stage_blob(st, stream, end, [&](char *h) {
+                 for (int v = 0; v < kTileVariants; ++v)
+                     if (!pw[v].empty()) std::memcpy(h + off[v], pw[v].data(), sizeof(TileWork<R>) * pw[v].size());
+             }, &d)))
+            return rc;
+        {
+            Timed t(nm == 1 ? members[0] : nullptr, MPMW_K_P2G);
+            for (int v = 0; v < kTileVariants; ++v)
+                if (!pw[v].empty())
+                    launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off[v]), (int)pw[v].size(),
+                                         pblocks[v]);
+        }
+        HIP_TRY(hipGetLastError());
+        for (int m : pm) {
+            mpmw_solver *s = members[m];
+            rc = s->has_back ? loose_pass_t<R, false, true, true, false>(s, dts[m], plans[m].zcut, G2PFlags{})
+                             : loose_pass_t<R, false, true, false, false>(s, dts[m], plans[m].zcut, G2PFlags{});
+            if (rc) return rc;
+        }
+    }
+    for (size_t m = 0; m < nm; ++m) {
+        if (!live[m]) continue;
+        mpmw_solver *s = members[m];
+        StepPlan &sp = plans[m];
+        // the binning may have changed in det_settle
+        if ((rc = ensure_part(s, s->sort.geom.ntiles + loose_blocks(s)))) return rc;
+        sp.diffusion = s->prm.specific_heat > 0.0 &&
+                       s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) > 0.0;
+        sp.alpha = sp.diffusion ? s->prm.thermal_conductivity / (s->prm.density * s->prm.specific_heat) : 0.0;
+        // G2P + chip pass, fused with the next step's P2G unless the next
+        // step rebuilds the grid (then the AABB and metrics are reduced
+        // instead)
+        sp.next_rebuild = ((s->step + 1) % 25 == 0);
+        sp.fl.chip = (s->step % 2 == 0);
+        sp.fl.stats = sp.next_rebuild ? 1 : 0;
+        sp.run = true;
+    }
+    return MPMW_OK;
+}
+
 // One step of every solver in `members` (all of precision R, on `stream`):
 // the host phases of each solver's own step, and each kernel of the step
 // launched once for all of them over a flattened block list.
@@ -2175,8 +2351,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
                                    hipStream_t stream, StepStaging &st) {
     std::vector<StepPlan> plans(nm);
     int rc;
-    for (size_t m = 0; m < nm; ++m)
-        if ((rc = step_begin<R>(members[m], dts[m], plans[m]))) return rc;
+    if ((rc = step_begin_many<R>(members, nm, dts, stream, st, plans))) return rc;
     std::vector<int> run;
     for (size_t m = 0; m < nm; ++m)
         if (plans[m].run) run.push_back((int)m);
@@ -2215,23 +2390,8 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         sw[r].out = s->d_cf;
         sw[r].counters = sp.next_rebuild ? nullptr : s->sort.oow;
         const int v = tile_variant(s, sp);
-        TileWork<R> t;
-        t.P = pview<R>(s);
-        t.g = g.g;
-        t.G = g.G;
-        t.M = g.M;
-        t.dt = (R)dt;
-        t.dtd = dt;
-        t.zcut = sp.zcut;
-        t.ref_temp = s->prm.ref_temp;
-        t.flags = sp.fl;
-        t.part = s->part;
-        t.cell_start = s->sort.cell_start;
-        t.tg = s->sort.geom;
-        t.counters = s->sort.oow;
-        t.tpart = (R *)s->tpart;
+        TileWork<R> t = tile_work<R>(s, dt, sp.zcut, sp.fl);
         t.block0 = tblocks[v];
-        t.blocks = t.tg.ntiles;
         tblocks[v] += t.blocks;
         tw[v].push_back(t);
     }
@@ -2246,41 +2406,15 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
             off_tw[v] = end;
             end += align256(sizeof(TileWork<R>) * tw[v].size());
         }
-        StepStaging::Area &A = st.area[st.parity];
-        st.parity ^= 1;
-        if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
-        if (A.pending) {
-            HIP_TRY(hipEventSynchronize(A.copied));
-            A.pending = false;
-        }
-        if (end > A.cap) {
-            pool::put(A.h);
-            A.h = nullptr;
-            A.cap = 0;
-            int dev = 0;
-            HIP_TRY(hipGetDevice(&dev));
-            HIP_TRY(pool::get(dev, true, end * 2, &A.h));
-            A.cap = end * 2;
-        }
-        if (end > st.d_cap) {
-            HIP_TRY(hipStreamSynchronize(stream));
-            pool::put(st.d_blob);
-            st.d_blob = nullptr;
-            st.d_cap = 0;
-            int dev = 0;
-            HIP_TRY(hipGetDevice(&dev));
-            HIP_TRY(pool::get(dev, false, end * 2, &st.d_blob));
-            st.d_cap = end * 2;
-        }
-        char *h = (char *)A.h;
-        std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
-        std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
-        for (int v = 0; v < kTileVariants; ++v)
-            if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
-        HIP_TRY(hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream));
-        HIP_TRY(hipEventRecord(A.copied, stream));
-        A.pending = true;
-        char *d = (char *)st.d_blob;
+        char *d = nullptr;
+        if ((rc = stage_blob(st, stream, end, [&](char *h) {
+                 std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
+                 std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
+                 for (int v = 0; v < kTileVariants; ++v)
+                     if (!tw[v].empty())
+                         std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
+             }, &d)))
+            return rc;
         const GridWork<R> *d_gw = (const GridWork<R> *)d;
         {
             Timed t(one, MPMW_K_GRID);
