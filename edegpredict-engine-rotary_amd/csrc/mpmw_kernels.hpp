@@ -3,8 +3,8 @@
 // Kernel map (reference kernel -> here):
 //   clearMPMGridKernel          -> the grid update zeroes the accumulators it has read
 //   mpmP2GKernel                -> k_tile<.., P2G> (mpmw_tile.hpp)
-//   mpmGridUpdateKernel         -> k_grid_update
-//   mpmDiffuseTemperatureKernel -> k_diffuse (race-free Jacobi form)
+//   mpmGridUpdateKernel         -> k_grid_update_batch (grid_update_block)
+//   mpmDiffuseTemperatureKernel -> k_post_batch (diffuse_block, race-free Jacobi form)
 //   mpmG2PKernel + chipFractureAndSeparationKernel -> k_tile<.., G2P, ..>
 //   updateMetrics / rebuild AABB host loops -> block partials in k_tile / k_reduce
 #pragma once
@@ -155,9 +155,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce(Particles<R> P, double ref_te
 // Where the grid update finds this step's P2G sums: in the tile windows,
 // part[(tile * WN + window node) * 8 + channel], written whole by every tile
 // (zeros when it has no particles), plus, when *acc_flag is set, the node
-// accumulators (loose
-// particles, and in the atomic mode particles scattered outside their tile's
-// window).
+// accumulators (loose particles, and in the atomic mode particles scattered
+// outside their tile's window).
 template <typename R> struct TileParts {
     gp<const R> part;
     gp<const int32_t> acc_flag;
@@ -630,10 +629,23 @@ __global__ void k_nearest_finalize(Particles<R> P, const double *best_d2, const 
     }
 }
 
-// Gather every particle array through a permutation: dst[i] = src[perm[i]].
-template <typename T> __global__ void k_gather(T *dst, const T *src, const int32_t *perm, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+// The particle arrays of one permutation, dst[i] = src[perm[i]], gathered
+// by one launch: array blockIdx.y of the table, 4- or 8-byte elements.
+constexpr int kMaxGatherFields = 48;
+struct GatherTable {
+    void *dst[kMaxGatherFields];
+    const void *src[kMaxGatherFields];
+    int32_t wide[kMaxGatherFields];  // 1: 8-byte elements
+    int32_t nf;
+};
+
+__global__ void __launch_bounds__(kBlock) k_gather_fields(GatherTable t, const int32_t *perm, int n) {
+    const int f = blockIdx.y;
+    const int i = blockIdx.x * kBlock + (int)threadIdx.x;
+    if (f >= t.nf || i >= n) return;
+    const int j = perm[i];
+    if (t.wide[f]) static_cast<uint64_t *>(t.dst[f])[i] = static_cast<const uint64_t *>(t.src[f])[j];
+    else static_cast<uint32_t *>(t.dst[f])[i] = static_cast<const uint32_t *>(t.src[f])[j];
 }
 
 }  // namespace mpmw

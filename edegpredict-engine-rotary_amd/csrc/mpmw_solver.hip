@@ -1013,19 +1013,20 @@ template <typename R> int permute_particles(mpmw_solver *s, const int32_t *perm,
         dperm = (const int32_t *)tmp;
     }
     {
-        Timed t(s, MPMW_K_SORT);
+        // every live array in one launch
+        static_assert(NF <= kMaxGatherFields, "gather table too small");
+        GatherTable gt{};
         for (int k = 0; k < NF; ++k) {
             if (!field_live(s, k)) continue;
-            if (k >= kNumReal)
-                hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
-                                   (int32_t *)s->alt[k], (const int32_t *)s->f[k], dperm, count);
-            else if (is_pos(k))
-                hipLaunchKernelGGL(k_gather<double>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream,
-                                   (double *)s->alt[k], (const double *)s->f[k], dperm, count);
-            else
-                hipLaunchKernelGGL(k_gather<R>, dim3(blocks_for(count)), dim3(kBlock), 0, s->stream, (R *)s->alt[k],
-                                   (const R *)s->f[k], dperm, count);
+            gt.dst[gt.nf] = s->alt[k];
+            gt.src[gt.nf] = s->f[k];
+            gt.wide[gt.nf] = (k < kNumReal && (is_pos(k) || sizeof(R) == 8)) ? 1 : 0;
+            ++gt.nf;
         }
+        Timed t(s, MPMW_K_SORT);
+        if (gt.nf > 0 && count > 0)
+            hipLaunchKernelGGL(k_gather_fields, dim3(blocks_for(count), gt.nf), dim3(kBlock), 0, s->stream, gt, dperm,
+                               count);
     }
     hipError_t e = hipGetLastError();
     if (host_perm) {
