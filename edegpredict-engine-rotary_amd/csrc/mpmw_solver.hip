@@ -150,6 +150,7 @@ struct mpmw_solver {
 
     // timing
     bool timing = false;
+    bool timing_all = false;       // every kernel class, else the fused particle pass only
     struct Ev { hipEvent_t a, b; int cls; };
     std::vector<Ev> events;
     std::vector<hipEvent_t> ev_pool;
@@ -499,8 +500,10 @@ hipEvent_t take_event(mpmw_solver *s) {
         s->ev_pool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence (with one, every record wrote back
+    // and invalidated the caches, 30-60 us per step of the drill slab)
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
@@ -508,7 +511,8 @@ struct Timed {
     mpmw_solver *s;
     mpmw_solver::Ev ev{};
     bool on;
-    Timed(mpmw_solver *s_, int cls) : s(s_), on(s_ && s_->timing) {
+    Timed(mpmw_solver *s_, int cls)
+        : s(s_), on(s_ && s_->timing && (s_->timing_all || cls == MPMW_K_G2P_P2G)) {
         if (!on) return;
         ev.a = take_event(s);
         ev.b = take_event(s);
@@ -2028,10 +2032,21 @@ int mpmw_get_grid_snapshot(mpmw_solver *s, double *mass, double *mom, double *fo
     return MPMW_OK;
 }
 
-int mpmw_set_kernel_timing(mpmw_solver *s, int32_t on) {
+int mpmw_set_kernel_timing(mpmw_solver *s, int32_t mode) {
     CHECK_S(s);
+    if (mode != MPMW_TIMING_OFF && mode != MPMW_TIMING_ALL && mode != MPMW_TIMING_PARTICLE_PASS)
+        return fail(MPMW_E_INVAL, "set_kernel_timing: unknown mode");
     drain_events(s);
-    s->timing = on != 0;
+    s->timing = mode != MPMW_TIMING_OFF;
+    s->timing_all = mode == MPMW_TIMING_ALL;
+    // events for 256 timed steps (three pairs each) made here rather than
+    // one by one inside the steps, where creating them held up the host
+    constexpr size_t kPool = 256 * 6;
+    while (s->timing && s->ev_pool.size() < kPool) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) break;
+        s->ev_pool.push_back(e);
+    }
     for (int k = 0; k < 8; ++k) { s->t_ms[k] = 0; s->t_n[k] = 0; }
     return MPMW_OK;
 }
@@ -2177,7 +2192,8 @@ void free_staging(StepStaging &st) {
 template <typename Fill> int stage_blob(StepStaging &st, hipStream_t stream, size_t end, Fill fill, char **dev) {
     StepStaging::Area &A = st.area[st.parity];
     st.parity ^= 1;
-    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming));
+    // the event only tells the host that the copy has read the area
+    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming | hipEventDisableSystemFence));
     if (A.pending) {
         HIP_TRY(hipEventSynchronize(A.copied));
         A.pending = false;

@@ -17,6 +17,7 @@ FP32, FP64 = 0, 1
 TRANSFER_PIC, TRANSFER_APIC = 0, 1
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
 K_G2P_P2G, K_P2G, K_GRID, K_DIFFUSE, K_SORT, K_REDUCE, K_OTHER = range(7)
+TIMING_OFF, TIMING_ALL, TIMING_PARTICLE_PASS = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -403,7 +404,10 @@ class Solver:
         return {"mass": m, "mom": p, "force": f, "tm": tm}
 
     def set_kernel_timing(self, on):
-        _check(lib().mpmw_set_kernel_timing(self.h, 1 if on else 0), "set_kernel_timing")
+        """True / TIMING_ALL: every kernel class; TIMING_PARTICLE_PASS: the
+        fused particle pass only; False / TIMING_OFF: none."""
+        mode = TIMING_ALL if on is True else (TIMING_OFF if on is False else int(on))
+        _check(lib().mpmw_set_kernel_timing(self.h, mode), "set_kernel_timing")
 
     def kernel_times(self):
         t = KernelTimes()

@@ -256,7 +256,12 @@ int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);
 int mpmw_set_grid_snapshot(mpmw_solver *s, int32_t on);
 /* Copy the last snapshot to host arrays of grid-node length (NULL skips). */
 int mpmw_get_grid_snapshot(mpmw_solver *s, double *mass, double *mom, double *force, double *tm);
-int mpmw_set_kernel_timing(mpmw_solver *s, int32_t on);
+/* Per-kernel timing: MPMW_TIMING_OFF, MPMW_TIMING_ALL (every kernel class;
+ * each timed launch adds two event records to the stream) or
+ * MPMW_TIMING_PARTICLE_PASS (the fused particle pass only, the least
+ * disturbance of the step). */
+enum { MPMW_TIMING_OFF = 0, MPMW_TIMING_ALL = 1, MPMW_TIMING_PARTICLE_PASS = 2 };
+int mpmw_set_kernel_timing(mpmw_solver *s, int32_t mode);
 int mpmw_get_kernel_times(mpmw_solver *s, mpmw_kernel_times *t);
 /* The hipStream_t the solver launches on, as an opaque pointer. */
 int mpmw_get_stream(mpmw_solver *s, void **stream);
