@@ -2554,3 +2554,5 @@ int mpmw_batch_synchronize(mpmw_batch *b) {
 }
 
 }  // extern "C"
+
+#include "mpmw_export.hpp"

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmpmw_hip.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
+OK, E_INVAL, E_HIP, E_STATE, E_NOMEM, E_RANGE = 0, -1, -2, -3, -4, -5
 FP32, FP64 = 0, 1
 TRANSFER_PIC, TRANSFER_APIC = 0, 1
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
@@ -35,7 +36,7 @@ EXPORTS = (
     "mpmw_get_kernel_times", "mpmw_get_stream", "mpmw_device_bytes", "mpmw_get_diagnostics",
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
-    "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine",
+    "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
 )
 
 
@@ -135,6 +136,8 @@ def lib():
         "mpmw_set_transfer": (C.c_int, [vp, C.c_int32]),
         "mpmw_get_affine": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_set_affine": (C.c_int, [vp, _dp, C.c_int32]),
+        "mpmw_export_ndjson": (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "mpmw_export_vtk": (C.c_int, [vp, C.c_char_p]),
         "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
         "mpmw_step": (C.c_int, [vp, C.c_double]),
@@ -329,6 +332,21 @@ class Solver:
     def set_affine(self, C9):
         C9 = np.ascontiguousarray(C9, dtype=np.float64).reshape(-1, 9)
         _check(lib().mpmw_set_affine(self.h, _p(C9), len(C9)), "set_affine")
+
+    def export_ndjson(self, step):
+        """The NDJSON particle message of the current state (bytes, newline
+        included; b"" when every particle is INACTIVE)."""
+        n = C.c_size_t(0)
+        _check(lib().mpmw_export_ndjson(self.h, int(step), None, 0, C.byref(n)), "export_ndjson")
+        if n.value == 0:
+            return b""
+        buf = C.create_string_buffer(n.value)
+        _check(lib().mpmw_export_ndjson(self.h, int(step), buf, n.value, C.byref(n)), "export_ndjson")
+        return buf.raw[:n.value]
+
+    def export_vtk(self, path):
+        """Write the legacy VTK particle file of the current state."""
+        _check(lib().mpmw_export_vtk(self.h, os.fsencode(path)), "export_vtk")
 
     def set_tool_pose(self, pose):
         _check(lib().mpmw_set_tool_pose(self.h, C.byref(pose)), "set_tool_pose")

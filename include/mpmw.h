@@ -250,6 +250,23 @@ int mpmw_get_affine(mpmw_solver *s, double *C, int32_t n);
 /* Replace C of all n particles (n = particle count); APIC only. */
 int mpmw_set_affine(mpmw_solver *s, const double *C, int32_t n);
 
+/* --- result formats of the serving loop
+ * mpmw_export_ndjson replaces StdoutIPCExporter::exportParticles
+ * (StdoutIPCExporter.cpp:70-113, with getParticlesForExport
+ * MPMWorkpieceSolver.cu:870-894 as its input): one NDJSON line
+ *   {"count":N,"positions_b64":"..","step":S,"temps_b64":"..","type":"particles"}\n
+ * over the particles that are not INACTIVE, FP32 x y z and FP32 temperature
+ * as little-endian bytes in standard base64 (Base64.h:13-52), encoded on the
+ * device.  *len receives the line's length; with buf NULL nothing else is
+ * written, with cap < *len the call fails with MPMW_E_RANGE.  No particle
+ * exported: *len = 0 (the reference writes nothing).  The line is not NUL
+ * terminated.
+ * mpmw_export_vtk replaces VTKExporter::exportParticles(filename, particles)
+ * (VTKExporter.cpp:252-336): the legacy ASCII POLYDATA file of the particles
+ * that are not INACTIVE. */
+int mpmw_export_ndjson(mpmw_solver *s, int32_t step, char *buf, size_t cap, size_t *len);
+int mpmw_export_vtk(mpmw_solver *s, const char *path);
+
 /* --- inspection (parity / measurement) */
 int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);
 /* Keep a copy of each step's post-P2G grid (mass, momentum, force, T*m). */
