@@ -293,7 +293,8 @@ def main():
                                        args.precision, args.batch_steps)
     if args.serve_seconds > 0:
         sv = {"workload": f"Poisson stream of 20 requests/s for {args.serve_seconds:g} s, sizes and parameters of "
-                          "the batched distribution, 200-2000 steps each"}
+                          "the batched distribution, 200-2000 steps each; a request's result is the NDJSON "
+                          "particle message of its final state"}
         sv["served"] = serve.summary(serve.serve_batched(serve.request_stream(duration=args.serve_seconds),
                                                          args.precision))
         sv["serial"] = serve.summary(serve.serve_serial(serve.request_stream(duration=args.serve_seconds),

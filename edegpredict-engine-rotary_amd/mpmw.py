@@ -340,9 +340,10 @@ class Solver:
         _check(lib().mpmw_export_ndjson(self.h, int(step), None, 0, C.byref(n)), "export_ndjson")
         if n.value == 0:
             return b""
-        buf = C.create_string_buffer(n.value)
-        _check(lib().mpmw_export_ndjson(self.h, int(step), buf, n.value, C.byref(n)), "export_ndjson")
-        return buf.raw[:n.value]
+        buf = np.empty(n.value, dtype=np.uint8)  # filled by the call, no zeroing
+        _check(lib().mpmw_export_ndjson(self.h, int(step), buf.ctypes.data_as(C.c_char_p), n.value, C.byref(n)),
+               "export_ndjson")
+        return buf[:n.value].tobytes()
 
     def export_vtk(self, path):
         """Write the legacy VTK particle file of the current state."""

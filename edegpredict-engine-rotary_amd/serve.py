@@ -12,6 +12,9 @@ The reference solver steps one workpiece per call; this loop has no
 counterpart there.
 
 The serial baseline serves the same stream with one plain solver at a time.
+A request's result is the NDJSON particle message of its final state
+(StdoutIPCExporter::exportParticles' format, built on the device); its
+time-to-result ends when that message is on the host.
 
     python serve.py --duration 4 --rate 20
 prints one JSON line with p50/p99 time-to-result and sims/hour for both.
@@ -44,6 +47,7 @@ class Request:
         self.finished = None
         self.metrics = None
         self.particles = None
+        self.payload = None             # the result message (NDJSON particle line)
 
     @property
     def n(self):
@@ -87,8 +91,13 @@ def _new_solver(req, precision, deterministic):
     return s
 
 
-def _finish(req, now, keep):
-    req.finished = now
+def _finish(req, t0, keep, payload):
+    """Complete a request: its result message (the NDJSON particle line of
+    the final state, built on the device) and metrics; the time-to-result
+    ends when the message is on the host."""
+    if payload == "ndjson":
+        req.payload = req.solver.export_ndjson(req.steps)
+    req.finished = time.perf_counter() - t0
     req.metrics = req.solver.metrics()
     if keep:
         req.particles = req.solver.get_particles()
@@ -97,7 +106,7 @@ def _finish(req, now, keep):
 
 
 def serve_batched(requests, precision="fp32", deterministic=False, budget=64 << 30, max_slots=256,
-                  keep_results=False):
+                  keep_results=False, payload="ndjson"):
     """Serve the stream with continuous batching; returns the requests with
     admission and completion times (seconds after the start)."""
     pending = sorted(requests, key=lambda q: q.arrival)
@@ -131,17 +140,16 @@ def serve_batched(requests, precision="fp32", deterministic=False, budget=64 << 
         done = [q for q in active if q.done_steps >= q.steps]
         if done:
             b.synchronize()
-            now = time.perf_counter() - t0
             for q in done:
                 b.remove(q.solver)
-                _finish(q, now, keep_results)
+                _finish(q, t0, keep_results, payload)
                 used -= state_bytes(q, precision)
             active = [q for q in active if q.solver is not None]
     b.close()
     return requests
 
 
-def serve_serial(requests, precision="fp32", deterministic=False, keep_results=False):
+def serve_serial(requests, precision="fp32", deterministic=False, keep_results=False, payload="ndjson"):
     """Serve the stream one plain solver at a time, first come first served."""
     t0 = time.perf_counter()
     for q in sorted(requests, key=lambda r: r.arrival):
@@ -153,7 +161,7 @@ def serve_serial(requests, precision="fp32", deterministic=False, keep_results=F
         q.solver.run(q.steps, q.cfg["dt"])
         q.solver.synchronize()
         q.done_steps = q.steps
-        _finish(q, time.perf_counter() - t0, keep_results)
+        _finish(q, t0, keep_results, payload)
     return requests
 
 
@@ -179,14 +187,16 @@ def main():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--seed", type=int, default=workloads.SEED_SERVE)
     ap.add_argument("--no-serial", action="store_true")
+    ap.add_argument("--payload", choices=["ndjson", "none"], default="ndjson",
+                    help="result message of each request (NDJSON particle line)")
     args = ap.parse_args()
     out = {"workload": f"Poisson stream, {args.rate:g} requests/s for {args.duration:g} s, sizes of the batched "
                        f"distribution, 200-2000 steps each", "precision": args.precision}
     reqs = request_stream(args.seed, args.rate, args.duration)
-    out["served"] = summary(serve_batched(reqs, args.precision, args.deterministic))
+    out["served"] = summary(serve_batched(reqs, args.precision, args.deterministic, payload=args.payload))
     if not args.no_serial:
         reqs = request_stream(args.seed, args.rate, args.duration)
-        out["serial"] = summary(serve_serial(reqs, args.precision, args.deterministic))
+        out["serial"] = summary(serve_serial(reqs, args.precision, args.deterministic, payload=args.payload))
         out["sims_per_hour_ratio"] = out["served"]["sims_per_hour"] / out["serial"]["sims_per_hour"]
     print(json.dumps(out), flush=True)
 

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import mpmw
+import formats
 import serve
 import workloads
 
@@ -36,6 +37,11 @@ def test_served_results_equal_alone_bit_exact():
                 continue
             assert np.array_equal(np.asarray(q.particles[k]).view(np.uint8), np.asarray(v).view(np.uint8)), (
                 q.cfg["name"], k)
+        # the result message: the same NDJSON line served and alone, and the
+        # restatement's line of the final state
+        assert q.payload == r.payload and q.payload.startswith(b'{"count":')
+        p = q.particles
+        assert q.payload == formats.ndjson_particles(q.steps, p["pos"], p["temperature"], p["status"])
 
 
 def test_served_summary_and_budget():
