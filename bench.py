@@ -140,6 +140,19 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     s.synchronize()
     kms, kn = s.kernel_times()
     s.set_kernel_timing(mpmw.TIMING_OFF)
+    # the result message of the final state (NDJSON particle line, built on
+    # the device), timed as the C call into a preallocated host buffer
+    nb = C.c_size_t(0)
+    mpmw.lib().mpmw_export_ndjson(s.h, steps, None, 0, C.byref(nb))
+    buf = np.empty(max(nb.value, 1), dtype=np.uint8)
+    t_exp = []
+    for _ in range(3):
+        t = time.perf_counter()
+        rc = mpmw.lib().mpmw_export_ndjson(s.h, steps, buf.ctypes.data_as(C.c_char_p), nb.value, C.byref(nb))
+        t_exp.append(time.perf_counter() - t)
+        assert rc == 0, mpmw.lib().mpmw_last_error()
+    message = {"bytes": nb.value, "ms": min(t_exp) * 1e3,
+               "GBs_to_host": nb.value / min(t_exp) / 1e9}
     diag = s.diagnostics()
     g = s.grid_info()
     s.close()
@@ -147,7 +160,7 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     return {
         "n": n, "ms_total": ms_total, "wall": wall, "setup_s": setup_s, "diag": diag,
         "grid_nodes": g.nx * g.ny * g.nz,
-        "k_avg_ms": fused_ms / max(fused_n, 1), "k_launches": fused_n,
+        "k_avg_ms": fused_ms / max(fused_n, 1), "k_launches": fused_n, "message": message,
         "kernels": {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]},
     }
 
@@ -280,6 +293,7 @@ def main():
                            "frac": pps * b_alg / 1e9 / HBM_PEAK_GBS},
         },
         "kernels": r["kernels"],  # every class, from steps after the timed region
+        "result_message": dict(r["message"], format="NDJSON particle line of the final state"),
         "binning": r["diag"],
         "wall_s": r["wall"],
         "setup_s": r["setup_s"],
