@@ -68,6 +68,14 @@ inline int blocks_for(int64_t n, int b = kBlock) { return (int)std::max<int64_t>
 
 // Staging of the work items of one step (see step_many): two pinned host
 // areas used in turn, an event guarding each area's reuse, one device area.
+// The rest mesh of a voxel tool, kept for wear morphing.
+struct ToolMesh {
+    std::vector<double> verts, normals, vnormals;  // xyz per vertex, per triangle, per vertex
+    std::vector<int32_t> tris;
+    double padding = 0.0;
+    int32_t voxels = 0;
+};
+
 struct StepStaging {
     struct Area {
         void *h = nullptr;
@@ -94,6 +102,7 @@ struct mpmw_solver {
     int32_t alt_cap = 0;
     bool has_back = false, has_fext = false;
     bool apic = false;     // APIC transfer variant (affine fields F_C0..F_C8 live)
+    ToolMesh mesh;         // rest mesh of the voxel tool (wear morphing)
 
     // grid
     double minx = 0, miny = 0, minz = 0, h = 0;
@@ -1595,11 +1604,16 @@ int mpmw_set_tool_analytic(mpmw_solver *s, const mpmw_tool_desc *d) {
     return MPMW_OK;
 }
 
-int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double *verts, int32_t nverts,
-                           const int32_t *tris, const double *normals, int32_t ntris, double padding,
-                           int32_t voxels_per_axis) {
+}  // extern "C"
+
+namespace {
+
+// ToolSDF::build / rebuild (ToolSDF.cu:212-346) of a triangle mesh into the
+// solver's voxel SDF, on the device; the tool becomes the voxel tool `d`.
+int build_tool_sdf(mpmw_solver *s, const mpmw_tool_desc *d, const double *verts, int32_t nverts,
+                   const int32_t *tris, const double *normals, int32_t ntris, double padding,
+                   int32_t voxels_per_axis) {
 #pragma clang fp contract(off)
-    CHECK_S(s);
     if (!d || !verts || !tris || !normals) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: null argument");
     // build, ToolSDF.cu:219-222: an empty mesh builds nothing
     if (nverts <= 0 || ntris <= 0) return fail(MPMW_E_INVAL, "set_tool_sdf_mesh: empty mesh");
@@ -1725,6 +1739,66 @@ int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double
     s->desc.kind = MPMW_TOOL_VOXEL;
     s->tool_kind = MPMW_TOOL_VOXEL;
     return MPMW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double *verts, int32_t nverts,
+                           const int32_t *tris, const double *normals, int32_t ntris, double padding,
+                           int32_t voxels_per_axis) {
+#pragma clang fp contract(off)
+    CHECK_S(s);
+    int rc = build_tool_sdf(s, d, verts, nverts, tris, normals, ntris, padding, voxels_per_axis);
+    if (rc) return rc;
+    // the rest mesh and its vertex normals for wear morphing
+    // (ContactSolver::initSDF, ContactSolver.cu:1024-1044): each face normal
+    // normalised (when its length exceeds 1e-12) and added to its three
+    // vertices, each sum normalised the same way
+    ToolMesh &m = s->mesh;
+    m.verts.assign(verts, verts + 3 * (size_t)nverts);
+    m.tris.assign(tris, tris + 3 * (size_t)ntris);
+    m.normals.assign(normals, normals + 3 * (size_t)ntris);
+    m.vnormals.assign(3 * (size_t)nverts, 0.0);
+    for (int32_t t = 0; t < ntris; ++t) {
+        double n[3] = {normals[3 * t], normals[3 * t + 1], normals[3 * t + 2]};
+        const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len > 1e-12)
+            for (int k = 0; k < 3; ++k) n[k] /= len;
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) m.vnormals[3 * (size_t)tris[3 * t + j] + k] += n[k];
+    }
+    for (int32_t i = 0; i < nverts; ++i) {
+        double *v = &m.vnormals[3 * (size_t)i];
+        const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (len > 1e-12)
+            for (int k = 0; k < 3; ++k) v[k] /= len;
+    }
+    m.padding = padding;
+    m.voxels = voxels_per_axis;
+    return MPMW_OK;
+}
+
+int mpmw_morph_tool_sdf(mpmw_solver *s, const double *wear, int32_t nwear) {
+#pragma clang fp contract(off)
+    CHECK_S(s);
+    ToolMesh &m = s->mesh;
+    if (m.verts.empty()) return fail(MPMW_E_STATE, "morph_tool_sdf: no tool mesh was set");
+    if (nwear < 0 || (nwear > 0 && !wear)) return fail(MPMW_E_INVAL, "morph_tool_sdf: bad wear array");
+    if (!finite_array(wear, nwear)) return fail(MPMW_E_INVAL, "morph_tool_sdf: non-finite wear");
+    // ContactSolver::morphToolSDF (:1053-1083): worn = rest - normal * wear
+    // per vertex (vertex i takes wear[i], none past nwear), then the build
+    // with the rest mesh's triangles and face normals
+    const int32_t nv = (int32_t)(m.verts.size() / 3);
+    std::vector<double> worn(m.verts.size());
+    for (int32_t i = 0; i < nv; ++i) {
+        const double w = i < nwear ? wear[i] : 0.0;
+        for (int k = 0; k < 3; ++k) worn[3 * (size_t)i + k] = m.verts[3 * (size_t)i + k] - m.vnormals[3 * (size_t)i + k] * w;
+    }
+    const mpmw_tool_desc d = s->desc;
+    return build_tool_sdf(s, &d, worn.data(), nv, m.tris.data(), m.normals.data(), (int32_t)(m.tris.size() / 3),
+                          m.padding, m.voxels);
 }
 
 int mpmw_get_tool_sdf(mpmw_solver *s, float *grid, int32_t dims[3], double origin[3], double *voxel) {

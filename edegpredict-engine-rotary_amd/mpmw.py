@@ -37,6 +37,7 @@ EXPORTS = (
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
     "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
+    "mpmw_morph_tool_sdf",
 )
 
 
@@ -138,6 +139,7 @@ def lib():
         "mpmw_set_affine": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_export_ndjson": (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mpmw_export_vtk": (C.c_int, [vp, C.c_char_p]),
+        "mpmw_morph_tool_sdf": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
         "mpmw_step": (C.c_int, [vp, C.c_double]),
@@ -332,6 +334,12 @@ class Solver:
     def set_affine(self, C9):
         C9 = np.ascontiguousarray(C9, dtype=np.float64).reshape(-1, 9)
         _check(lib().mpmw_set_affine(self.h, _p(C9), len(C9)), "set_affine")
+
+    def morph_tool_sdf(self, wear):
+        """Move the voxel tool's mesh vertices inward by wear[i] along their
+        vertex normals and rebuild its SDF."""
+        w = np.ascontiguousarray(wear, dtype=np.float64).reshape(-1)
+        _check(lib().mpmw_morph_tool_sdf(self.h, _p(w), len(w)), "morph_tool_sdf")
 
     def export_ndjson(self, step):
         """The NDJSON particle message of the current state (bytes, newline

@@ -201,6 +201,15 @@ int mpmw_set_tool_sdf_mesh(mpmw_solver *s, const mpmw_tool_desc *d, const double
  * MPMW_E_STATE when no SDF was built.  No counterpart in the reference
  * (its grid lives only in the texture). */
 int mpmw_get_tool_sdf(mpmw_solver *s, float *grid, int32_t dims[3], double origin[3], double *voxel);
+/* Replaces ContactSolver::morphToolSDF (ContactSolver.cu:1053-1083): vertex i
+ * of the mesh last given to mpmw_set_tool_sdf_mesh moves inward by wear[i]
+ * along its vertex normal (the normalised sum of its normalised face
+ * normals, ContactSolver::initSDF :1024-1044; vertices past nwear do not
+ * move) and the SDF is rebuilt from the worn vertices with the mesh's
+ * triangles and face normals (ToolSDF::rebuild, ToolSDF.cu:282-346).  The
+ * reference takes the wear from its FEM tool's nodes; here the caller
+ * passes it.  MPMW_E_STATE when no mesh was set. */
+int mpmw_morph_tool_sdf(mpmw_solver *s, const double *wear, int32_t nwear);
 /* setToolTipPosition, MPMWorkpieceSolver.cuh:133-135 */
 int mpmw_set_tool_tip(mpmw_solver *s, double x, double y, double z);
 /* applyExternalForce, MPMWorkpieceSolver.cu:812-818 */

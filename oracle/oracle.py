@@ -394,6 +394,39 @@ def point_tri_d2(p, a, b, c):
     return float(d2), q
 
 
+def vertex_normals(nverts, tris, normals):
+    """ContactSolver::initSDF (src/ContactSolver.cu:1024-1044): each face
+    normal divided by its length when that exceeds 1e-12, added to its three
+    vertices; each vertex sum divided by its length the same way.  FP64, in
+    triangle order."""
+    vn = np.zeros((nverts, 3))
+    for t, tri in enumerate(np.asarray(tris)):
+        n = [float(c) for c in normals[t]]
+        ln = np.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
+        if ln > 1e-12:
+            n = [c / ln for c in n]
+        for j in tri:
+            for k in range(3):
+                vn[j, k] += n[k]
+    for i in range(nverts):
+        v = vn[i]
+        ln = np.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        if ln > 1e-12:
+            vn[i] = [c / ln for c in v]
+    return vn
+
+
+def worn_vertices(verts, tris, normals, wear):
+    """ContactSolver::morphToolSDF (src/ContactSolver.cu:1053-1075): vertex i
+    moved to rest - normal * wear[i] (no wear past the array's end)."""
+    v = np.asarray(verts, dtype=np.float64)
+    vn = vertex_normals(len(v), tris, normals)
+    w = np.zeros(len(v))
+    k = min(len(wear), len(v))
+    w[:k] = np.asarray(wear, dtype=np.float64)[:k]
+    return v - vn * w[:, None]
+
+
 class SdfGrid:
     """A voxel SDF built by the oracle (owns its grid)."""
 

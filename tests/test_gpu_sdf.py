@@ -8,6 +8,8 @@
   own SDF tests.
 * A wedge workpiece cut by the insert mesh through its voxel SDF matches the
   oracle to the parity tolerances (FP64 1e-9, FP32 1e-5), grid included.
+* Wear morphing rebuilds the SDF from the worn vertices bit for bit as the
+  oracle does.
 * Argument errors are reported, not run.
 """
 import numpy as np
@@ -101,6 +103,34 @@ def test_sdf_build_matches_oracle_bit_exact(name):
     assert origin == ref.origin and vox == ref.voxel
     assert (R < 0).any()
     assert np.array_equal(G.view(np.uint32), R.view(np.uint32))
+
+
+def test_worn_sdf_matches_oracle_bit_exact():
+    """Wear morphing (ContactSolver::morphToolSDF): the insert mesh's
+    vertices moved inward by per-vertex wear of up to 40 um (the last
+    vertices given none, past the wear array), the SDF rebuilt on the device,
+    equal bit for bit to the oracle's build of the oracle's worn vertices;
+    a second morph starts again from the rest mesh."""
+    V, T, N = workloads.prism_tool_mesh(17 * workloads.H + 2 * workloads.H)
+    s = mpmw.Solver(**dict(workloads.MATERIAL, fixture_thickness=0.0))
+    d = mpmw.ToolDesc()
+    for k, v in workloads.voxel_desc_values().items():
+        setattr(d, k, v)
+    with pytest.raises(mpmw.MPMError):      # no mesh yet
+        s.morph_tool_sdf(np.zeros(len(V)))
+    s.set_tool_sdf_mesh(d, V, T, N, 0.002, 128)
+    rest = s.get_tool_sdf()[0]
+    rng = np.random.default_rng(7)
+    for wear in (rng.uniform(0.0, 40e-6, len(V) - 2), rng.uniform(0.0, 10e-6, len(V))):
+        s.morph_tool_sdf(wear)
+        G, origin, vox = s.get_tool_sdf()
+        ref = oracle.SdfGrid(oracle.worn_vertices(V, T, N, wear), T, N, padding=0.002, voxels_per_axis=128)
+        R = ref.grid()
+        print(f"\n[worn sdf] dims={ref.dims} max wear={wear.max():.2e} changed vs rest="
+              f"{int((G.shape == rest.shape) and (G != rest).sum())}", flush=True)
+        assert origin == ref.origin and vox == ref.voxel
+        assert np.array_equal(G.view(np.uint32), R.view(np.uint32))
+    s.close()
 
 
 def voxel_case(steps):

@@ -167,3 +167,29 @@ def test_sampler(cube_sdf):
     assert s.in_range(o) and s.in_range(o + v * np.array([nx - 1, ny - 1, nz - 1]) - 1e-12)
     assert not s.in_range(o - np.array([1e-9, 0, 0]))
     assert not s.in_range(o + v * np.array([nx - 1, 0, 0]) + np.array([1e-9, 0, 0]))
+
+
+def test_vertex_normals_and_wear_known_answers():
+    """ContactSolver::initSDF's vertex normals: normalised face normals
+    summed per triangle a vertex belongs to, then normalised.  A cube of two
+    triangles per face: corner 0 lies in both triangles of its three faces,
+    (-2, -2, -2) / |.|; corner 1 in one triangle of the bottom and front
+    faces and both of the right face, (2, -1, -1) / sqrt(6).  A flat square
+    whose face normals are (0, 0, 2) has every vertex normal (0, 0, 1); wear
+    moves a vertex inward along it, vertices past the wear array stay."""
+    V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    quads = [((0, 3, 2, 1), (0, 0, -1)), ((4, 5, 6, 7), (0, 0, 1)), ((0, 1, 5, 4), (0, -1, 0)),
+             ((2, 3, 7, 6), (0, 1, 0)), ((1, 2, 6, 5), (1, 0, 0)), ((0, 4, 7, 3), (-1, 0, 0))]
+    T, N = [], []
+    for (i, j, k, l), n in quads:
+        T += [(i, j, k), (i, k, l)]
+        N += [n, n]
+    vn = oracle.vertex_normals(8, T, np.array(N, float))
+    assert np.allclose(vn[0], np.array([-1, -1, -1]) / np.sqrt(3), rtol=0, atol=1e-15)
+    assert np.allclose(vn[1], np.array([2, -1, -1]) / np.sqrt(6), rtol=0, atol=1e-15)
+    sq = np.array([[0, 0, 0], [2e-3, 0, 0], [2e-3, 2e-3, 0], [0, 2e-3, 0]], float)
+    tris = [(0, 1, 2), (0, 2, 3)]
+    nrm = np.array([[0, 0, 2.0], [0, 0, 2.0]])
+    assert np.array_equal(oracle.vertex_normals(4, tris, nrm), np.tile([0.0, 0.0, 1.0], (4, 1)))
+    worn = oracle.worn_vertices(sq, tris, nrm, [1e-4, 0.0, 2e-4])
+    assert np.array_equal(worn, sq - np.array([[0, 0, 1e-4], [0, 0, 0], [0, 0, 2e-4], [0, 0, 0]]))
