@@ -154,8 +154,62 @@ __device__ __forceinline__ void quad_channels(const QP2G<R> &a, R w, R gx, R gy,
     c[7] = a.T * m;
 }
 
-// Scatter into the LDS window (window index wb of the base node, channel
-// stride QN).
+// FP32 LDS window of the APIC pass: the 8 channels of a node as four 64-bit
+// words of channel pairs (0, 7) (1, 2) (3, 4) (5, 6), word q of node w at
+// index q * QN + w, so that a node takes four compare-and-swaps instead of
+// eight (ds_cmpst_rtn_b64 ran at 3.2 lane-ops per clock per CU against 4.6
+// for the 32-bit form).  A pair whose two channels are both skipped (no
+// momentum from FIXED_BOUNDARY, no force from BOUNDARY) is not touched; in
+// pair (3, 4) the skipped one of the two adds +0.
+__device__ __forceinline__ void lds_add_pairs(unsigned long long *p, const float c[8], bool momentum, bool ext) {
+    const float v[4][2] = {{c[0], c[7]},
+                           {c[1], c[2]},
+                           {momentum ? c[3] : 0.0f, ext ? c[4] : 0.0f},
+                           {c[5], c[6]}};
+    unsigned live = 1u | (momentum ? 2u : 0u) | ((momentum || ext) ? 4u : 0u) | (ext ? 8u : 0u);
+    unsigned long long old[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        old[q] = (live & (1u << q)) ? __hip_atomic_load(p + q * QN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                    : 0ull;
+    while (live) {
+        unsigned again = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!(live & (1u << q))) continue;
+            unsigned long long expect = old[q];
+            const float lo = __uint_as_float((unsigned)expect) + v[q][0];
+            const float hi = __uint_as_float((unsigned)(expect >> 32)) + v[q][1];
+            const unsigned long long want =
+                (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
+            if (!__hip_atomic_compare_exchange_strong(p + q * QN, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                old[q] = expect;
+                again |= 1u << q;
+            }
+        }
+        live = again;
+    }
+}
+
+// The FP32 pair window stored node-major (the layout store_window writes).
+__device__ __forceinline__ void store_window_pairs(float *dst, const float *s_acc, int tid) {
+    const unsigned long long *p = reinterpret_cast<const unsigned long long *>(s_acc);
+    float4 *d = reinterpret_cast<float4 *>(dst);
+    for (int w = tid; w < QN; w += kBlock) {
+        const unsigned long long p0 = p[w], p1 = p[QN + w], p2 = p[2 * QN + w], p3 = p[3 * QN + w];
+        float4 a, b;
+        a.x = __uint_as_float((unsigned)p0); b.w = __uint_as_float((unsigned)(p0 >> 32));
+        a.y = __uint_as_float((unsigned)p1); a.z = __uint_as_float((unsigned)(p1 >> 32));
+        a.w = __uint_as_float((unsigned)p2); b.x = __uint_as_float((unsigned)(p2 >> 32));
+        b.y = __uint_as_float((unsigned)p3); b.z = __uint_as_float((unsigned)(p3 >> 32));
+        d[2 * w] = a;
+        d[2 * w + 1] = b;
+    }
+}
+
+// Scatter into the LDS window (window index wb of the base node): FP32 as
+// channel pairs (lds_add_pairs), FP64 per channel with stride QN.
 template <typename R>
 __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFrac<R> &q, R h, const QP2G<R> &a) {
     const unsigned live = 0x81u | (a.momentum ? 0x0eu : 0u) | (a.ext ? 0x70u : 0u);
@@ -168,7 +222,11 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
                 R w, gx, gy, gz, dx, dy, dz, c[8];
                 quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
                 quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
-                lds_add8(s_acc + wb + qnode(i, j, k), QN, c, live);
+                if constexpr (sizeof(R) == 4)
+                    lds_add_pairs(reinterpret_cast<unsigned long long *>(s_acc) + wb + qnode(i, j, k), c, a.momentum,
+                                  a.ext);
+                else
+                    lds_add8(s_acc + wb + qnode(i, j, k), QN, c, live);
             }
 }
 
@@ -350,7 +408,8 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
         }
         if (DO_P2G) {
             __syncthreads();  // every LDS atomic is in
-            store_window<R, QN>(tpart + (int64_t)t * (8 * QN), s_acc, tid);
+            if constexpr (sizeof(R) == 4) store_window_pairs(tpart + (int64_t)t * (8 * QN), s_acc, tid);
+            else store_window<R, QN>(tpart + (int64_t)t * (8 * QN), s_acc, tid);
         }
         int cntv = my_oow, cnta = my_acc;
 #pragma unroll
