@@ -5,7 +5,7 @@
   slab over 60 steps (two grid rebuilds, so the G2P-only pass, re-binning
   and the P2G after a rebuild run), the analytic wedge of the batched
   distribution, and the same wedge workpiece inside a batch with trilinear
-  members.
+  members, and against the insert mesh's voxel SDF.
 * Known answer without the oracle: an affine velocity field v0 + A x with
   C = A gives C = 0.7425 A and v = v0 + A x after one step.
 * The deterministic mode is refused with APIC.
@@ -56,6 +56,16 @@ def test_apic_wedge_matches_oracle(prec, tol):
     cfg = dict(cfg, steps=30, transfer="apic")
     res = parity.run_case(cfg, prec)
     check(res, tol, f"wedge {cfg['name']} {prec}")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_apic_with_voxel_tool_matches_oracle(prec, tol):
+    """The APIC transfer against the insert mesh's voxel SDF."""
+    cfg = sorted(workloads.batch_config2(), key=lambda c: np.prod(c["cells"]))[0]
+    cfg = dict(workloads.voxel_tool_case(dict(cfg, steps=30)), transfer="apic")
+    res = parity.run_case(cfg, prec)
+    assert res["metrics"][1].contact_force > 0.0
+    check(res, tol, f"voxel {cfg['name']} {prec}")
 
 
 def test_apic_member_of_mixed_batch():

@@ -9,7 +9,8 @@
 * A wedge workpiece cut by the insert mesh through its voxel SDF matches the
   oracle to the parity tolerances (FP64 1e-9, FP32 1e-5), grid included.
 * Wear morphing rebuilds the SDF from the worn vertices bit for bit as the
-  oracle does.
+  oracle does, also in the middle of a cut, which then stays within the
+  parity tolerances.
 * Argument errors are reported, not run.
 """
 import numpy as np
@@ -130,6 +131,40 @@ def test_worn_sdf_matches_oracle_bit_exact():
               f"{int((G.shape == rest.shape) and (G != rest).sum())}", flush=True)
         assert origin == ref.origin and vox == ref.voxel
         assert np.array_equal(G.view(np.uint32), R.view(np.uint32))
+    s.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_wear_morph_mid_run_matches_oracle(prec, tol):
+    """A voxel-tool cut whose tool wears after 15 steps: the solver morphs
+    its SDF, the oracle is given the SDF of the same worn vertices, and both
+    take 15 more steps."""
+    cfg = voxel_case(30)
+    o, _ = parity.build_oracle(cfg)
+    s = parity.build_gpu(cfg, prec)
+    for _ in range(15):
+        o.step(cfg["dt"])
+        s.step(cfg["dt"])
+    V, T, N = cfg["tool"][3]
+    wear = np.random.default_rng(3).uniform(0.0, 30e-6, len(V))
+    s.morph_tool_sdf(wear)
+    desc = oracle.ToolDesc()
+    for k, v in cfg["tool"][1].items():
+        setattr(desc, k, v)
+    worn = oracle.SdfGrid(oracle.worn_vertices(V, T, N, wear), T, N, cfg["tool"][4], cfg["tool"][5])
+    o.set_tool_sdf(worn, desc)
+    G = s.get_tool_sdf()[0]
+    assert np.array_equal(G.view(np.uint32), worn.grid().view(np.uint32))
+    for _ in range(15):
+        o.step(cfg["dt"])
+        s.step(cfg["dt"])
+    errs, mism = parity.compare_particles(s.get_particles(), o.get_particles())
+    gm, om = s.metrics(), o.metrics()
+    print(f"\n[worn tool {prec}] cf gpu/oracle {gm.contact_force:.6g}/{om.contact_force:.6g} "
+          + " ".join(f"{k}={v:.2e}" for k, v in sorted(errs.items())), flush=True)
+    assert mism == 0 and om.contact_force > 0.0
+    for k in GATED:
+        assert errs[k] <= tol, (k, errs[k])
     s.close()
 
 
