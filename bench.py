@@ -39,6 +39,8 @@ B_NODE_STEP = 96.0
 # of which the fused G2P+P2G kernel moves: all particle bytes, the G2P node
 # read (16 B) and the P2G flush (32 B)
 B_NODE_G2P = 48.0
+# the APIC transfer's fused pass also writes the affine matrix C (9 x 4 B)
+B_PARTICLE_APIC = 136.0 + 36.0
 
 
 class HipEvents:
@@ -237,6 +239,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="seconds of Poisson arrivals (20/s) for the served run (0: skip it)")
+    ap.add_argument("--transfer", choices=["pic", "apic"], default="pic",
+                    help="apic: the APIC transfer variant (atomic mode only)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
@@ -251,6 +255,10 @@ def main():
     cfg = workloads.drill_config1()
     if cells != cfg["cells"]:
         cfg = workloads.scaled(cfg, cells)
+    if args.transfer == "apic":
+        # the deterministic mode is not available with the APIC transfer
+        cfg = dict(cfg, transfer="apic")
+        modes = [m for m in modes if m == "atomic"] or ["atomic"]
     runs = {m: run_mode(cfg, m, args.precision, args.steps, args.warmup) for m in modes}
     r = runs[modes[0]]
     n = r["n"]
@@ -260,9 +268,10 @@ def main():
 
     # dominant kernel: fused G2P + constitutive + chip + next-step P2G
     k_avg_ms = r["k_avg_ms"]
-    bytes_launch = B_PARTICLE * n + B_NODE_G2P * nodes
+    b_part = B_PARTICLE_APIC if args.transfer == "apic" else B_PARTICLE
+    bytes_launch = b_part * n + B_NODE_G2P * nodes
     achieved = bytes_launch / (k_avg_ms / 1e3) / 1e9
-    b_alg = B_PARTICLE + B_NODE_STEP * nodes / n
+    b_alg = b_part + B_NODE_STEP * nodes / n
 
     out = {
         "metric": "MPM particle-steps/s at 1 GPU",
@@ -280,12 +289,14 @@ def main():
         "config": {
             "workload": cfg["name"], "particles": n, "cells": list(cells),
             "grid_nodes": r["grid_nodes"], "occupied_nodes": nodes, "tool": "analytic two-flute drill",
-            "dt": cfg["dt"], "mode": modes[0], "precision": args.precision,
+            "dt": cfg["dt"], "mode": modes[0], "precision": args.precision, "transfer": args.transfer,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-            "kernel": "k_tile<G2P,P2G> (fused G2P + J-C return + chip pass + next-step P2G)",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic() if args.transfer == "pic" else None,
+            "kernel": ("k_tile<G2P,P2G> (fused G2P + J-C return + chip pass + next-step P2G)"
+                       if args.transfer == "pic" else
+                       "k_apic_tile<G2P,P2G> (APIC: 27-node gather, C update, J-C return, chip pass, next-step P2G)"),
             "algorithmic_bytes_per_launch": bytes_launch,
             "avg_launch_ms": k_avg_ms, "timed_launches": r["k_launches"],
             "step_level": {"B_alg_per_particle_step": b_alg,
