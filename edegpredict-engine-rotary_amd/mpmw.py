@@ -37,8 +37,35 @@ EXPORTS = (
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
     "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
-    "mpmw_morph_tool_sdf",
+    "mpmw_morph_tool_sdf", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
 )
+
+
+TOOLMESH_DRILL, TOOLMESH_BORING, TOOLMESH_REAMING, TOOLMESH_THREADING, TOOLMESH_MILLING = range(5)
+TOOLMESH_NODES, TOOLMESH_TRIS = 674, 1344
+
+
+class ToolMeshParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("boring_bar_length_mm", C.c_double),
+                ("boring_bar_diameter_mm", C.c_double), ("reamer_diameter_mm", C.c_double),
+                ("thread_major_diameter_mm", C.c_double), ("thread_form_angle_deg", C.c_double)]
+
+
+def parametric_tool_mesh(kind=TOOLMESH_DRILL, **over):
+    """The reference application's parametric tool mesh (verts, tris,
+    normals) of `kind`, with machining-parameter overrides (host only)."""
+    p = ToolMeshParams()
+    lib().mpmw_tool_mesh_params_default(C.byref(p))
+    p.kind = kind
+    for k, v in over.items():
+        setattr(p, k, v)
+    V = np.zeros((TOOLMESH_NODES, 3))
+    T = np.zeros((TOOLMESH_TRIS, 3), dtype=np.int32)
+    N = np.zeros((TOOLMESH_TRIS, 3))
+    nv, nt = C.c_int32(), C.c_int32()
+    _check(lib().mpmw_parametric_tool_mesh(C.byref(p), _p(V), _p(T, _ip), _p(N), C.byref(nv), C.byref(nt)),
+           "parametric_tool_mesh")
+    return V[:nv.value], T[:nt.value], N[:nt.value]
 
 
 class Params(C.Structure):
@@ -140,6 +167,8 @@ def lib():
         "mpmw_export_ndjson": (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mpmw_export_vtk": (C.c_int, [vp, C.c_char_p]),
         "mpmw_morph_tool_sdf": (C.c_int, [vp, _dp, C.c_int32]),
+        "mpmw_tool_mesh_params_default": (None, [C.POINTER(ToolMeshParams)]),
+        "mpmw_parametric_tool_mesh": (C.c_int, [C.POINTER(ToolMeshParams), _dp, _ip, _dp, _ip, _ip]),
         "mpmw_set_tool_tip": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
         "mpmw_apply_external_force": (C.c_int, [vp, _dp, C.c_double, _dp]),
         "mpmw_step": (C.c_int, [vp, C.c_double]),

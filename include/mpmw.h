@@ -259,6 +259,26 @@ int mpmw_get_affine(mpmw_solver *s, double *C, int32_t n);
 /* Replace C of all n particles (n = particle count); APIC only. */
 int mpmw_set_affine(mpmw_solver *s, const double *C, int32_t n);
 
+/* --- parametric tool mesh: generateParametricToolMesh (main.cpp:83-233),
+ * the reference application's tool surface when no mesh file is given, for
+ * mpmw_set_tool_sdf_mesh.  Drill (r 5 mm, flute 25 mm, 118 deg point cone),
+ * boring bar, reamer (flat bottom, 20 mm), thread mill (r 0.8 x major / 2,
+ * 15 mm, V tip of the form angle), end mill (r 5 mm, 25 mm, flat).  The
+ * caller's arrays hold MPMW_TOOLMESH_NODES nodes (xyz) and
+ * MPMW_TOOLMESH_TRIS triangles (3 indices, xyz normal).  Host only. */
+enum { MPMW_TOOLMESH_DRILL = 0, MPMW_TOOLMESH_BORING = 1, MPMW_TOOLMESH_REAMING = 2, MPMW_TOOLMESH_THREADING = 3,
+       MPMW_TOOLMESH_MILLING = 4 };
+enum { MPMW_TOOLMESH_NODES = 674, MPMW_TOOLMESH_TRIS = 1344 };
+typedef struct mpmw_tool_mesh_params {
+    int32_t kind, pad_;
+    double boring_bar_length_mm, boring_bar_diameter_mm;   /* config machining_parameters, defaults 40, 10 */
+    double reamer_diameter_mm;                             /* default 10 */
+    double thread_major_diameter_mm, thread_form_angle_deg; /* defaults 10, 60 */
+} mpmw_tool_mesh_params;
+void mpmw_tool_mesh_params_default(mpmw_tool_mesh_params *p);
+int mpmw_parametric_tool_mesh(const mpmw_tool_mesh_params *p, double *verts, int32_t *tris, double *normals,
+                              int32_t *nverts, int32_t *ntris);
+
 /* --- result formats of the serving loop
  * mpmw_export_ndjson replaces StdoutIPCExporter::exportParticles
  * (StdoutIPCExporter.cpp:70-113, with getParticlesForExport

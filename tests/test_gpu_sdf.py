@@ -3,7 +3,8 @@
 * The device build of the SDF (scatter of squared distances, band sign,
   host flood fill) equals the oracle's grid bit for bit, with the same
   dimensions, origin and voxel size, for the cutting-insert prism of the
-  voxel workload, a closed sphere mesh of many small triangles (many
+  voxel workload, the reference application's parametric drill mesh, a
+  closed sphere mesh of many small triangles (many
   triangles per voxel, ties between triangles) and the cube of the oracle's
   own SDF tests.
 * A wedge workpiece cut by the insert mesh through its voxel SDF matches the
@@ -87,6 +88,9 @@ MESHES = {
     "insert": (lambda: workloads.prism_tool_mesh(17 * workloads.H + 2 * workloads.H), 0.002, 128),
     "sphere": (sphere_mesh, 0.4e-3, 96),
     "cube": (lambda: cube_mesh((0.0, 0.0, 0.0), (4e-3, 3e-3, 2e-3)), 1e-3, 24),
+    # the reference application's parametric drill (main.cpp:83-233) at the
+    # SDF build's own padding and resolution (ContactSolver.cu:1019)
+    "parametric_drill": (lambda: mpmw.parametric_tool_mesh(mpmw.TOOLMESH_DRILL), 0.002, 128),
 }
 
 
