@@ -601,7 +601,7 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
     constexpr int kAccBytes = DO_P2G ? 8 * WN * (int)sizeof(R) : 0;
     __shared__ __align__(32) unsigned char smem[kNodeBytes + kAccBytes + 32];
     __shared__ int32_t s_cnt[2][kBlock / kWave];
-    __shared__ int32_t s_red[2][kBlock / kWave];
+    __shared__ int32_t s_red[3][kBlock / kWave];
     vec4_t<R> *s_node = reinterpret_cast<vec4_t<R> *>(smem);
     R *s_acc = DO_P2G ? reinterpret_cast<R *>(smem + kNodeBytes) : nullptr;
 
@@ -648,18 +648,28 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                 if (w < WN) s_node[w] = window_record<R>(wrec[r]);
             }
         }
-        // largest and smallest particle count of the tile's cells
-        int mx = cnt, mn = cnt;
+        // largest and smallest particle count of the tile's cells, and the
+        // smallest of its non-empty cells
+        int mx = cnt, mn = cnt, mnz = cnt > 0 ? cnt : INT32_MAX;
 #pragma unroll
         for (int o = kWave / 2; o > 0; o >>= 1) {
             mx = max(mx, __shfl_xor(mx, o, kWave));
             mn = min(mn, __shfl_xor(mn, o, kWave));
+            mnz = min(mnz, __shfl_xor(mnz, o, kWave));
         }
-        if (lane == 0) { s_red[0][wv] = mx; s_red[1][wv] = mn; }
+        if (lane == 0) { s_red[0][wv] = mx; s_red[1][wv] = mn; s_red[2][wv] = mnz; }
         __syncthreads();
-        int maxc = s_red[0][0], minc = s_red[1][0];
-        for (int w = 1; w < kBlock / kWave; ++w) { maxc = max(maxc, s_red[0][w]); minc = min(minc, s_red[1][w]); }
+        int maxc = s_red[0][0], minc = s_red[1][0], minnz = s_red[2][0];
+        for (int w = 1; w < kBlock / kWave; ++w) {
+            maxc = max(maxc, s_red[0][w]);
+            minc = min(minc, s_red[1][w]);
+            minnz = min(minnz, s_red[2][w]);
+        }
         const bool uniform = (maxc == minc);
+        // every non-empty cell holds maxc particles (a workpiece's boundary
+        // tiles): slot s of the cell of rank q among the non-empty cells is
+        // particle b0 + s * nnz + q, found once instead of per slot
+        const bool semi = !uniform && minnz == maxc;
 
         // home cell: its global cell index
         const int hgx = wx0 + hx, hgy = wy0 + hy, hgz = wz0 + hz;
@@ -674,12 +684,26 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
         int my_oow = 0, my_exc = 0, my_acc = 0;
         const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         int base = 0;
+        int nnz = 0, rank_nz = 0;
+        if (semi) {  // uniform over the workgroup
+            const unsigned long long m = __ballot(cnt > 0);
+            if (lane == 0) s_cnt[0][wv] = __popcll(m);
+            __syncthreads();
+            for (int w = 0; w < kBlock / kWave; ++w) {
+                const int c = s_cnt[0][w];
+                rank_nz += (w < wv) ? c : 0;
+                nnz += c;
+            }
+            rank_nz += __popcll(m & lt);
+        }
         // Particle index of this thread's cell in slot s: uniform tiles
-        // b0 + s * kBlock + tid; otherwise the slot-major position (cells
-        // with more than s particles, in key order), from a ballot and a
-        // prefix over the waves.
+        // b0 + s * kBlock + tid, tiles whose non-empty cells hold equal
+        // counts b0 + s * nnz + rank; otherwise the slot-major position
+        // (cells with more than s particles, in key order), from a ballot
+        // and a prefix over the waves.
         auto slot_index = [&](int sl) -> int {
             if (uniform) return b0 + sl * kBlock + tid;
+            if (semi) return b0 + sl * nnz + rank_nz;
             const unsigned long long mask = __ballot(cnt > sl);
             if (lane == 0) s_cnt[sl & 1][wv] = __popcll(mask);
             __syncthreads();
