@@ -556,7 +556,7 @@ __device__ __forceinline__ void combine_layer(const CellAcc<R> &A, R *s_acc, int
 // (deterministic P2G): a particle outside its binned cell is only counted in
 // counters[1]; the caller re-bins and repeats the P2G when that count is not
 // zero.  Every sum then has a fixed order: per cell in slot order, across a
-// wave by the fixed shuffle tree, across tiles in tile order (k_grid_update).
+// wave by the fixed shuffle tree, across tiles in tile order (k_grid_update_batch).
 // Waves per SIMD the tile pass is compiled for (register budget 512 / waves).
 // FP32: 2 (a budget of 3 waves spills to scratch and measured 18 % slower);
 // FP64 keeps the compiler's choice.
