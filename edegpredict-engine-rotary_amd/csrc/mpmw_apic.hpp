@@ -434,15 +434,6 @@ __global__ void __launch_bounds__(kBlock) k_apic_tile_batch(const TileWork<R> *_
                                                  (int)blockIdx.x - W.block0);
 }
 
-template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
-__global__ void __launch_bounds__(kBlock)
-    k_apic_tile(Particles<R> P, GridView<R> g, Grid<R> G, Material<R> M, R dt, double dtd, double zcut,
-                G2PFlags flags, double ref_temp, double *part, const int32_t *cell_start, TileGeom tg,
-                int32_t *counters, R *tpart) {
-    apic_tile_block<R, DO_G2P, DO_P2G, HAS_BACK>(P, g, G, M, dt, dtd, zcut, flags, ref_temp, part, cell_start, tg,
-                                                 counters, tpart, (int)blockIdx.x);
-}
-
 // The loose particles of the APIC pass, one thread each, nodes read from
 // and accumulated into device memory (sets acc_flag).  Statistics partials
 // go to part[blockIdx.x].

@@ -873,50 +873,33 @@ int loose_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
     return MPMW_OK;
 }
 
-template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
-int particle_pass_t(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
-    const Particles<R> P = pview<R>(s);
-    const GridView<R> g = gview<R>(s);
-    const Grid<R> G = grid_arrays<R>(s);
-    const Material<R> M = material<R>(s);
-    SortState &S = s->sort;
-    const TileGeom tg = S.geom;
-    // counters [1] (deterministic: particles off their cell) and [2] (node
-    // accumulators written) start at zero for every P2G pass
-    if (DO_P2G) hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, S.oow + 1, (int64_t)2, 0);
-    if (s->apic)
-        hipLaunchKernelGGL((k_apic_tile<R, DO_G2P, DO_P2G, HAS_BACK>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream,
-                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
-                           (R *)s->tpart);
-    else
-        hipLaunchKernelGGL((k_tile<R, DO_G2P, DO_P2G, HAS_BACK, DET>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream,
-                           P, g, G, M, (R)dt, dt, zcut, fl, s->prm.ref_temp, s->part, S.cell_start, tg, S.oow,
-                           (R *)s->tpart);
-    HIP_TRY(hipGetLastError());
-    return loose_pass_t<R, DO_G2P, DO_P2G, HAS_BACK, DET>(s, dt, zcut, fl);
-}
-
-template <typename R, bool DO_G2P, bool DO_P2G>
-int particle_pass(mpmw_solver *s, double dt, double zcut, G2PFlags fl) {
-    const bool back = s->has_back, det = s->deterministic;
-    if (back)
-        return det ? particle_pass_t<R, DO_G2P, DO_P2G, true, DO_P2G>(s, dt, zcut, fl)
-                   : particle_pass_t<R, DO_G2P, DO_P2G, true, false>(s, dt, zcut, fl);
-    return det ? particle_pass_t<R, DO_G2P, DO_P2G, false, DO_P2G>(s, dt, zcut, fl)
-               : particle_pass_t<R, DO_G2P, DO_P2G, false, false>(s, dt, zcut, fl);
-}
-
-// P2G alone into empty sums (first step, after a rebuild or a re-binning).
-// The node accumulators are cleared: a repeated P2G may follow a pass that
-// wrote them without a grid update in between.
-template <typename R> int p2g_alone(mpmw_solver *s, double dt, double zcut) {
-    int rc;
-    if ((rc = zero_acc<R>(s))) return rc;
+// The deterministic solver's standalone P2G pass (its re-binning check
+// repeats it): the tile pass, then the loose particles in a fixed order.
+// Counters [1] (particles off their cell) and [2] (node accumulators
+// written) start at zero.  The standalone P2G of a step's begin otherwise
+// runs batched (step_begin_many).
+template <typename R, bool HAS_BACK> int det_p2g_pass(mpmw_solver *s, double dt, double zcut) {
+    const TileGeom tg = s->sort.geom;
     G2PFlags fl;
     fl.chip = 0;
     fl.stats = 0;
+    hipLaunchKernelGGL(k_fill<int32_t>, dim3(1), dim3(64), 0, s->stream, s->sort.oow + 1, (int64_t)2, 0);
+    hipLaunchKernelGGL((k_tile<R, false, true, HAS_BACK, true>), dim3(tg.ntiles), dim3(kBlock), 0, s->stream,
+                       pview<R>(s), gview<R>(s), grid_arrays<R>(s), material<R>(s), (R)dt, dt, zcut, fl,
+                       s->prm.ref_temp, s->part, s->sort.cell_start, tg, s->sort.oow, (R *)s->tpart);
+    HIP_TRY(hipGetLastError());
+    return loose_pass_t<R, false, true, HAS_BACK, true>(s, dt, zcut, fl);
+}
+
+// P2G alone into empty sums for a deterministic solver (first step, after a
+// rebuild or a re-binning).  The node accumulators are cleared: a repeated
+// P2G may follow a pass that wrote them without a grid update in between.
+template <typename R> int p2g_alone(mpmw_solver *s, double dt, double zcut) {
+    if (!s->deterministic) return fail(MPMW_E_STATE, "p2g_alone: deterministic solvers only");
+    int rc;
+    if ((rc = zero_acc<R>(s))) return rc;
     Timed t(s, MPMW_K_P2G);
-    return particle_pass<R, false, true>(s, dt, zcut, fl);
+    return s->has_back ? det_p2g_pass<R, true>(s, dt, zcut) : det_p2g_pass<R, false>(s, dt, zcut);
 }
 
 // Deterministic mode: when a particle of the last P2G pass had left its
