@@ -134,24 +134,46 @@ template <typename R> struct QP2G {
     bool ext;        // stress or CHIP: external force (:121-132)
 };
 
-// The 8 channels of one node: mass, momentum, force, T * m.
+// A particle's P2G terms in the form the 27 nodes share: with node offset
+// x_i - x_p = (idx - d0) h, the affine velocity v + C (x_i - x_p) is
+// u + (h C) idx with u = v - (h C) d0, so a node adds (h C) idx to u with
+// its small integer idx (the sum over idx x and y is shared by the nodes of
+// a column); the stress force is S grad w with S = -V sigma.
+template <typename R> struct QScatter {
+    R m, T, u[3], hc[9], S[6], fe[3];
+};
+
 template <typename R>
-__device__ __forceinline__ void quad_channels(const QP2G<R> &a, R w, R gx, R gy, R gz, R dx, R dy, R dz, R c[8]) {
-    const R m = a.mass * w;
+__device__ __forceinline__ QScatter<R> quad_scatter_terms(const QP2G<R> &a, const QFrac<R> &q, R h) {
+    QScatter<R> z;
+    z.m = a.mass;
+    z.T = a.T;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) z.hc[c] = a.C[c] * h;
+    const R v[3] = {a.vx, a.vy, a.vz};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        z.u[r] = v[r] - (z.hc[3 * r] * q.d0[0] + z.hc[3 * r + 1] * q.d0[1] + z.hc[3 * r + 2] * q.d0[2]);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) z.S[c] = -a.vol * a.s[c];
+    z.fe[0] = a.fex; z.fe[1] = a.fey; z.fe[2] = a.fez;
+    return z;
+}
+
+// The 8 channels of node (i, j, k) of the stencil: mass, momentum, force,
+// T * m.
+template <typename R>
+__device__ __forceinline__ void quad_node_channels(const QScatter<R> &z, int i, int j, int k, R w, R gx, R gy, R gz,
+                                                   R c[8]) {
+    const R m = z.m * w;
     c[0] = m;
-    const R ax = a.C[0] * dx + a.C[1] * dy + a.C[2] * dz;
-    const R ay = a.C[3] * dx + a.C[4] * dy + a.C[5] * dz;
-    const R az = a.C[6] * dx + a.C[7] * dy + a.C[8] * dz;
-    c[1] = m * (a.vx + ax);
-    c[2] = m * (a.vy + ay);
-    c[3] = m * (a.vz + az);
-    const R fX = -(a.s[0] * gx + a.s[3] * gy + a.s[4] * gz) * a.vol;
-    const R fY = -(a.s[3] * gx + a.s[1] * gy + a.s[5] * gz) * a.vol;
-    const R fZ = -(a.s[4] * gx + a.s[5] * gy + a.s[2] * gz) * a.vol;
-    c[4] = fX + a.fex * w;
-    c[5] = fY + a.fey * w;
-    c[6] = fZ + a.fez * w;
-    c[7] = a.T * m;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        c[1 + r] = m * (z.u[r] + z.hc[3 * r] * R(i) + z.hc[3 * r + 1] * R(j) + z.hc[3 * r + 2] * R(k));
+    c[4] = z.S[0] * gx + z.S[3] * gy + z.S[4] * gz + z.fe[0] * w;
+    c[5] = z.S[3] * gx + z.S[1] * gy + z.S[5] * gz + z.fe[1] * w;
+    c[6] = z.S[4] * gx + z.S[5] * gy + z.S[2] * gz + z.fe[2] * w;
+    c[7] = z.T * m;
 }
 
 // FP32 LDS window of the APIC pass: the 8 channels of a node as four 64-bit
@@ -213,6 +235,7 @@ __device__ __forceinline__ void store_window_pairs(float *dst, const float *s_ac
 template <typename R>
 __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFrac<R> &q, R h, const QP2G<R> &a) {
     const unsigned live = 0x81u | (a.momentum ? 0x0eu : 0u) | (a.ext ? 0x70u : 0u);
+    const QScatter<R> z = quad_scatter_terms(a, q, h);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -221,7 +244,7 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
             for (int k = 0; k < 3; ++k) {
                 R w, gx, gy, gz, dx, dy, dz, c[8];
                 quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
-                quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
+                quad_node_channels(z, i, j, k, w, gx, gy, gz, c);
                 if constexpr (sizeof(R) == 4)
                     lds_add_pairs(reinterpret_cast<unsigned long long *>(s_acc) + wb + qnode(i, j, k), c, a.momentum,
                                   a.ext);
@@ -235,6 +258,7 @@ __device__ __forceinline__ void quad_scatter_window(R *s_acc, int wb, const QFra
 template <typename R>
 __device__ __forceinline__ void quad_scatter_global(const GridView<R> &g, gp<R> acc, const QFrac<R> &q, R h,
                                                     const QP2G<R> &a) {
+    const QScatter<R> zt = quad_scatter_terms(a, q, h);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -245,7 +269,7 @@ __device__ __forceinline__ void quad_scatter_global(const GridView<R> &g, gp<R> 
                 if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
                 R w, gx, gy, gz, dx, dy, dz, c[8];
                 quad_node(q, i, j, k, h, w, gx, gy, gz, dx, dy, dz);
-                quad_channels(a, w, gx, gy, gz, dx, dy, dz, c);
+                quad_node_channels(zt, i, j, k, w, gx, gy, gz, c);
                 R *p = acc + ((int64_t)z * g.ny + y) * g.nx + x;
                 unsafeAtomicAdd(p, c[0]);
                 if (a.momentum) {
