@@ -325,6 +325,15 @@ def main():
         sv["serial"] = serve.summary(serve.serve_serial(serve.request_stream(duration=args.serve_seconds),
                                                         args.precision))
         sv["sims_per_hour_ratio"] = sv["served"]["sims_per_hour"] / sv["serial"]["sims_per_hour"]
+        # At 20 requests/s the server keeps up, so its sims/hour is the arrival
+        # rate; the same requests queued at once give each server's capacity.
+        sat = {"workload": "the same requests, all queued at time 0"}
+        sat["served"] = serve.summary(serve.serve_batched(
+            serve.request_stream(duration=args.serve_seconds, backlog=True), args.precision))
+        sat["serial"] = serve.summary(serve.serve_serial(
+            serve.request_stream(duration=args.serve_seconds, backlog=True), args.precision))
+        sat["sims_per_hour_ratio"] = sat["served"]["sims_per_hour"] / sat["serial"]["sims_per_hour"]
+        sv["saturated"] = sat
         out["served"] = sv
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

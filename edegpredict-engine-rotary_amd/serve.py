@@ -55,8 +55,11 @@ class Request:
         return cx * cy * cz * 8
 
 
-def request_stream(seed=workloads.SEED_SERVE, rate=20.0, duration=4.0, step_lo=200, step_hi=2000):
-    """Poisson arrivals at `rate` per second over `duration` seconds."""
+def request_stream(seed=workloads.SEED_SERVE, rate=20.0, duration=4.0, step_lo=200, step_hi=2000,
+                   backlog=False):
+    """Poisson arrivals at `rate` per second over `duration` seconds.  With
+    `backlog` the same requests are all queued at time 0: a server's
+    makespan then measures its capacity rather than the arrival rate."""
     rng = np.random.default_rng(seed)
     out, t, i = [], 0.0, 0
     while True:
@@ -64,7 +67,7 @@ def request_stream(seed=workloads.SEED_SERVE, rate=20.0, duration=4.0, step_lo=2
         if t >= duration:
             break
         steps = int(rng.integers(step_lo, step_hi + 1))
-        out.append(Request(i, t, workloads.sample_workpiece(rng, i, seed, steps)))
+        out.append(Request(i, 0.0 if backlog else t, workloads.sample_workpiece(rng, i, seed, steps)))
         i += 1
     return out
 
@@ -187,15 +190,17 @@ def main():
     ap.add_argument("--deterministic", action="store_true")
     ap.add_argument("--seed", type=int, default=workloads.SEED_SERVE)
     ap.add_argument("--no-serial", action="store_true")
+    ap.add_argument("--backlog", action="store_true", help="queue every request at time 0")
     ap.add_argument("--payload", choices=["ndjson", "none"], default="ndjson",
                     help="result message of each request (NDJSON particle line)")
     args = ap.parse_args()
     out = {"workload": f"Poisson stream, {args.rate:g} requests/s for {args.duration:g} s, sizes of the batched "
-                       f"distribution, 200-2000 steps each", "precision": args.precision}
-    reqs = request_stream(args.seed, args.rate, args.duration)
+                       f"distribution, 200-2000 steps each" + (", all queued at time 0" if args.backlog else ""),
+           "precision": args.precision}
+    reqs = request_stream(args.seed, args.rate, args.duration, backlog=args.backlog)
     out["served"] = summary(serve_batched(reqs, args.precision, args.deterministic, payload=args.payload))
     if not args.no_serial:
-        reqs = request_stream(args.seed, args.rate, args.duration)
+        reqs = request_stream(args.seed, args.rate, args.duration, backlog=args.backlog)
         out["serial"] = summary(serve_serial(reqs, args.precision, args.deterministic, payload=args.payload))
         out["sims_per_hour_ratio"] = out["served"]["sims_per_hour"] / out["serial"]["sims_per_hour"]
     print(json.dumps(out), flush=True)

@@ -72,3 +72,13 @@ def test_workload_tool_geometry():
     assert np.abs((pts - [0.0, 0.0, 3.5e-3]) @ n).max() < 1e-15
     c1 = workloads.drill_config1()
     assert np.prod(c1["cells"]) * 8 == 16_777_216
+
+
+def test_backlog_stream_is_the_poisson_stream_queued_at_once():
+    import serve
+    live = serve.request_stream(duration=2.0)
+    queued = serve.request_stream(duration=2.0, backlog=True)
+    assert len(live) == len(queued) > 20
+    assert all(0.0 < a.arrival < 2.0 for a in live) and all(q.arrival == 0.0 for q in queued)
+    assert [a.steps for a in live] == [q.steps for q in queued]
+    assert [a.cfg["cells"] for a in live] == [q.cfg["cells"] for q in queued]
