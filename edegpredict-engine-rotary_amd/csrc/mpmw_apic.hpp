@@ -314,13 +314,14 @@ __device__ __forceinline__ QP2G<R> quad_inputs(const PState<R> &p, const R C[9],
 }
 
 // G2P of one particle with the quadratic gather, then its affine matrix:
-// C = B * 3/h^2, C *= 0.99 (g2pKernel :577-591) unless the particle was CHIP
+// C = B * 3/h^2, C *= 0.99 (g2pKernel :577-591; cfac and decay, which the
+// textbook form sets to 4/h^2 and 1) unless the particle was CHIP
 // (:570-573) or FIXED_BOUNDARY (:516-519) at entry.  C is left in C[] for a
 // following P2G.
 template <typename R, bool HAS_BACK, typename GatherFn>
 __device__ __forceinline__ void apic_g2p(const Particles<R> &P, int i, const G2PIn<R> &in, PState<R> &p,
                                          const Material<R> &M, R dt, double dtd, bool chip_step, bool need_stress,
-                                         double zcut, R cfac, R C[9], GatherFn gather_fn) {
+                                         double zcut, R cfac, R decay, R C[9], GatherFn gather_fn) {
     R B[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) B[c] = R(0);
@@ -330,7 +331,7 @@ __device__ __forceinline__ void apic_g2p(const Particles<R> &P, int i, const G2P
 #pragma unroll
         for (int c = 0; c < 9; ++c) {
             C[c] = B[c] * cfac;
-            C[c] = C[c] * R(0.99);
+            C[c] = C[c] * decay;
             P.aff[c][i] = C[c];
         }
     } else {
@@ -363,7 +364,7 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
     const int b0 = cell_start[t * kTileCells];
     const int b1 = cell_start[(t + 1) * kTileCells];
     const R h = (R)g.hd;
-    const R cfac = (R)(3.0 * g.inv_hd * g.inv_hd);
+    const R cfac = (R)(g.apic_k * g.inv_hd * g.inv_hd), decay = (R)g.apic_decay;
     double st[kRed];
     stats_identity(st, ref_temp);
     if (b0 < b1) {  // uniform over the workgroup
@@ -400,7 +401,7 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
                                  (unsigned)lz <= (unsigned)TZ;
                 const int wb = qnode(lx, ly, lz);
                 my_oow += inw ? 0 : 1;
-                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, stats, zcut, cfac, C,
+                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, stats, zcut, cfac, decay, C,
                                       [&](GatherSums<R> &s, R B[9]) {
                                           if (inw) quad_gather_window(s_node, wb, q, h, s, B);
                                           else quad_gather_global(g, G.node, q, h, s, B);
@@ -469,7 +470,7 @@ __global__ void __launch_bounds__(kBlock)
     stats_identity(st, ref_temp);
     const int i = begin + (int)blockIdx.x * kBlock + (int)threadIdx.x;
     const R h = (R)g.hd;
-    const R cfac = (R)(3.0 * g.inv_hd * g.inv_hd);
+    const R cfac = (R)(g.apic_k * g.inv_hd * g.inv_hd), decay = (R)g.apic_decay;
     if (i < end) {
         PState<R> p;
         p.status = P.status[i];
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(kBlock)
                 G2PIn<R> in;
                 load_g2p_in(P, i, in);
                 const QFrac<R> q = quad_frac(g, in.x, in.y, in.z);
-                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut, cfac, C,
+                apic_g2p<R, HAS_BACK>(P, i, in, p, M, dt, dtd, flags.chip != 0, flags.stats != 0, zcut, cfac, decay, C,
                                       [&](GatherSums<R> &s, R B[9]) { quad_gather_global(g, G.node, q, h, s, B); });
                 if (flags.stats) {
                     const double vx = p.vx, vy = p.vy, vz = p.vz;

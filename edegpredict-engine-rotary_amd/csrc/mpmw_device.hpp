@@ -82,6 +82,9 @@ template <typename R> struct GridView {
     R inv_h;
     int32_t nx, ny, nz, kfix;
     int64_t nn;
+    // APIC: C = B * apic_k / h^2, then C *= apic_decay (the reference's 3 and
+    // 0.99; the textbook quadratic B-spline's 4 and 1)
+    double apic_k, apic_decay;
 };
 
 // Particle state, structure of arrays.  a0 == nullptr means all backstress is

@@ -102,6 +102,7 @@ struct mpmw_solver {
     int32_t alt_cap = 0;
     bool has_back = false, has_fext = false;
     bool apic = false;     // APIC transfer variant (affine fields F_C0..F_C8 live)
+    bool apic_textbook = false;  // APIC with 4/h^2 and no decay instead of 3/h^2 and 0.99
     ToolMesh mesh;         // rest mesh of the voxel tool (wear morphing)
 
     // grid
@@ -419,6 +420,8 @@ template <typename R> GridView<R> gview(const mpmw_solver *s) {
     g.nx = s->nx; g.ny = s->ny; g.nz = s->nz;
     g.kfix = s->kfix;
     g.nn = s->nn;
+    g.apic_k = s->apic_textbook ? 4.0 : 3.0;
+    g.apic_decay = s->apic_textbook ? 1.0 : 0.99;
     return g;
 }
 
@@ -1991,12 +1994,14 @@ int mpmw_set_deterministic(mpmw_solver *s, int32_t on) {
 
 int mpmw_set_transfer(mpmw_solver *s, int32_t mode) {
     CHECK_S(s);
-    if (mode != MPMW_TRANSFER_PIC && mode != MPMW_TRANSFER_APIC) return fail(MPMW_E_INVAL, "unknown transfer");
-    const bool apic = mode == MPMW_TRANSFER_APIC;
+    if (mode != MPMW_TRANSFER_PIC && mode != MPMW_TRANSFER_APIC && mode != MPMW_TRANSFER_APIC_TEXTBOOK)
+        return fail(MPMW_E_INVAL, "unknown transfer");
+    const bool apic = mode != MPMW_TRANSFER_PIC;
     if (apic && s->deterministic)
         return fail(MPMW_E_STATE, "the deterministic mode is not available with the APIC transfer");
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->apic = apic;
+    s->apic_textbook = mode == MPMW_TRANSFER_APIC_TEXTBOOK;
     for (int c = F_C0; c <= F_C8; ++c) {
         if (apic) {
             int rc = ensure_field(s, c);  // zero-filled

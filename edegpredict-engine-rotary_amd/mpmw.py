@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmpmw_hip.so")
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
 OK, E_INVAL, E_HIP, E_STATE, E_NOMEM, E_RANGE = 0, -1, -2, -3, -4, -5
 FP32, FP64 = 0, 1
-TRANSFER_PIC, TRANSFER_APIC = 0, 1
+TRANSFER_PIC, TRANSFER_APIC, TRANSFER_APIC_TEXTBOOK = 0, 1, 2
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
 K_G2P_P2G, K_P2G, K_GRID, K_DIFFUSE, K_SORT, K_REDUCE, K_OTHER = range(7)
 TIMING_OFF, TIMING_ALL, TIMING_PARTICLE_PASS = 0, 1, 2
@@ -351,7 +351,8 @@ class Solver:
         return g, tuple(org), vox.value
 
     def set_transfer(self, mode):
-        """TRANSFER_PIC (trilinear) or TRANSFER_APIC (quadratic; every C set to zero)."""
+        """TRANSFER_PIC (trilinear), TRANSFER_APIC (quadratic; every C set to zero) or
+        TRANSFER_APIC_TEXTBOOK (quadratic with C = 4/h^2 B and no 0.99 decay)."""
         _check(lib().mpmw_set_transfer(self.h, mode), "set_transfer")
 
     def get_affine(self):

@@ -241,3 +241,5 @@ def apply_to_solver(mpmw, solver, cfg):
     solver.set_tool_tip(*cfg["tip"])
     if cfg.get("transfer") == "apic":
         solver.set_transfer(mpmw.TRANSFER_APIC)
+    elif cfg.get("transfer") == "apic_textbook":
+        solver.set_transfer(mpmw.TRANSFER_APIC_TEXTBOOK)

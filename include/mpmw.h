@@ -47,8 +47,11 @@ enum mpmw_precision { MPMW_FP32 = 0, MPMW_FP64 = 1 };
 
 /* Particle-grid transfer: the workpiece solver's PIC with trilinear weights
  * (MPMWorkpieceSolver.cu:52-90), or the APIC variant with quadratic
- * B-splines (MPMSolver.cu:20-34, 370-421, 554-592). */
-enum mpmw_transfer { MPMW_TRANSFER_PIC = 0, MPMW_TRANSFER_APIC = 1 };
+ * B-splines (MPMSolver.cu:20-34, 370-421, 554-592).  APIC_TEXTBOOK is an
+ * extension with no counterpart in the reference: the same transfer with the
+ * quadratic B-spline's own C = 4/h^2 * B and no 0.99 decay, under which an
+ * affine velocity field is carried exactly. */
+enum mpmw_transfer { MPMW_TRANSFER_PIC = 0, MPMW_TRANSFER_APIC = 1, MPMW_TRANSFER_APIC_TEXTBOOK = 2 };
 
 enum mpmw_tool_kind {
     MPMW_TOOL_NONE = 0,
@@ -249,7 +252,8 @@ int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
  *     P2G momentum m w (v + C (x_i - x_p)) with none from FIXED_BOUNDARY
  *     particles (p2gKernel :370-421), and from G2P
  *     C = 0.99 * 3/h^2 * sum w v_i (x_i - x_p)^T, unchanged for CHIP and
- *     FIXED_BOUNDARY particles (g2pKernel :554-592).  Everything else in the
+ *     FIXED_BOUNDARY particles (g2pKernel :554-592); MPMW_TRANSFER_APIC_TEXTBOOK
+ *     uses C = 4/h^2 * sum w v_i (x_i - x_p)^T.  Everything else in the
  *     step is the workpiece solver's.  Selecting a transfer sets every
  *     particle's C to zero; new particles start at zero.  The deterministic
  *     mode is not available with APIC (MPMW_E_STATE either way round). */

@@ -51,6 +51,7 @@ struct orc_solver {
     double max_stress, max_temp, ke, contact_force;
     int compactions;
     int transfer;         /* ORC_TRANSFER_PIC / ORC_TRANSFER_APIC */
+    int apic_textbook;    /* APIC with 4/h^2 and no decay */
 };
 
 static int g_threads = 1;
@@ -1151,10 +1152,10 @@ static void g2p_particle_apic(const orc_solver *s, orc_particle *p, double dt) {
     chip_transition(mat, p);
     if (!chip0) {
         const double inv = 1.0 / g->cell;
-        const double dinv = 3.0 * inv * inv;
+        const double dinv = (s->apic_textbook ? 4.0 : 3.0) * inv * inv;
         for (int c = 0; c < 9; ++c) {
             p->C[c] = B[c] * dinv;
-            p->C[c] *= 0.99;
+            if (!s->apic_textbook) p->C[c] *= 0.99;
         }
     }
     p->fx = p->fy = p->fz = 0.0;
@@ -1236,7 +1237,9 @@ void orc_compact(orc_solver *s) {
 }
 
 void orc_set_transfer(orc_solver *s, int mode) {
-    s->transfer = mode == ORC_TRANSFER_APIC ? ORC_TRANSFER_APIC : ORC_TRANSFER_PIC;
+    s->transfer = (mode == ORC_TRANSFER_APIC || mode == ORC_TRANSFER_APIC_TEXTBOOK) ? ORC_TRANSFER_APIC
+                                                                                  : ORC_TRANSFER_PIC;
+    s->apic_textbook = mode == ORC_TRANSFER_APIC_TEXTBOOK;
     for (int i = 0; i < s->n; ++i) memset(s->p[i].C, 0, sizeof(s->p[i].C));
 }
 

@@ -41,8 +41,9 @@ typedef struct {
 
 /* Particle-grid transfer: the workpiece solver's PIC with trilinear weights,
  * or the APIC variant with quadratic B-splines (MPMSolver.cu:20-34,370-421,
- * 554-592). */
-enum { ORC_TRANSFER_PIC = 0, ORC_TRANSFER_APIC = 1 };
+ * 554-592).  APIC_TEXTBOOK: the same APIC with the quadratic B-spline's own
+ * C = 4/h^2 * B and no 0.99 decay (no counterpart in the reference). */
+enum { ORC_TRANSFER_PIC = 0, ORC_TRANSFER_APIC = 1, ORC_TRANSFER_APIC_TEXTBOOK = 2 };
 
 /* MPMGridNodeGPU, include/MPMWorkpieceSolver.cuh:47-57 */
 typedef struct {

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "liborc_mpm.so")
 
 ACTIVE, INACTIVE, CHIP, BOUNDARY, FIXED_BOUNDARY = 0, 1, 2, 3, 4
 TOOL_NONE, TOOL_POINTS, TOOL_WEDGE, TOOL_DRILL, TOOL_VOXEL = 0, 1, 2, 3, 4
-TRANSFER_PIC, TRANSFER_APIC = 0, 1
+TRANSFER_PIC, TRANSFER_APIC, TRANSFER_APIC_TEXTBOOK = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -250,7 +250,8 @@ class Oracle:
         lib().orc_set_tool_pose(self.h, C.byref(pose))
 
     def set_transfer(self, mode):
-        """TRANSFER_PIC (trilinear) or TRANSFER_APIC (quadratic, affine C = 0)."""
+        """TRANSFER_PIC (trilinear), TRANSFER_APIC (quadratic, affine C = 0) or
+        TRANSFER_APIC_TEXTBOOK (quadratic with C = 4/h^2 B and no decay)."""
         lib().orc_set_transfer(self.h, mode)
 
     def get_affine(self):

@@ -51,6 +51,8 @@ def build_oracle(cfg):
     o.set_particles(d)
     if cfg.get("transfer") == "apic":
         o.set_transfer(oracle.TRANSFER_APIC)
+    elif cfg.get("transfer") == "apic_textbook":
+        o.set_transfer(oracle.TRANSFER_APIC_TEXTBOOK)
     kind = cfg["tool"][0]
     if kind == "points":
         o.set_tool_samples(cfg["tool"][1], cfg["tool"][2])
