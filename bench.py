@@ -94,10 +94,20 @@ def cpu_baseline(cfg, budget_s, threads):
     for _ in range(steps):
         o.step(cfg["dt"])
     el = time.time() - t0
+    # the serial build on one core: whole steps until about a third of the budget
+    oracle.set_threads(1)
+    serial_steps, t0 = 0, time.time()
+    while serial_steps == 0 or time.time() - t0 < budget_s / 3:
+        o.step(cfg["dt"])
+        serial_steps += 1
+    el1 = time.time() - t0
+    oracle.set_threads(threads)
     return {
         "value": n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
         "sample": f"{cfg['name']}: {n} particles, 1 warm-up + {steps} timed steps of the FP64 oracle "
                   f"({el:.1f} s; setup {setup:.1f} s)",
+        "serial": {"value": n * serial_steps / el1, "unit": "particle-steps/s", "cores": 1,
+                   "sample": f"the next {serial_steps} steps on one thread ({el1:.1f} s)"},
     }
 
 
