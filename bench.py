@@ -74,9 +74,27 @@ def occupied_nodes(cells):
     return (cx + 1) * (cy + 1) * (cz + 1)
 
 
-def cpu_baseline(cfg, budget_s, threads):
+def host_cpu_info():
+    """Cores this process may run on, the machine's count and its CPU model."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "model": None,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(cfg, budget_s, threads, runs=5):
     """The CPU FP64 oracle (restatement of the reference step) on the host
-    cores, over a bounded number of steps of the same workload."""
+    cores, over a bounded sample of the same workload: one warm-up step,
+    then `runs` timed runs of whole steps (each about budget_s / runs of
+    work); the reported rate is the median run.  The serial leg does the
+    same on one thread with `runs` single steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
@@ -89,25 +107,28 @@ def cpu_baseline(cfg, budget_s, threads):
     t0 = time.time()
     o.step(cfg["dt"])                      # warm-up step (includes the first grid rebuild)
     first = time.time() - t0
-    steps = max(1, min(50, int(budget_s / max(first, 1e-3))))
-    t0 = time.time()
-    for _ in range(steps):
-        o.step(cfg["dt"])
-    el = time.time() - t0
-    # the serial build on one core: whole steps until about a third of the budget
+    per_run = max(1, min(20, int(budget_s / runs / max(first, 1e-3))))
+    rates = []
+    for _ in range(runs):
+        t0 = time.time()
+        for _ in range(per_run):
+            o.step(cfg["dt"])
+        rates.append(n * per_run / (time.time() - t0))
     oracle.set_threads(1)
-    serial_steps, t0 = 0, time.time()
-    while serial_steps == 0 or time.time() - t0 < budget_s / 3:
+    srates = []
+    for _ in range(runs):
+        t0 = time.time()
         o.step(cfg["dt"])
-        serial_steps += 1
-    el1 = time.time() - t0
+        srates.append(n / (time.time() - t0))
     oracle.set_threads(threads)
+    info = host_cpu_info()
     return {
-        "value": n * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-        "sample": f"{cfg['name']}: {n} particles, 1 warm-up + {steps} timed steps of the FP64 oracle "
-                  f"({el:.1f} s; setup {setup:.1f} s)",
-        "serial": {"value": n * serial_steps / el1, "unit": "particle-steps/s", "cores": 1,
-                   "sample": f"the next {serial_steps} steps on one thread ({el1:.1f} s)"},
+        "value": float(np.median(rates)), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"{cfg['name']}: {n} particles, FP64 oracle, 1 warm-up step then the median of {runs} runs "
+                  f"of {per_run} steps on {threads} OpenMP threads (setup {setup:.1f} s)",
+        "runs": rates, "host": info,
+        "serial": {"value": float(np.median(srates)), "unit": "particle-steps/s", "cores": 1,
+                   "sample": f"median of {runs} single steps on one thread", "runs": srates},
     }
 
 
@@ -129,7 +150,14 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     dt = cfg["dt"]
     for _ in range(warmup):
         s.step(dt)
+    # untimed steps up to the next grid rebuild (every 25th step), so the
+    # timed window starts on a rebuild step and holds ceil(steps / 25) of
+    # them: never fewer than the workload's own share
+    align = (-s.metrics().step) % 25
+    for _ in range(align):
+        s.step(dt)
     s.synchronize()
+    first_timed = s.metrics().step
     ev = HipEvents()
     stream = s.stream()
     a, b = ev.create(), ev.create()
@@ -171,6 +199,8 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     names = ["g2p_p2g", "p2g", "grid_update", "diffuse", "sort", "reduce", "other"]
     return {
         "n": n, "ms_total": ms_total, "wall": wall, "setup_s": setup_s, "diag": diag,
+        "window": {"alignment_steps": align, "first_timed_step": first_timed,
+                   "rebuild_steps_timed": sum(1 for i in range(first_timed, first_timed + steps) if i % 25 == 0)},
         "grid_nodes": g.nx * g.ny * g.nz,
         "k_avg_ms": fused_ms / max(fused_n, 1), "k_launches": fused_n, "message": message,
         "kernels": {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]},
@@ -234,6 +264,27 @@ def measured_traffic():
     return None
 
 
+def copy_calibration(mib=1024, reps=3):
+    """A plain device copy of known size, the calibration kernel for the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters: `out = in * 1` over float32
+    buffers (an elementwise kernel that reads and writes every byte once,
+    which the runtime cannot turn into a DMA copy).  The ratio of the bytes a
+    launch must move to what the counters report for it scales the counters
+    of the solver's kernels to HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"kernel": "elementwise mul (copy)", "read_bytes": 4 * n, "write_bytes": 4 * n,
+                      "launches": reps, "GBs": 8 * n * reps / el / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +296,9 @@ def main():
     ap.add_argument("--cells", type=str, default="256,128,64")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 timing of the same workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--batch-count", type=int, default=64, help="workpieces of the batched run (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=4.0,
@@ -252,6 +306,9 @@ def main():
     ap.add_argument("--transfer", choices=["pic", "apic"], default="pic",
                     help="apic: the APIC transfer variant (atomic mode only)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
@@ -316,9 +373,19 @@ def main():
         "kernels": r["kernels"],  # every class, from steps after the timed region
         "result_message": dict(r["message"], format="NDJSON particle line of the final state"),
         "binning": r["diag"],
+        "timed_window": dict(r["window"], note="after the warm-up, untimed steps up to the next grid rebuild "
+                                               "(every 25th step) so the timed steps include the rebuild "
+                                               "steps at least in proportion"),
         "wall_s": r["wall"],
         "setup_s": r["setup_s"],
     }
+    if args.precision == "fp32" and not args.no_fp64:
+        # the same workload with every particle and grid field in FP64 (the
+        # reference's own arithmetic), timed the same way
+        q = run_mode(cfg, modes[0], "fp64", args.steps, args.warmup)
+        out["fp64"] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
+                       "mode": modes[0], "fused_kernel_avg_ms": q["k_avg_ms"], "kernels": q["kernels"],
+                       "timed_window": q["window"]}
     for m in modes[1:]:
         q = runs[m]
         out[m] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
@@ -346,7 +413,10 @@ def main():
         sv["saturated"] = sat
         out["served"] = sv
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host cores this process may use (the OpenMP thread count when
+        # the environment sets one: a share of a larger machine)
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        threads = int(omp) if omp.isdigit() and int(omp) > 0 else len(os.sched_getaffinity(0))
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget, threads)
     print(json.dumps(out), flush=True)
 

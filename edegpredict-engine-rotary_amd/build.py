@@ -47,8 +47,6 @@ def build(verbose=True):
     srcs = [s for s in srcs if os.path.exists(s)]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
-    # the HBM stream probe (practical bandwidth ceilings)
-    _run([HIPCC, *HIP_FLAGS, "-o", os.path.join(LIB_DIR, "hbm_probe"), os.path.join(HERE, "hbm_probe.hip")])
     # the C++ drop-in class driven through the reference interfaces
     test_src = os.path.join(ROOT, "tests", "cpp", "solver_api_test.cpp")
     if os.path.exists(test_src):

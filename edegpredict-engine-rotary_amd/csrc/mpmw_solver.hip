@@ -135,6 +135,7 @@ struct mpmw_solver {
     double sample_lo[3] = {0, 0, 0}, sample_hi[3] = {0, 0, 0};
     mpmw_tool_desc desc{};
     mpmw_tool_pose pose{};
+    mpmw_tool_pose pose0{};        // the pose as last set (restored by reset)
     void *sdf = nullptr;           // voxel SDF of the tool mesh (float, x fastest)
     int32_t sdf_dims[3] = {0, 0, 0};
     double sdf_origin[3] = {0, 0, 0}, sdf_voxel = 0.0;
@@ -344,11 +345,18 @@ int alloc_particles(mpmw_solver *s, int32_t count) {
     return MPMW_OK;
 }
 
+// Allocate a field if needed and set all cap entries to zero (a buffer kept
+// from an earlier particle set may hold stale values).
 int ensure_field(mpmw_solver *s, int fld) {
-    if (s->f[fld] || s->cap == 0) return MPMW_OK;
-    int rc = dalloc(s, &s->f[fld], field_bytes(s, fld, s->cap));
-    if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(s->f[fld], 0, field_bytes(s, fld, s->cap), s->stream));
+    if (s->cap == 0) return MPMW_OK;
+    if (!s->f[fld]) {
+        int rc = dalloc(s, &s->f[fld], field_bytes(s, fld, s->cap));
+        if (rc) return rc;
+    }
+    const int64_t words = (int64_t)(field_bytes(s, fld, s->cap) / 4);
+    hipLaunchKernelGGL(k_fill<int32_t>, dim3(std::min(blocks_for(words), 4096)), dim3(kBlock), 0, s->stream,
+                       (int32_t *)s->f[fld], words, 0);
+    HIP_TRY(hipGetLastError());
     return MPMW_OK;
 }
 
@@ -1412,6 +1420,8 @@ int mpmw_reset(mpmw_solver *s) {
     s->max_temp = s->prm.ref_temp;
     s->ke = 0.0;
     s->tip[0] = s->tip[1] = s->tip[2] = 0.0;
+    // the kinematic tool pose advances with every step: back to where it was set
+    s->pose = s->pose0;
     s->p2g_ready = false;
     s->stats_fresh = false;
     s->metrics_pending = false;
@@ -1806,7 +1816,7 @@ int mpmw_get_tool_sdf(mpmw_solver *s, float *grid, int32_t dims[3], double origi
 int mpmw_set_tool_pose(mpmw_solver *s, const mpmw_tool_pose *p) {
     CHECK_S(s);
     if (!p) return fail(MPMW_E_INVAL, "null pose");
-    s->pose = *p;
+    s->pose = s->pose0 = *p;
     return MPMW_OK;
 }
 

@@ -73,17 +73,30 @@ def request_stream(seed=workloads.SEED_SERVE, rate=20.0, duration=4.0, step_lo=2
 
 
 def state_bytes(req, precision):
-    """Device bytes a request holds while resident: particle fields and
-    their permutation copies (positions FP64), the binning arrays, the grid
-    (8 accumulator channels, the node record, mass and T*m) and the tile
-    windows, plus the rebuild padding of the grid."""
+    """Upper bound of the device bytes a request holds while resident (the
+    admission budget's unit).  Counts every particle field the request's set-up
+    makes live (positions FP64; backstress with kinematic hardening, the
+    affine matrix with the APIC transfer) in both buffer sets, the binning
+    arrays and the sort's temporary storage, the grid (8 accumulator channels,
+    node record, mass, T*m) and the tile windows at 5/4 of the starting box
+    (a grid grows by 5/4 when the workpiece outgrows it), the voxel SDF of a
+    mesh tool, and the pool's size classes (at most 1/8 over a request)."""
     r = 8 if precision == "fp64" else 4
+    cfg = req.cfg
+    P = cfg["params"]
+    apic = cfg.get("transfer") in ("apic", "apic_textbook")
     n = req.n
-    per_particle = 2 * (3 * 8 + 23 * r + 2 * 4) + 5 * 4
-    cx, cy, cz = req.cfg["cells"]
-    nodes = (cx + 10) * (cy + 10) * (cz + 10)
-    tiles = (cx // 8 + 3) * (cy // 8 + 3) * (cz // 4 + 3)
-    return n * per_particle + nodes * 14 * r + tiles * 8 * 405 * r
+    reals = 17 + (6 if P.get("kinematic_hardening", 0.0) > 0.0 else 0) + (9 if apic else 0)
+    per_particle = 2 * (3 * 8 + reals * r + 2 * 4) + 5 * 4 + 16
+    cx, cy, cz = cfg["cells"]
+    nodes = (cx + 10) * (cy + 10) * (cz + 10) * 5 // 4
+    tiles = (cx // 8 + 3) * (cy // 8 + 3) * (cz // 4 + 3) * 5 // 4
+    window = 847 if apic else 405
+    total = n * per_particle + nodes * 14 * r + tiles * 8 * window * r
+    if cfg["tool"][0] == "voxel":
+        vox = int(cfg["tool"][5])
+        total += 2 * 4 * vox ** 3
+    return int(total * 9 // 8) + (4 << 20)
 
 
 def _new_solver(req, precision, deterministic):

@@ -157,7 +157,9 @@ int mpmw_abi_version(void);
 void mpmw_params_default(mpmw_params *p);
 int mpmw_create(const mpmw_params *p, mpmw_solver **out);
 int mpmw_destroy(mpmw_solver *s);
-/* reset(), MPMWorkpieceSolver.cu:830-841 */
+/* reset(), MPMWorkpieceSolver.cu:830-841: particles, tool, time and step
+ * count cleared; the tool pose returns to the one last given to
+ * mpmw_set_tool_pose (the pose advances with every step). */
 int mpmw_reset(mpmw_solver *s);
 
 /* --- particles: uploadParticles/downloadParticles (:546-560) and

@@ -52,6 +52,7 @@ struct orc_solver {
     int compactions;
     int transfer;         /* ORC_TRANSFER_PIC / ORC_TRANSFER_APIC */
     int apic_textbook;    /* APIC with 4/h^2 and no decay */
+    int fp32_emulation;   /* sensitivity probe: FP32 P2G sums and stored state */
 };
 
 static int g_threads = 1;
@@ -610,7 +611,23 @@ static void rebuild_grid(orc_solver *s) {
     alloc_grid(s);
 }
 
+static void load_particles(orc_solver *s, const orc_particle_arrays *a);
+
 int orc_set_particles(orc_solver *s, const orc_particle_arrays *a) {
+    load_particles(s, a);
+    rebuild_grid(s);   /* as initializeParticleBox does before uploadParticles */
+    return 0;
+}
+
+/* The particle state replaced, the grid, step count and time kept: the
+ * shadow oracle of a parity test continues a trajectory from another
+ * solver's state.  No counterpart in the reference. */
+int orc_replace_particles(orc_solver *s, const orc_particle_arrays *a) {
+    load_particles(s, a);
+    return 0;
+}
+
+static void load_particles(orc_solver *s, const orc_particle_arrays *a) {
     ensure_particles(s, a->n);
     s->n = a->n;
     for (int i = 0; i < a->n; ++i) {
@@ -631,8 +648,6 @@ int orc_set_particles(orc_solver *s, const orc_particle_arrays *a) {
         q->id = a->id[i];
         q->status = a->status[i];
     }
-    rebuild_grid(s);   /* as initializeParticleBox does before uploadParticles */
-    return 0;
 }
 
 int orc_get_particles(orc_solver *s, orc_particle_arrays *a) {
@@ -756,7 +771,14 @@ void orc_apply_external_force(orc_solver *s, const double c[3], double radius, c
 /* ---------------------------------------------------------------- step --- */
 
 /* One particle of mpmP2GKernel, MPMWorkpieceSolver.cu:103-133. */
-static void p2g_particle_pic(const orc_grid *g, const orc_particle *p, orc_node *G) {
+/* x += v, rounded to FP32 after the add when f32 is set (the sensitivity
+ * probe's FP32 accumulators; off for every parity use of the oracle). */
+static void acc(double *x, double v, int f32) {
+    *x += v;
+    if (f32) *x = (double)(float)*x;
+}
+
+static void p2g_particle_pic(const orc_grid *g, const orc_particle *p, orc_node *G, int f32) {
     int32_t ids[8];
     double ws[8], gr[24];
     orc_weights(g, p->x, p->y, p->z, ids, ws, gr);
@@ -765,23 +787,23 @@ static void p2g_particle_pic(const orc_grid *g, const orc_particle *p, orc_node 
         if (gid < 0 || ws[n] <= 0.0) continue;
         orc_node *nd = &G[gid];
         double m = p->mass * ws[n];
-        nd->mass += m;
-        nd->vx += p->vx * m;
-        nd->vy += p->vy * m;
-        nd->vz += p->vz * m;
-        nd->tm += p->temperature * m;
+        acc(&nd->mass, m, f32);
+        acc(&nd->vx, p->vx * m, f32);
+        acc(&nd->vy, p->vy * m, f32);
+        acc(&nd->vz, p->vz * m, f32);
+        acc(&nd->tm, p->temperature * m, f32);
         if (p->status == ORC_ACTIVE || p->status == ORC_FIXED_BOUNDARY) {
             double gx = gr[3 * n], gy = gr[3 * n + 1], gz = gr[3 * n + 2];
             double fX = -(p->s[0] * gx + p->s[3] * gy + p->s[4] * gz) * p->volume;
             double fY = -(p->s[3] * gx + p->s[1] * gy + p->s[5] * gz) * p->volume;
             double fZ = -(p->s[4] * gx + p->s[5] * gy + p->s[2] * gz) * p->volume;
-            nd->fx += fX + p->fx * ws[n];
-            nd->fy += fY + p->fy * ws[n];
-            nd->fz += fZ + p->fz * ws[n];
+            acc(&nd->fx, fX + p->fx * ws[n], f32);
+            acc(&nd->fy, fY + p->fy * ws[n], f32);
+            acc(&nd->fz, fZ + p->fz * ws[n], f32);
         } else if (p->status == ORC_CHIP) {
-            nd->fx += p->fx * ws[n];
-            nd->fy += p->fy * ws[n];
-            nd->fz += p->fz * ws[n];
+            acc(&nd->fx, p->fx * ws[n], f32);
+            acc(&nd->fy, p->fy * ws[n], f32);
+            acc(&nd->fz, p->fz * ws[n], f32);
         }
     }
 }
@@ -852,7 +874,7 @@ static void p2g(orc_solver *s) {
             const orc_particle *p = &s->p[pi];
             if (p->status == ORC_INACTIVE) continue;
             if (s->transfer == ORC_TRANSFER_APIC) p2g_particle_apic(g, p, G);
-            else p2g_particle_pic(g, p, G);
+            else p2g_particle_pic(g, p, G, s->fp32_emulation);
         }
     }
     if (priv) {
@@ -1253,6 +1275,24 @@ void orc_set_affine(orc_solver *s, const double *C) {
 }
 
 /* MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791 */
+static double to_f32(double v) { return (double)(float)v; }
+
+void orc_set_fp32_emulation(orc_solver *s, int on) { s->fp32_emulation = on ? 1 : 0; }
+
+void orc_round_state_fp32(orc_solver *s) {
+    for (int i = 0; i < s->n; ++i) {
+        orc_particle *q = &s->p[i];
+        double *f[] = {&q->vx, &q->vy, &q->vz, &q->fx, &q->fy, &q->fz, &q->mass, &q->volume, &q->density,
+                       &q->temperature, &q->plasticStrain, &q->strainRate, &q->damage, &q->residualStress};
+        for (size_t k = 0; k < sizeof(f) / sizeof(f[0]); ++k) *f[k] = to_f32(*f[k]);
+        for (int c = 0; c < 6; ++c) {
+            q->s[c] = to_f32(q->s[c]);
+            q->a[c] = to_f32(q->a[c]);
+        }
+        for (int c = 0; c < 9; ++c) q->C[c] = to_f32(q->C[c]);
+    }
+}
+
 void orc_step(orc_solver *s, double dt) {
     if (s->n <= 0 || !s->grid) return;
     if (s->step % 25 == 0) rebuild_grid(s);
@@ -1276,6 +1316,7 @@ void orc_step(orc_solver *s, double dt) {
     ++s->step;
     if (s->step % 25 == 0) update_metrics(s);
     if (s->step % 1000 == 0 && s->step > 0) orc_compact(s);
+    if (s->fp32_emulation) orc_round_state_fp32(s);
 }
 
 void orc_run(orc_solver *s, int nsteps, double dt) {

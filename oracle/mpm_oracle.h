@@ -147,6 +147,20 @@ double orc_stable_dt(const orc_solver *s);
 void orc_get_metrics(const orc_solver *s, orc_metrics *m);
 void orc_sync_metrics(orc_solver *s);
 void orc_compact(orc_solver *s);
+/* Replace the particle state, keeping the grid, step count and time (a
+ * shadow oracle that continues from another solver's state).  No
+ * counterpart in the reference. */
+int orc_replace_particles(orc_solver *s, const orc_particle_arrays *a);
+/* Round every stored particle field except the positions to the nearest
+ * FP32 value (a sensitivity probe: the state an FP32 solver keeps between
+ * steps, with FP64 positions).  No counterpart in the reference. */
+void orc_round_state_fp32(orc_solver *s);
+/* Sensitivity probe (no counterpart in the reference; off by default): the
+ * PIC P2G rounds every node sum to FP32 after each add and the step ends
+ * with orc_round_state_fp32, i.e. FP64 arithmetic with the rounding of FP32
+ * accumulators and FP32 stored state.  The difference it makes to a run is
+ * the run's sensitivity to FP32-sized errors. */
+void orc_set_fp32_emulation(orc_solver *s, int on);
 void orc_reset(orc_solver *s);
 void orc_get_bounds(const orc_solver *s, double b[6]);
 double orc_temperature_at(const orc_solver *s, double x, double y, double z);

@@ -100,6 +100,9 @@ def lib():
         L.orc_get_metrics.argtypes = [C.c_void_p, C.POINTER(Metrics)]
         L.orc_sync_metrics.argtypes = [C.c_void_p]
         L.orc_compact.argtypes = [C.c_void_p]
+        L.orc_round_state_fp32.argtypes = [C.c_void_p]
+        L.orc_replace_particles.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_fp32_emulation.argtypes = [C.c_void_p, C.c_int]
         L.orc_reset.argtypes = [C.c_void_p]
         L.orc_get_bounds.argtypes = [C.c_void_p, _dp]
         L.orc_temperature_at.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
@@ -220,6 +223,11 @@ class Oracle:
         self._keep = d
         return lib().orc_set_particles(self.h, C.byref(_to_struct(d)))
 
+    def replace_particles(self, d):
+        """Replace the particle state; grid, step count and time are kept."""
+        a = _to_struct(d)
+        lib().orc_replace_particles(self.h, C.byref(a))
+
     def count(self):
         return lib().orc_particle_count(self.h)
 
@@ -288,6 +296,14 @@ class Oracle:
 
     def compact(self):
         lib().orc_compact(self.h)
+
+    def set_fp32_emulation(self, on):
+        """Sensitivity probe: FP32-rounded P2G sums and stored state."""
+        lib().orc_set_fp32_emulation(self.h, 1 if on else 0)
+
+    def round_state_fp32(self):
+        """Round every stored particle field but the positions to FP32."""
+        lib().orc_round_state_fp32(self.h)
 
     def reset(self):
         lib().orc_reset(self.h)

@@ -108,17 +108,23 @@ def compare_particles(g, o):
     return errs, int((~same).sum())
 
 
-def oracle_trajectory(cfg, checkpoints, order=None, perturb=0.0, seed=0):
+def oracle_trajectory(cfg, checkpoints, order=None, perturb=0.0, seed=0, perturb_vel=0.0, fp32_emulation=False):
     """Run the oracle over cfg['steps'] steps; return {step: particle dict}
     for every step index in `checkpoints` (state before that step).  `order`
     permutes the particle input order (changes the P2G summation order);
-    `perturb` adds relative noise of that size to the initial positions."""
+    `perturb` adds relative noise of that size to the initial positions,
+    `perturb_vel` to the initial velocities; `fp32_emulation` rounds the
+    P2G node sums after every add and the stored particle state (all but
+    the positions) after every step to FP32, as an FP32 solver keeps them."""
     o, d = build_oracle(cfg)
-    if order is not None or perturb:
+    if fp32_emulation:
+        o.set_fp32_emulation(True)
+    if order is not None or perturb or perturb_vel:
         d = {k: (None if v is None else v.copy()) for k, v in d.items()}
-        if perturb:
+        if perturb or perturb_vel:
             rng = np.random.default_rng(seed)
             d["pos"] *= 1.0 + perturb * rng.uniform(-1, 1, d["pos"].shape)
+            d["vel"] *= 1.0 + perturb_vel * rng.uniform(-1, 1, d["vel"].shape)
         if order is not None:
             for k, v in d.items():
                 if v is not None:
@@ -191,3 +197,30 @@ def run_case(cfg, precision, nsteps=None, deterministic=False, snapshot=True, mu
     out["solver"] = s
     out["oracle"] = o
     return out
+
+
+def shadow_run(cfg, precision, nsteps=None, deterministic=False, fields=("pos", "temperature", "plastic_strain",
+                                                                        "vel"), particles=None):
+    """Step parity along the GPU's own trajectory: before every step the
+    oracle takes the GPU's particle state (orc_replace_particles keeps the
+    oracle's grid, step count and rebuild phase, which follow the same rule
+    as the GPU's), then both take the step.  Returns the per-step errors
+    {step: (errors, status_mismatch)} and the worst error per field."""
+    nsteps = cfg["steps"] if nsteps is None else nsteps
+    o, d = build_oracle(cfg)
+    s = build_gpu(cfg, precision, deterministic, particles=particles)
+    if particles is not None:
+        o.set_particles(particles)
+    per_step, worst = {}, {f: 0.0 for f in fields}
+    for k in range(nsteps):
+        o.replace_particles(s.get_particles())
+        o.step(cfg["dt"])
+        s.step(cfg["dt"])
+        g, og = s.grid_info(), o.grid()
+        assert (g.nx, g.ny, g.nz) == (og.nx, og.ny, og.nz), ("grid differs", k)
+        e, m = compare_particles(s.get_particles(), o.get_particles())
+        per_step[k] = (e, m)
+        for f in fields:
+            worst[f] = max(worst[f], e[f])
+    s.close()
+    return per_step, worst

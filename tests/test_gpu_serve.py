@@ -56,3 +56,28 @@ def test_served_summary_and_budget():
             if q is not r:
                 assert q.finished <= r.admitted or r.finished <= q.admitted, (q.rid, r.rid)
     assert out["p99_time_to_result_s"] >= out["p50_time_to_result_s"] > 0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("kind", ["wedge", "voxel", "apic", "hardening"])
+def test_admission_bound_covers_device_bytes(kind, precision):
+    """The admission estimate is an upper bound of what a resident request
+    really holds on the device (pool size classes, permutation buffers, the
+    binning, tile windows, a voxel SDF, APIC and backstress fields included),
+    after enough steps to rebuild the grid and re-bin."""
+    cfg = sorted(workloads.batch_config2(), key=lambda c: np.prod(c["cells"]))[5]
+    if kind == "voxel":
+        cfg = workloads.voxel_tool_case(cfg)
+    elif kind == "apic":
+        cfg = dict(cfg, transfer="apic")
+    elif kind == "hardening":
+        cfg = dict(cfg, params=dict(cfg["params"], kinematic_hardening=2e9))
+    q = serve.Request(0, 0.0, dict(cfg, steps=30))
+    s = serve._new_solver(q, precision, False)
+    try:
+        s.run(30, cfg["dt"])
+        s.synchronize()
+        held = s.device_bytes()
+        assert 0 < held <= serve.state_bytes(q, precision), (kind, held, serve.state_bytes(q, precision))
+    finally:
+        s.close()
