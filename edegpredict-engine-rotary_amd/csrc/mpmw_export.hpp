@@ -201,6 +201,132 @@ template <typename R> int export_ndjson_impl(mpmw_solver *s, int32_t step, char 
     return rc;
 }
 
+// ---- the allocation-free NDJSON export ---------------------------------
+//
+// Every device buffer of mpmw_export_ndjson_noalloc (flags, ranks, the
+// scan's temporary storage, packed positions, packed temperatures and their
+// base64) lies in `scratch`, one static array per device that the library
+// holds until the process ends.  Host code never passes `scratch` itself to
+// the device (its name there is a host-side stand-in): the address of each
+// device's copy comes from hipGetSymbolAddress on the first call on that
+// device and is kept per device.  Calls on one device are serialised by a
+// lock, so they do not overlap in the array; the array keeps the last
+// call's contents, and no part of it needs to start at zero (every word a
+// call reads was written earlier in the same call).
+constexpr size_t kExportScratchBytes = MPMW_EXPORT_SCRATCH_BYTES;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kExportScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_addr;       // device -> address of its copy of `scratch`
+std::map<int, std::mutex> g_scratch_lock;   // device -> lock of its copy
+
+int scratch_on_device(int dev, char **out) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_addr.find(dev);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return MPMW_OK;
+    }
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(MPMW_E_HIP, std::string("export_ndjson_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = (char *)p;
+    *out = (char *)p;
+    return MPMW_OK;
+}
+
+std::mutex &scratch_lock(int dev) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_lock[dev];
+}
+
+// Offsets of the buffers of an export of n particles inside the scratch
+// array; `end` is the bytes it needs.
+struct ExportLayout {
+    size_t flag, off, scan, scan_bytes, pos, tmp, t64, end;
+};
+
+int export_layout(int n, ExportLayout *L) {
+    HIP_TRY(exclusive_scan_temp_bytes(n + 1, &L->scan_bytes));
+    L->flag = 0;
+    L->off = align256(L->flag + sizeof(int32_t) * ((size_t)n + 1));
+    L->scan = align256(L->off + sizeof(int32_t) * ((size_t)n + 1));
+    L->pos = align256(L->scan + L->scan_bytes);
+    L->tmp = align256(L->pos + 16 * (size_t)n);
+    L->t64 = align256(L->tmp + sizeof(float) * (size_t)n);
+    L->end = L->t64 + 4 * ((4 * (size_t)n + 2) / 3);
+    return MPMW_OK;
+}
+
+template <typename R>
+int export_ndjson_noalloc_impl(mpmw_solver *s, int32_t step, char *buf, size_t cap, size_t *len) {
+    *len = 0;
+    if (s->n <= 0) return MPMW_OK;
+    const int n = s->n;
+    ExportLayout L;
+    int rc = export_layout(n, &L);
+    if (rc) return rc;
+    if (L.end > kExportScratchBytes) {
+        // bytes per particle: flag + rank 8, position 16, temperature 4, its base64 16/3
+        const size_t most = (kExportScratchBytes - 4096 - L.scan_bytes) * 3 / 100;
+        return fail(MPMW_E_RANGE, "export_ndjson_noalloc: " + std::to_string(n) + " particles need " +
+                                      std::to_string(L.end) + " bytes of scratch, the limit is " +
+                                      std::to_string(kExportScratchBytes) + " (about " + std::to_string(most) +
+                                      " particles); use mpmw_export_ndjson");
+    }
+    const int dev = s->prm.device;
+    char *base = nullptr;
+    if ((rc = scratch_on_device(dev, &base))) return rc;
+    std::lock_guard<std::mutex> lk(scratch_lock(dev));
+    int32_t *flag = (int32_t *)(base + L.flag), *off = (int32_t *)(base + L.off);
+    hipLaunchKernelGGL(k_export_flags, dim3(blocks_for(n + 1)), dim3(kBlock), 0, s->stream,
+                       (const int32_t *)s->f[F_STATUS], flag, n);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(exclusive_scan_buf(base + L.scan, L.scan_bytes, flag, off, n + 1, s->stream));
+    int32_t count = 0;
+    HIP_TRY(hipMemcpyAsync(&count, off + n, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (count == 0) return MPMW_OK;
+    const size_t npos = 16 * (size_t)count;
+    const size_t ntmp = 4 * (((size_t)count * 4 + 2) / 3);
+    const std::string h0 = "{\"count\":" + std::to_string(count) + ",\"positions_b64\":\"";
+    const std::string h1 = "\",\"step\":" + std::to_string(step) + ",\"temps_b64\":\"";
+    const std::string h2 = "\",\"type\":\"particles\"}\n";
+    const size_t total = h0.size() + npos + h1.size() + ntmp + h2.size();
+    *len = total;
+    if (!buf) return MPMW_OK;
+    if (cap < total)
+        return fail(MPMW_E_RANGE, "export_ndjson_noalloc: buffer of " + std::to_string(cap) + " bytes, " +
+                                      std::to_string(total) + " needed");
+    hipLaunchKernelGGL(k_export_pack<R>, dim3(blocks_for(n)), dim3(kBlock), 0, s->stream,
+                       (const double *)s->f[F_X], (const double *)s->f[F_Y], (const double *)s->f[F_Z],
+                       (const R *)s->f[F_T], (const int32_t *)s->f[F_STATUS], (const int32_t *)off,
+                       (uint4 *)(base + L.pos), (float *)(base + L.tmp), n);
+    const int64_t groups = (4 * (int64_t)count + 2) / 3;
+    hipLaunchKernelGGL(k_b64, dim3((unsigned)((groups + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                       (const uint8_t *)(base + L.tmp), 4 * (int64_t)count, (uint32_t *)(base + L.t64));
+    HIP_TRY(hipGetLastError());
+    char *p = buf;
+    std::memcpy(p, h0.data(), h0.size());
+    char *p_pos = p + h0.size();
+    p = p_pos + npos;
+    std::memcpy(p, h1.data(), h1.size());
+    char *p_tmp = p + h1.size();
+    p = p_tmp + ntmp;
+    std::memcpy(p, h2.data(), h2.size());
+    // straight into the caller's memory: no pinned staging is allocated
+    HIP_TRY(hipMemcpyAsync(p_pos, base + L.pos, npos, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipMemcpyAsync(p_tmp, base + L.t64, ntmp, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return MPMW_OK;
+}
+
 template <typename R> int export_vtk_impl(mpmw_solver *s, const char *path) {
     const int n = s->n;
     std::vector<double> pos(3 * (size_t)n), vel(3 * (size_t)n), T(n), stress(6 * (size_t)n);
@@ -253,6 +379,12 @@ int mpmw_export_ndjson(mpmw_solver *s, int32_t step, char *buf, size_t cap, size
     CHECK_S(s);
     if (!len) return fail(MPMW_E_INVAL, "export_ndjson: null length");
     return DISPATCH(s, export_ndjson_impl, s, step, buf, cap, len);
+}
+
+int mpmw_export_ndjson_noalloc(mpmw_solver *s, int32_t step, char *buf, size_t cap, size_t *len) {
+    CHECK_S(s);
+    if (!len) return fail(MPMW_E_INVAL, "export_ndjson_noalloc: null length");
+    return DISPATCH(s, export_ndjson_noalloc_impl, s, step, buf, cap, len);
 }
 
 int mpmw_export_vtk(mpmw_solver *s, const char *path) {

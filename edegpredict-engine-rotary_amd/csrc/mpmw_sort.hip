@@ -38,6 +38,15 @@ hipError_t exclusive_scan(ScratchBuf &temp, const int32_t *in, int32_t *out, int
     return hipcub::DeviceScan::ExclusiveSum(temp.ptr, have, in, out, n, st);
 }
 
+hipError_t exclusive_scan_temp_bytes(int n, size_t *bytes) {
+    *bytes = 0;
+    return hipcub::DeviceScan::ExclusiveSum(nullptr, *bytes, (const int32_t *)nullptr, (int32_t *)nullptr, n);
+}
+
+hipError_t exclusive_scan_buf(void *temp, size_t bytes, const int32_t *in, int32_t *out, int n, hipStream_t st) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, bytes, in, out, n, st);
+}
+
 void free_scratch(ScratchBuf &t) {
     if (t.ptr) (void)hipFree(t.ptr);
     t.ptr = nullptr;

@@ -47,6 +47,10 @@ hipError_t radix_sort_pairs(ScratchBuf &temp, const uint32_t *kin, uint32_t *kou
                             int32_t *vout, int n, int nbits, hipStream_t st);
 // Exclusive prefix sum of n int32 values.
 hipError_t exclusive_scan(ScratchBuf &temp, const int32_t *in, int32_t *out, int n, hipStream_t st);
+// The same scan into caller-provided temporary storage (no allocation):
+// exclusive_scan_temp_bytes gives the bytes it needs for n values.
+hipError_t exclusive_scan_temp_bytes(int n, size_t *bytes);
+hipError_t exclusive_scan_buf(void *temp, size_t bytes, const int32_t *in, int32_t *out, int n, hipStream_t st);
 void free_scratch(ScratchBuf &t);
 
 }  // namespace mpmw

@@ -37,7 +37,7 @@ EXPORTS = (
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
     "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
-    "mpmw_morph_tool_sdf", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
+    "mpmw_export_ndjson_noalloc", "mpmw_morph_tool_sdf", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
 )
 
 
@@ -166,6 +166,8 @@ def lib():
         "mpmw_set_affine": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_export_ndjson": (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
         "mpmw_export_vtk": (C.c_int, [vp, C.c_char_p]),
+        "mpmw_export_ndjson_noalloc": (C.c_int, [vp, C.c_int32, C.c_char_p, C.c_size_t,
+                                                 C.POINTER(C.c_size_t)]),
         "mpmw_morph_tool_sdf": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_tool_mesh_params_default": (None, [C.POINTER(ToolMeshParams)]),
         "mpmw_parametric_tool_mesh": (C.c_int, [C.POINTER(ToolMeshParams), _dp, _ip, _dp, _ip, _ip]),
@@ -381,6 +383,19 @@ class Solver:
         buf = np.empty(n.value, dtype=np.uint8)  # filled by the call, no zeroing
         _check(lib().mpmw_export_ndjson(self.h, int(step), buf.ctypes.data_as(C.c_char_p), n.value, C.byref(n)),
                "export_ndjson")
+        return buf[:n.value].tobytes()
+
+    def export_ndjson_noalloc(self, step):
+        """The same message as export_ndjson, built in the library's static
+        per-device scratch array (no device or pinned allocation; at most
+        about 2.5 M particles)."""
+        n = C.c_size_t(0)
+        _check(lib().mpmw_export_ndjson_noalloc(self.h, int(step), None, 0, C.byref(n)), "export_ndjson_noalloc")
+        if n.value == 0:
+            return b""
+        buf = np.empty(n.value, dtype=np.uint8)
+        _check(lib().mpmw_export_ndjson_noalloc(self.h, int(step), buf.ctypes.data_as(C.c_char_p), n.value,
+                                                C.byref(n)), "export_ndjson_noalloc")
         return buf[:n.value].tobytes()
 
     def export_vtk(self, path):

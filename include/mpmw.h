@@ -301,6 +301,18 @@ int mpmw_parametric_tool_mesh(const mpmw_tool_mesh_params *p, double *verts, int
  * that are not INACTIVE. */
 int mpmw_export_ndjson(mpmw_solver *s, int32_t step, char *buf, size_t cap, size_t *len);
 int mpmw_export_vtk(mpmw_solver *s, const char *path);
+/* mpmw_export_ndjson with no device or pinned-host allocation, for callers
+ * that cannot allocate (a serving loop that must not call hipMalloc between
+ * steps, code around HIP graph capture).  No counterpart in the reference.
+ * The same line, byte for byte, as mpmw_export_ndjson; its device buffers
+ * live in one static scratch array of MPMW_EXPORT_SCRATCH_BYTES per device
+ * (held until the process ends), which limits it to about 2.5 million
+ * particles: a larger solver gets MPMW_E_RANGE and nothing is written.  The
+ * line is copied straight into buf (pageable memory is slower than the
+ * staged copy of mpmw_export_ndjson).  Calls on one device share the array
+ * and are serialised by a lock inside the library. */
+#define MPMW_EXPORT_SCRATCH_BYTES (80u << 20)
+int mpmw_export_ndjson_noalloc(mpmw_solver *s, int32_t step, char *buf, size_t cap, size_t *len);
 
 /* --- inspection (parity / measurement) */
 int mpmw_get_grid_info(mpmw_solver *s, mpmw_grid_info *g);
