@@ -132,17 +132,21 @@ def cpu_baseline(cfg, budget_s, threads, runs=5):
     }
 
 
-def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
+def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25, graph=False):
     """Set up the workload in one solver and time `steps` steps after
     `warmup` untimed ones; HIP events on the solver's stream bracket the
     timed region, and library events bracket every launch of the fused
     particle pass inside it (the roofline's kernel duration).  Events around
     every kernel class would add two records per launch to the stream, which
     lengthens the step, so the other classes' averages come from
-    `breakdown_steps` further steps after the timed region."""
+    `breakdown_steps` further steps after the timed region.  With `graph`
+    the step's launches are replayed as HIP graphs and the timed region
+    carries no per-kernel events (the kernel averages then come from the
+    breakdown steps, stepped eagerly)."""
     P = dict(cfg["params"], precision=mpmw.FP64 if precision == "fp64" else mpmw.FP32,
              deterministic=1 if mode == "deterministic" else 0)
     s = mpmw.Solver(**P)
+    s.set_graph(graph)
     t0 = time.time()
     workloads.apply_to_solver(mpmw, s, cfg)
     setup_s = time.time() - t0
@@ -161,7 +165,7 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     ev = HipEvents()
     stream = s.stream()
     a, b = ev.create(), ev.create()
-    s.set_kernel_timing(mpmw.TIMING_PARTICLE_PASS)
+    s.set_kernel_timing(mpmw.TIMING_OFF if graph else mpmw.TIMING_PARTICLE_PASS)
     s.synchronize()
     t_wall = time.perf_counter()
     ev.record(a, stream)
@@ -173,6 +177,7 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
     ms_total = ev.elapsed_ms(a, b)
     kms, kn = s.kernel_times()
     fused_ms, fused_n = kms[mpmw.K_G2P_P2G], kn[mpmw.K_G2P_P2G]
+    graph_stats = s.graph_stats()
     # per-class breakdown (not part of the timed region)
     s.set_kernel_timing(mpmw.TIMING_ALL)
     for _ in range(breakdown_steps):
@@ -203,11 +208,12 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25):
                    "rebuild_steps_timed": sum(1 for i in range(first_timed, first_timed + steps) if i % 25 == 0)},
         "grid_nodes": g.nx * g.ny * g.nz,
         "k_avg_ms": fused_ms / max(fused_n, 1), "k_launches": fused_n, "message": message,
+        "graph": graph_stats if graph else None,
         "kernels": {names[i]: {"avg_ms": kms[i] / kn[i], "launches": kn[i]} for i in range(7) if kn[i]},
     }
 
 
-def batched_bench(cfgs, precision, steps):
+def batched_bench(cfgs, precision, steps, graph=False):
     """The batched workload (64 workpieces of uneven size): wall time of the
     serial plain solves (one solver after another, each set up, stepped and
     synchronised) against the same set-ups stepped together in one batch.
@@ -227,6 +233,7 @@ def batched_bench(cfgs, precision, steps):
     t0 = time.perf_counter()
     solvers = []
     b = mpmw.Batch()
+    b.set_graph(graph)
     for c in cfgs:
         s = mpmw.Solver(**P(c))
         workloads.apply_to_solver(mpmw, s, c)
@@ -237,6 +244,7 @@ def batched_bench(cfgs, precision, steps):
         b.step(dts)
     b.synchronize()
     batched = time.perf_counter() - t0
+    graph_stats = b.graph_stats()
     b.close()
     for s in solvers:
         s.close()
@@ -248,6 +256,7 @@ def batched_bench(cfgs, precision, steps):
         "batched_sims_per_hour": nsim / batched * 3600.0,
         "speedup": serial / batched,
         "batched_particle_steps_per_s": parts * steps / batched,
+        "graph": graph_stats if graph else None,
     }
 
 
@@ -297,6 +306,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 timing of the same workload")
+    ap.add_argument("--graph", type=int, default=1, help="1: also time the step replayed as HIP graphs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--batch-count", type=int, default=64, help="workpieces of the batched run (0: skip it)")
@@ -379,6 +389,13 @@ def main():
         "wall_s": r["wall"],
         "setup_s": r["setup_s"],
     }
+    if args.graph:
+        # the same atomic-mode workload with the step's launches replayed as
+        # HIP graphs: the timed region has no kernel events, so the roofline
+        # above keeps the eager run's fused-pass timing (the same kernel)
+        q = run_mode(cfg, modes[0], args.precision, args.steps, args.warmup, graph=True)
+        out["graph"] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
+                        "mode": modes[0], "graph_stats": q["graph"], "timed_window": q["window"]}
     if args.precision == "fp32" and not args.no_fp64:
         # the same workload with every particle and grid field in FP64 (the
         # reference's own arithmetic), timed the same way
@@ -391,8 +408,12 @@ def main():
         out[m] = {"value": n * args.steps / (q["ms_total"] / 1e3), "ms_per_step": q["ms_total"] / args.steps,
                   "fused_kernel_avg_ms": q["k_avg_ms"], "kernels": q["kernels"], "binning": q["diag"]}
     if args.batch_count > 0:
-        out["batched"] = batched_bench(workloads.batch_config2(args.batch_count, steps=args.batch_steps),
-                                       args.precision, args.batch_steps)
+        bcfg = workloads.batch_config2(args.batch_count, steps=args.batch_steps)
+        out["batched"] = batched_bench(bcfg, args.precision, args.batch_steps)
+        if args.graph:
+            g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True)
+            out["batched"]["graph"] = {k: g[k] for k in ("serial_s", "batched_s", "speedup", "graph",
+                                                         "batched_sims_per_hour")}
     if args.serve_seconds > 0:
         sv = {"workload": f"Poisson stream of 20 requests/s for {args.serve_seconds:g} s, sizes and parameters of "
                           "the batched distribution, 200-2000 steps each; a request's result is the NDJSON "

@@ -42,6 +42,16 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+}  // namespace
+
+// The error text of mpmw_last_error, for the library's other translation
+// units (the F-tracking solver).
+namespace mpmw_internal {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace mpmw_internal
+
+namespace {
+
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \
@@ -87,6 +97,14 @@ struct StepStaging {
     int parity = 0;
     void *d_blob = nullptr;
     size_t d_cap = 0;
+    // HIP graphs of the step's launches, one per staging area: the copy of
+    // that area into the device blob, then the kernels that read the blob.
+    // A graph is replayed while its launch configuration (the signature)
+    // stays the same, and captured again when it changes.
+    bool graph_on = false;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    std::vector<uint64_t> sig[2];
+    int64_t captures = 0, replays = 0;
 };
 
 struct mpmw_solver {
@@ -984,6 +1002,7 @@ template <typename R> int step_end(mpmw_solver *s, double dt, const StepPlan &sp
 template <typename R> int step_many(mpmw_solver *const *members, size_t nm, const double *dts,
                                    hipStream_t stream, StepStaging &st);
 void free_staging(StepStaging &st);
+void drop_graphs(StepStaging &st);
 
 // MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791: a step of a batch
 // of one, so a solver runs the same kernels alone and in a batch.
@@ -1993,6 +2012,22 @@ int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t coun
     return MPMW_OK;
 }
 
+int mpmw_set_graph(mpmw_solver *s, int32_t on) {
+    CHECK_S(s);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->staging.graph_on = on != 0;
+    if (!on) drop_graphs(s->staging);
+    return MPMW_OK;
+}
+
+int mpmw_graph_stats(mpmw_solver *s, int64_t out[2]) {
+    CHECK_S(s);
+    if (!out) return fail(MPMW_E_INVAL, "null");
+    out[0] = s->staging.captures;
+    out[1] = s->staging.replays;
+    return MPMW_OK;
+}
+
 int mpmw_set_deterministic(mpmw_solver *s, int32_t on) {
     CHECK_S(s);
     if (on && s->apic) return fail(MPMW_E_STATE, "the deterministic mode is not available with the APIC transfer");
@@ -2246,8 +2281,17 @@ template <typename R> int loose_after_batch(mpmw_solver *s, double dt, const Ste
                : loose_pass_t<R, true, true, false, false>(s, dt, sp.zcut, sp.fl);
 }
 
+void drop_graphs(StepStaging &st) {
+    for (int k = 0; k < 2; ++k) {
+        if (st.exec[k]) (void)hipGraphExecDestroy(st.exec[k]);
+        st.exec[k] = nullptr;
+        st.sig[k].clear();
+    }
+}
+
 // Free the staging areas of a solver or a batch.
 void free_staging(StepStaging &st) {
+    drop_graphs(st);
     for (auto &a : st.area) {
         if (a.copied) (void)hipEventDestroy(a.copied);
         pool::put(a.h);
@@ -2294,6 +2338,79 @@ template <typename Fill> int stage_blob(StepStaging &st, hipStream_t stream, siz
     HIP_TRY(hipEventRecord(A.copied, stream));
     A.pending = true;
     *dev = (char *)st.d_blob;
+    return MPMW_OK;
+}
+
+// The step's launches as a HIP graph: `fill` writes the blob into the next
+// pinned area, then the graph of that area (its copy into the device blob
+// and the kernels `launch` enqueues on the blob) is launched, captured first
+// when its signature (blob size and addresses, grid sizes; whatever the
+// kernels' arguments hold) differs from the one it was captured with.  The
+// per-step values (dt, tool pose, flags) travel in the blob, so a replay
+// computes exactly what the eager launches compute.
+template <typename Fill, typename Launch>
+int graph_step(StepStaging &st, hipStream_t stream, size_t end, std::vector<uint64_t> sig, Fill fill,
+               Launch launch) {
+    const int k = st.parity;
+    StepStaging::Area &A = st.area[k];
+    st.parity ^= 1;
+    if (!A.copied) HIP_TRY(hipEventCreateWithFlags(&A.copied, hipEventDisableTiming | hipEventDisableSystemFence));
+    if (A.pending) {
+        HIP_TRY(hipEventSynchronize(A.copied));
+        A.pending = false;
+    }
+    int dev_id = 0;
+    HIP_TRY(hipGetDevice(&dev_id));
+    if (end > A.cap) {
+        if (st.exec[k]) {
+            (void)hipGraphExecDestroy(st.exec[k]);
+            st.exec[k] = nullptr;
+        }
+        pool::put(A.h);
+        A.h = nullptr;
+        A.cap = 0;
+        HIP_TRY(pool::get(dev_id, true, end * 2, &A.h));
+        A.cap = end * 2;
+    }
+    if (end > st.d_cap) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        drop_graphs(st);
+        pool::put(st.d_blob);
+        st.d_blob = nullptr;
+        st.d_cap = 0;
+        HIP_TRY(pool::get(dev_id, false, end * 2, &st.d_blob));
+        st.d_cap = end * 2;
+    }
+    fill((char *)A.h);
+    sig.push_back((uint64_t)end);
+    sig.push_back((uint64_t)(uintptr_t)A.h);
+    sig.push_back((uint64_t)(uintptr_t)st.d_blob);
+    if (!st.exec[k] || st.sig[k] != sig) {
+        if (st.exec[k]) HIP_TRY(hipGraphExecDestroy(st.exec[k]));
+        st.exec[k] = nullptr;
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipMemcpyAsync(st.d_blob, A.h, end, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) {
+            launch((char *)st.d_blob);
+            e = hipGetLastError();
+        }
+        const hipError_t e2 = hipStreamEndCapture(stream, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&st.exec[k], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            st.exec[k] = nullptr;
+            return fail(MPMW_E_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+        }
+        st.sig[k] = std::move(sig);
+        st.captures++;
+    } else {
+        st.replays++;
+    }
+    HIP_TRY(hipGraphLaunch(st.exec[k], stream));
+    HIP_TRY(hipEventRecord(A.copied, stream));
+    A.pending = true;
     return MPMW_OK;
 }
 
@@ -2495,32 +2612,49 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
             off_tw[v] = end;
             end += align256(sizeof(TileWork<R>) * tw[v].size());
         }
-        char *d = nullptr;
-        if ((rc = stage_blob(st, stream, end, [&](char *h) {
-                 std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
-                 std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
-                 for (int v = 0; v < kTileVariants; ++v)
-                     if (!tw[v].empty())
-                         std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
-             }, &d)))
-            return rc;
-        const GridWork<R> *d_gw = (const GridWork<R> *)d;
-        {
-            Timed t(one, MPMW_K_GRID);
-            hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
-        }
-        {
-            // diffusion, contact-force sums and flag resets in one launch
-            Timed t(one, MPMW_K_DIFFUSE);
-            hipLaunchKernelGGL(k_post_batch<R>, dim3(dblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
-                               (const SumWork *)(d + off_sw), dblocks);
-        }
-        {
-            Timed t(one, MPMW_K_G2P_P2G);
+        auto fill = [&](char *h) {
+            std::memcpy(h, gw.data(), sizeof(GridWork<R>) * nr);
+            std::memcpy(h + off_sw, sw.data(), sizeof(SumWork) * nr);
             for (int v = 0; v < kTileVariants; ++v)
-                if (!tw[v].empty())
-                    launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
-                                         tblocks[v]);
+                if (!tw[v].empty()) std::memcpy(h + off_tw[v], tw[v].data(), sizeof(TileWork<R>) * tw[v].size());
+        };
+        // the kernels of the step, reading their work items from blob d;
+        // with `time_it` each class is bracketed by the solver's timing events
+        auto launch = [&](char *d, bool time_it) {
+            mpmw_solver *tm = time_it ? one : nullptr;
+            const GridWork<R> *d_gw = (const GridWork<R> *)d;
+            {
+                Timed t(tm, MPMW_K_GRID);
+                hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
+            }
+            {
+                // diffusion, contact-force sums and flag resets in one launch
+                Timed t(tm, MPMW_K_DIFFUSE);
+                hipLaunchKernelGGL(k_post_batch<R>, dim3(dblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
+                                   (const SumWork *)(d + off_sw), dblocks);
+            }
+            {
+                Timed t(tm, MPMW_K_G2P_P2G);
+                for (int v = 0; v < kTileVariants; ++v)
+                    if (!tw[v].empty())
+                        launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
+                                             tblocks[v]);
+            }
+        };
+        // kernel timing needs events between the launches: eager then
+        if (st.graph_on && !(one && one->timing)) {
+            std::vector<uint64_t> sig = {(uint64_t)nr, (uint64_t)gblocks, (uint64_t)dblocks, (uint64_t)off_sw};
+            for (int v = 0; v < kTileVariants; ++v) {
+                sig.push_back((uint64_t)tw[v].size());
+                sig.push_back((uint64_t)tblocks[v]);
+                sig.push_back((uint64_t)off_tw[v]);
+            }
+            if ((rc = graph_step(st, stream, end, std::move(sig), fill, [&](char *d) { launch(d, false); })))
+                return rc;
+        } else {
+            char *d = nullptr;
+            if ((rc = stage_blob(st, stream, end, fill, &d))) return rc;
+            launch(d, true);
         }
     }
     HIP_TRY(hipGetLastError());
@@ -2616,6 +2750,22 @@ int mpmw_batch_step(mpmw_batch *b, const double *dt, int32_t n) {
         if (!std::isfinite(dt[i])) return fail(MPMW_E_INVAL, "batch_step: dt must be finite");
     return b->fp64 ? step_many<double>(b->members.data(), b->members.size(), dt, b->stream, b->staging)
                    : step_many<float>(b->members.data(), b->members.size(), dt, b->stream, b->staging);
+}
+
+int mpmw_batch_set_graph(mpmw_batch *b, int32_t on) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->staging.graph_on = on != 0;
+    if (!on) drop_graphs(b->staging);
+    return MPMW_OK;
+}
+
+int mpmw_batch_graph_stats(mpmw_batch *b, int64_t out[2]) {
+    if (!b || !out) return fail(MPMW_E_INVAL, "null");
+    out[0] = b->staging.captures;
+    out[1] = b->staging.replays;
+    return MPMW_OK;
 }
 
 int mpmw_batch_synchronize(mpmw_batch *b) {

@@ -37,7 +37,8 @@ EXPORTS = (
     "mpmw_batch_create", "mpmw_batch_destroy", "mpmw_batch_add", "mpmw_batch_remove", "mpmw_batch_count",
     "mpmw_batch_step", "mpmw_batch_synchronize", "mpmw_set_tool_sdf_mesh", "mpmw_get_tool_sdf",
     "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
-    "mpmw_export_ndjson_noalloc", "mpmw_morph_tool_sdf", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
+    "mpmw_export_ndjson_noalloc", "mpmw_morph_tool_sdf", "mpmw_set_graph", "mpmw_graph_stats",
+    "mpmw_batch_set_graph", "mpmw_batch_graph_stats", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
 )
 
 
@@ -200,6 +201,10 @@ def lib():
         "mpmw_batch_count": (C.c_int, [vp, _ip]),
         "mpmw_batch_step": (C.c_int, [vp, _dp, C.c_int32]),
         "mpmw_batch_synchronize": (C.c_int, [vp]),
+        "mpmw_set_graph": (C.c_int, [vp, C.c_int32]),
+        "mpmw_graph_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mpmw_batch_set_graph": (C.c_int, [vp, C.c_int32]),
+        "mpmw_batch_graph_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -457,6 +462,16 @@ class Solver:
         t = np.ascontiguousarray(t, dtype=np.float64)
         _check(lib().mpmw_set_particle_temperatures(self.h, _p(t), len(t)), "set_particle_temperatures")
 
+    def set_graph(self, on):
+        """Replay the step's launches as a HIP graph (captured anew when the
+        launch configuration changes)."""
+        _check(lib().mpmw_set_graph(self.h, 1 if on else 0), "set_graph")
+
+    def graph_stats(self):
+        v = (C.c_int64 * 2)()
+        _check(lib().mpmw_graph_stats(self.h, v), "graph_stats")
+        return {"captures": v[0], "replays": v[1]}
+
     def set_deterministic(self, on):
         _check(lib().mpmw_set_deterministic(self.h, 1 if on else 0), "set_deterministic")
 
@@ -548,3 +563,11 @@ class Batch:
 
     def synchronize(self):
         _check(lib().mpmw_batch_synchronize(self.h), "batch_synchronize")
+
+    def set_graph(self, on):
+        _check(lib().mpmw_batch_set_graph(self.h, 1 if on else 0), "batch_set_graph")
+
+    def graph_stats(self):
+        v = (C.c_int64 * 2)()
+        _check(lib().mpmw_batch_graph_stats(self.h, v), "batch_graph_stats")
+        return {"captures": v[0], "replays": v[1]}

@@ -249,6 +249,17 @@ int mpmw_set_particle_temperatures(mpmw_solver *s, const double *t, int32_t coun
  * cell since the last binning.  Takes effect at the next step. */
 int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
 
+/* HIP graph of the step (no counterpart in the reference solver; its
+ * MPMSolver records its step as a CUDA graph, MPMSolver.cu:1077-1157): the
+ * copy of the step's work items and the step's kernels are replayed as one
+ * graph while their launch configuration stays the same (captured again
+ * after a re-binning changes it).  The per-step values travel in the copied
+ * work items, so results are bit for bit those of the eager launches.  Off
+ * by default; kernel timing (mpmw_set_kernel_timing) steps eagerly.
+ * mpmw_graph_stats: captures and replays so far. */
+int mpmw_set_graph(mpmw_solver *s, int32_t on);
+int mpmw_graph_stats(mpmw_solver *s, int64_t out[2]);
+
 /* --- APIC transfer variant: quadratic B-spline weights over 27 nodes with
  *     base = floor(x/h - 0.5) (getWeightsAndGradients, MPMSolver.cu:20-34),
  *     P2G momentum m w (v + C (x_i - x_p)) with none from FIXED_BOUNDARY
@@ -351,6 +362,10 @@ int mpmw_batch_count(mpmw_batch *b, int32_t *n);
  * order they were added (removals close the gaps). */
 int mpmw_batch_step(mpmw_batch *b, const double *dt, int32_t n);
 int mpmw_batch_synchronize(mpmw_batch *b);
+/* mpmw_set_graph / mpmw_graph_stats for the batch's step (the graph holds
+ * every member's launches; a member joining or leaving captures anew). */
+int mpmw_batch_set_graph(mpmw_batch *b, int32_t on);
+int mpmw_batch_graph_stats(mpmw_batch *b, int64_t out[2]);
 
 /* Layout diagnostics of the particle binning. */
 typedef struct mpmw_diagnostics {

@@ -151,6 +151,12 @@ def test_mixed_status_chip_fracture_external_force(prec):
     print(f"  shadow oracle: steps with a status mismatch {mism} worst "
           + " ".join(f"{k}={v:.2e}" for k, v in sorted(worst.items())), flush=True)
     tol = 1e-9 if prec == "fp64" else 1e-5
+    # every step of the GPU's own trajectory takes the reference step's
+    # statuses and values; over the 26 steps the FP32 atomics' order may
+    # still move a particle across a status threshold (up to 2 allowed)
+    assert not mism
+    for k, v in worst.items():
+        assert v <= tol, ("shadow", k, v)
     assert res["status_mismatch"] <= (0 if prec == "fp64" else 2)
     for k in GATED:
         assert res["errors"][k] <= tol, (k, res["errors"][k])
