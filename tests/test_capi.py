@@ -28,6 +28,41 @@ def test_library_exports_every_declared_symbol():
     assert L.mpmw_abi_version() == 1
 
 
+def test_ftrack_library_exports_every_declared_symbol():
+    """include/mpms.h (the F-tracking solver): every declared entry point is
+    exported and bound."""
+    import ftrack
+    src = open(os.path.join(ROOT, "include", "mpms.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(mpms_[a-z0-9_]+)\s*\(", src)))
+    L = ftrack.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(ftrack.EXPORTS)
+
+
+def test_ftrack_params_default_match_oracle_defaults():
+    import ftrack
+    import mpms as orc_ms
+    p, o = ftrack.default_params(), orc_ms.default_params()
+    for name, _ in orc_ms.Params._fields_:
+        a, b = getattr(p, name), getattr(o, name)
+        if hasattr(a, "__len__"):
+            assert list(a) == list(b), name
+        else:
+            assert a == b, name
+    assert p.precision == ftrack.FP32 and p.island_det == 0.5 and p.newton_fp64 == 1
+
+
+def test_ftrack_create_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    import ftrack
+    with pytest.raises(mpmw.MPMError):
+        ftrack.Solver()
+
+
 def test_params_default_match_oracle_defaults(orc):
     p = mpmw.default_params()
     o = orc.default_params()
