@@ -260,6 +260,67 @@ def batched_bench(cfgs, precision, steps, graph=False):
     }
 
 
+# F-tracking step (the reference's MPMSolver), FP32 bulk: bytes a particle
+# must move per step.  P2G reads x (24), status + LOD (8), m, V (8), v (12),
+# f_ext (12), C (36), T (4), F - I (36), Fp - I (36), backstress (24), eps_p,
+# rate, damage, residual (16) = 216 and writes stress (24), eps_p, T, damage
+# (12), Fp - I (36), status + LOD (8) = 80; G2P reads x (24), status + LOD
+# (8), F - I (36), stress and damage for the chip pass (28) = 96 and writes x
+# (24), v (12), C (36), F - I (36), rate (4), f_ext (12) = 124.  The grid
+# (one node per 8 particles: 7 channels cleared, accumulated, read and 10
+# written by the update, 3 read by G2P, 4 B each) adds ~12 B.
+B_FTRACK = 216.0 + 80.0 + 96.0 + 124.0 + 12.0
+
+
+def ftrack_bench(steps, warmup, dims=(512, 256, 128)):
+    """The F-tracking step on a slab of particles at spacing dx / 2 (8 per
+    cell of the reference's dx = 0.5 mm lattice), FP32 bulk with FP64
+    islands, pressed by a rasterised cylindrical tool block (radius 5 mm,
+    nodes every 0.5 mm) moving down at 5 m/s whose nodes sit in contact."""
+    import ftrack
+    sp = 2.5e-4
+    mn = (0.0, 0.0, 0.0)
+    mx = tuple((d - 1) * sp for d in dims)
+    P = ftrack.default_params(jc_A=1098e6, jc_B=1092e6, jc_n=0.93, jc_C=0.014, jc_m=1.1, young=113.8e9,
+                              poisson=0.342, density=4430.0, specific_heat=526.0, melt_temp=1878.0,
+                              ambient_temp=293.0, fixture_thickness=0.0, lod_enabled=1,
+                              lod_active_radius=0.006, lod_near_radius=0.025, max_particles=2 ** 31 - 1)
+    s = ftrack.Solver(P)
+    t0 = time.time()
+    s.init_box(mn, mx, sp)
+    setup = time.time() - t0
+    n = s.count()
+    cx, cy, top = 0.5 * mx[0], 0.5 * mx[1], mx[2]
+    g = np.arange(-10, 11) * 5e-4
+    pts = np.array([(cx + a, cy + b, top + 0.2e-3 + c) for a in g for b in g for c in np.arange(0, 20) * 5e-4
+                    if a * a + b * b <= 25e-6])
+    s.set_tool_nodes(pts, np.tile([0.0, 0.0, -5.0], (len(pts), 1)))
+    s.set_tool_position(cx, cy, top)
+    dt = 1e-8
+    for _ in range(warmup):
+        s.step(dt)
+    s.synchronize()
+    ev = HipEvents()
+    a, b = ev.create(), ev.create()
+    stream = s.stream()
+    ev.record(a, stream)
+    for _ in range(steps):
+        s.step(dt)
+    ev.record(b, stream)
+    ms = ev.elapsed_ms(a, b)
+    d = s.diagnostics()
+    m = s.metrics()
+    s.close()
+    pps = n * steps / (ms / 1e3)
+    return {"workload": f"slab {dims[0]}x{dims[1]}x{dims[2]} particles at 0.25 mm (8 per 0.5 mm cell), "
+                        f"{len(pts)} tool nodes pressing at 5 m/s, FP32 with FP64 islands, dt 1e-8",
+            "particles": n, "steps": steps, "warmup": warmup, "ms_per_step": ms / steps,
+            "value": pps, "unit": "particle-steps/s", "setup_s": setup,
+            "roofline": {"bound": "hbm", "B_alg_per_particle_step": B_FTRACK,
+                         "achieved_GBs": pps * B_FTRACK / 1e9, "frac": pps * B_FTRACK / 1e9 / HBM_PEAK_GBS},
+            "max_plastic_strain_proxy_max_stress": m["max_stress"], "diagnostics": d}
+
+
 def measured_traffic():
     """HBM bytes per launch of the fused tile kernel on this workload, from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
@@ -307,6 +368,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 timing of the same workload")
     ap.add_argument("--graph", type=int, default=1, help="1: also time the step replayed as HIP graphs")
+    ap.add_argument("--ftrack-steps", type=int, default=50,
+                    help="steps of the F-tracking step's timing (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--batch-count", type=int, default=64, help="workpieces of the batched run (0: skip it)")
@@ -414,6 +477,8 @@ def main():
             g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True)
             out["batched"]["graph"] = {k: g[k] for k in ("serial_s", "batched_s", "speedup", "graph",
                                                          "batched_sims_per_hour")}
+    if args.ftrack_steps > 0:
+        out["ftrack"] = ftrack_bench(args.ftrack_steps, 5)
     if args.serve_seconds > 0:
         sv = {"workload": f"Poisson stream of 20 requests/s for {args.serve_seconds:g} s, sizes and parameters of "
                           "the batched distribution, 200-2000 steps each; a request's result is the NDJSON "

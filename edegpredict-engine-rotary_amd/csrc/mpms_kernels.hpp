@@ -401,42 +401,83 @@ __device__ __forceinline__ void constitutive(const Particles<R> &P, int i, const
 }
 
 // p2gKernel (MPMSolver.cu:124-412) with the LOD classification of the step
-// (lodClassifyKernel :655-676) folded in: one thread per particle.
+// (lodClassifyKernel :655-676) folded in: one thread per particle.  The
+// particles are sorted by their stencil's base node, so consecutive lanes of
+// a wave mostly share a base node and hence all 27 target nodes: each of a
+// node's 7 channel values is summed over each run of equal base nodes in the
+// wave (a segmented suffix sum by shuffles, log2 64 steps) and only the
+// run's first lane issues the device atomics, ~8 times fewer than one per
+// particle and without lanes of a wave contending for one address.  A run
+// ends wherever the base node changes, so an unsorted order gives the same
+// sums (in another order), only with more atomics.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R> M, StepArgs sa) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
-    int32_t st = P.status[i];
-    if (st == kInactive) return;
-    const double x = P.x[i], y = P.y[i], z = P.z[i];
-    int32_t lod = P.lod[i];
-    if (sa.lod_enabled && (st == kActive || st == kChip)) {
-        const double dx = x - sa.tool[0], dy = y - sa.tool[1], dz = z - sa.tool[2];
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        lod = d <= sa.lod_active ? kLodActive : (d <= sa.lod_near ? kLodNear : kLodFar);
-        P.lod[i] = lod;
+    const int lane = threadIdx.x & 63;
+    bool live = i < P.n;
+    int32_t st = live ? P.status[i] : kInactive;
+    live = live && st != kInactive;
+    double x = 0, y = 0, z = 0;
+    if (live) {
+        x = P.x[i]; y = P.y[i]; z = P.z[i];
+        int32_t lod = P.lod[i];
+        if (sa.lod_enabled && (st == kActive || st == kChip)) {
+            const double dx = x - sa.tool[0], dy = y - sa.tool[1], dz = z - sa.tool[2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            lod = d <= sa.lod_active ? kLodActive : (d <= sa.lod_near ? kLodNear : kLodFar);
+            P.lod[i] = lod;
+        }
+        const int cx = (int)floor((x - g.ox) * g.inv_dx), cy = (int)floor((y - g.oy) * g.inv_dx),
+                  cz = (int)floor((z - g.oz) * g.inv_dx);
+        if (cx < 1 || cx >= g.dims[0] - 2 || cy < 1 || cy >= g.dims[1] - 2 || cz < 1 || cz >= g.dims[2] - 2) {
+            P.status[i] = kInactive;
+            live = false;
+        }
     }
-    const int cx = (int)floor((x - g.ox) * g.inv_dx), cy = (int)floor((y - g.oy) * g.inv_dx),
-              cz = (int)floor((z - g.oz) * g.inv_dx);
-    if (cx < 1 || cx >= g.dims[0] - 2 || cy < 1 || cy >= g.dims[1] - 2 || cz < 1 || cz >= g.dims[2] - 2) {
-        P.status[i] = kInactive;
-        return;
-    }
-    const bool fixed = st == kFixed;
-    bool skip = lod == kLodFar || fixed || st == kChip;
+    const bool fixed = live && st == kFixed;
+    bool skip = true;
     R sig[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) sig[c] = R(0);
     R J = R(1);
-    if (!skip && st == kActive) constitutive<R>(P, i, M, sig, J, skip, sa.islands);
-    const Axis<R> ax = axis_weights<R>(x - g.ox, g.inv_dx), ay = axis_weights<R>(y - g.oy, g.inv_dx),
-                  az = axis_weights<R>(z - g.oz, g.inv_dx);
-    const R m = P.m[i], VJ = P.V[i] * J;
-    const R vx = P.v[0][i], vy = P.v[1][i], vz = P.v[2][i];
-    const R efx = P.ef[0][i], efy = P.ef[1][i], efz = P.ef[2][i];
-    R C[9];
+    Axis<R> ax, ay, az;
+    R m = 0, VJ = 0, vx = 0, vy = 0, vz = 0, efx = 0, efy = 0, efz = 0, C[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) C[c] = P.C[c][i];
+    for (int c = 0; c < 9; ++c) C[c] = R(0);
+    int key = -1 - lane;   // a run of its own unless live
+    if (live) {
+        skip = P.lod[i] == kLodFar || fixed || st == kChip;
+        if (!skip && st == kActive) constitutive<R>(P, i, M, sig, J, skip, sa.islands);
+        ax = axis_weights<R>(x - g.ox, g.inv_dx);
+        ay = axis_weights<R>(y - g.oy, g.inv_dx);
+        az = axis_weights<R>(z - g.oz, g.inv_dx);
+        m = P.m[i];
+        VJ = P.V[i] * J;
+        vx = P.v[0][i]; vy = P.v[1][i]; vz = P.v[2][i];
+        efx = P.ef[0][i]; efy = P.ef[1][i]; efz = P.ef[2][i];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) C[c] = P.C[c][i];
+        key = (az.b * g.dims[1] + ay.b) * g.dims[0] + ax.b;
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { ax.w[c] = ay.w[c] = az.w[c] = R(0); ax.dw[c] = ay.dw[c] = az.dw[c] = R(0); }
+        ax.b = ay.b = az.b = 0;
+    }
+    // the run structure: a run starts where the base node changes and ends
+    // before the next start; same[k] when lane + 2^k is in this lane's run
+    // (contiguity matters: an equal key after a different one is another
+    // run, with its own head)
+    const int prev = __shfl_up(key, 1, 64);
+    const bool starts = lane == 0 || prev != key;
+    const unsigned long long starts_mask = __ballot(starts);
+    const unsigned long long later = lane == 63 ? 0ull : starts_mask & (~0ull << (lane + 1));
+    const int run_end = later ? (int)__builtin_ctzll(later) : 64;
+    bool same[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) same[k] = lane + (1 << k) < run_end;
+    const bool head = live && starts;
+    const R pvx = fixed ? R(0) : vx, pvy = fixed ? R(0) : vy, pvz = fixed ? R(0) : vz;
+    const R mm = fixed ? R(0) : m;   // FIXED_BOUNDARY particles add no momentum
     int missed = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -445,30 +486,47 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int gx = ax.b + a, gy = ay.b + b, gz = az.b + c;
-                const int64_t n = box_index(g, gx, gy, gz);
-                if (n < 0) { ++missed; continue; }
                 const R w = ax.w[a] * ay.w[b] * az.w[c];
                 const R wgx = ax.dw[a] * ay.w[b] * az.w[c];
                 const R wgy = ax.w[a] * ay.dw[b] * az.w[c];
                 const R wgz = ax.w[a] * ay.w[b] * az.dw[c];
                 const R dpx = (R)((gx * g.dx + g.ox) - x), dpy = (R)((gy * g.dx + g.oy) - y),
                         dpz = (R)((gz * g.dx + g.oz) - z);
-                const R wm = w * m;
-                unsafeAtomicAdd(g.mass + n, wm);
-                if (!fixed) {
-                    unsafeAtomicAdd(g.p[0] + n, wm * (vx + C[0] * dpx + C[1] * dpy + C[2] * dpz));
-                    unsafeAtomicAdd(g.p[1] + n, wm * (vy + C[3] * dpx + C[4] * dpy + C[5] * dpz));
-                    unsafeAtomicAdd(g.p[2] + n, wm * (vz + C[6] * dpx + C[7] * dpy + C[8] * dpz));
-                }
-                R fx = w * efx, fy = w * efy, fz = w * efz;
+                const R wm = w * mm;
+                R ch[7];
+                ch[0] = w * m;
+                ch[1] = wm * (pvx + C[0] * dpx + C[1] * dpy + C[2] * dpz);
+                ch[2] = wm * (pvy + C[3] * dpx + C[4] * dpy + C[5] * dpz);
+                ch[3] = wm * (pvz + C[6] * dpx + C[7] * dpy + C[8] * dpz);
+                ch[4] = w * efx; ch[5] = w * efy; ch[6] = w * efz;
                 if (!skip) {
-                    fx += -VJ * (sig[0] * wgx + sig[1] * wgy + sig[2] * wgz);
-                    fy += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
-                    fz += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
+                    ch[4] += -VJ * (sig[0] * wgx + sig[1] * wgy + sig[2] * wgz);
+                    ch[5] += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
+                    ch[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
                 }
-                unsafeAtomicAdd(g.f[0] + n, fx);
-                unsafeAtomicAdd(g.f[1] + n, fy);
-                unsafeAtomicAdd(g.f[2] + n, fz);
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) {
+                        const R o = __shfl_down(ch[q], 1 << k, 64);
+                        if (same[k]) ch[q] += o;
+                    }
+                if (head) {
+                    const int64_t n = box_index(g, gx, gy, gz);
+                    if (n < 0) {
+                        ++missed;
+                    } else {
+                        unsafeAtomicAdd(g.mass + n, ch[0]);
+                        if (ch[1] != R(0) || ch[2] != R(0) || ch[3] != R(0)) {
+                            unsafeAtomicAdd(g.p[0] + n, ch[1]);
+                            unsafeAtomicAdd(g.p[1] + n, ch[2]);
+                            unsafeAtomicAdd(g.p[2] + n, ch[3]);
+                        }
+                        unsafeAtomicAdd(g.f[0] + n, ch[4]);
+                        unsafeAtomicAdd(g.f[1] + n, ch[5]);
+                        unsafeAtomicAdd(g.f[2] + n, ch[6]);
+                    }
+                }
             }
     if (missed) atomicAdd(g.overflow, (unsigned long long)missed);
 }

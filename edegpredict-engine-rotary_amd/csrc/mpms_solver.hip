@@ -499,6 +499,11 @@ int mpms_create(const mpms_params *p, mpms_solver **out) {
         if (d >= 2147483647.0) { delete s; return fail(MPMW_E_RANGE, "lattice beyond 2^31 nodes per axis"); }
         s->dims[a] = (int32_t)d;
     }
+    // node indices of the whole lattice are 32-bit (the P2G's run keys)
+    if ((int64_t)s->dims[0] * s->dims[1] * s->dims[2] >= ((int64_t)1 << 31)) {
+        delete s;
+        return fail(MPMW_E_RANGE, "lattice of 2^31 nodes or more: shrink the domain or grow dx");
+    }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&s->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&s->d_part, sizeof(double) * kStatBlocks * kStats) != hipSuccess) {
