@@ -60,6 +60,14 @@ struct StepArgs {
     unsigned long long *islands;   // [0] kinematics, [1] Newton
 };
 
+// The step's scalars live in device memory, written by this one-thread
+// kernel before the step's kernels: the kernels that read them can then be
+// replayed from a captured graph while dt, the tool position and the chip
+// step change.
+__global__ void k_step_args(StepArgs *__restrict__ d, StepArgs v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d = v;
+}
+
 // ------------------------------------------------------------ matrices ---
 
 template <typename K> __device__ __forceinline__ K det3(const K A[9]) {
@@ -411,7 +419,8 @@ __device__ __forceinline__ void constitutive(const Particles<R> &P, int i, const
 // ends wherever the base node changes, so an unsorted order gives the same
 // sums (in another order), only with more atomics.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R> M, StepArgs sa) {
+__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R> M, const StepArgs *__restrict__ sap) {
+    const StepArgs sa = *sap;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const int lane = threadIdx.x & 63;
     bool live = i < P.n;
@@ -472,9 +481,12 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     const unsigned long long starts_mask = __ballot(starts);
     const unsigned long long later = lane == 63 ? 0ull : starts_mask & (~0ull << (lane + 1));
     const int run_end = later ? (int)__builtin_ctzll(later) : 64;
-    bool same[6];
+    bool same[6], need[6];   // need[k]: some lane of the wave adds at distance 2^k (wave-uniform)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) same[k] = lane + (1 << k) < run_end;
+    for (int k = 0; k < 6; ++k) {
+        same[k] = lane + (1 << k) < run_end;
+        need[k] = __ballot(same[k]) != 0ull;
+    }
     const bool head = live && starts;
     const R pvx = fixed ? R(0) : vx, pvy = fixed ? R(0) : vy, pvz = fixed ? R(0) : vz;
     const R mm = fixed ? R(0) : m;   // FIXED_BOUNDARY particles add no momentum
@@ -505,12 +517,14 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                     ch[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
                 }
 #pragma unroll
-                for (int k = 0; k < 6; ++k)
+                for (int k = 0; k < 6; ++k) {
+                    if (!need[k]) continue;
 #pragma unroll
                     for (int q = 0; q < 7; ++q) {
                         const R o = __shfl_down(ch[q], 1 << k, 64);
                         if (same[k]) ch[q] += o;
                     }
+                }
                 if (head) {
                     const int64_t n = box_index(g, gx, gy, gz);
                     if (n < 0) {
@@ -549,7 +563,8 @@ template <typename R> __global__ void __launch_bounds__(kBlock) k_raster(Grid<R>
 
 // updateGridKernel (:424-442), applyFixtureBCKernel (:81-111) and
 // applyGridCollisionKernel (:475-492) on one node of the sub-box.
-template <typename R> __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g, StepArgs sa) {
+template <typename R> __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g, const StepArgs *__restrict__ sap) {
+    const StepArgs sa = *sap;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (n >= g.nn) return;
     const R m = g.mass[n];
@@ -591,7 +606,8 @@ template <typename R> __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g
 // steps, MPMSolver's chipFractureAndSeparationKernel (:1007-1060, critical
 // damage 1e-6, cut depth 1.5 dx) on one particle.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, StepArgs sa, R dth) {
+__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, const StepArgs *__restrict__ sap, R dth) {
+    const StepArgs sa = *sap;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= P.n) return;
     const int32_t st = P.status[i];

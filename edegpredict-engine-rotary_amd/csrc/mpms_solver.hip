@@ -78,6 +78,16 @@ struct mpms_solver {
     int32_t *idx = nullptr, *idx2 = nullptr;
     int32_t sort_cap = 0;
     mpmw::ScratchBuf temp;
+    // the step's scalars on the device (k_step_args) and its captured
+    // launches: one graph per launch configuration, the two most recent kept
+    // (the fields swap with their sorted copies at every fit)
+    StepArgs *d_sa = nullptr;
+    bool graph_on = false;
+    struct Captured {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    } graphs[2];
+    int64_t captures = 0, replays = 0;
 };
 
 namespace {
@@ -391,6 +401,68 @@ template <typename R> int fit(mpms_solver *s) {
     return MPMW_OK;
 }
 
+void drop_graphs(mpms_solver *s) {
+    for (auto &c : s->graphs) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        c.exec = nullptr;
+        c.key.clear();
+    }
+}
+
+// The launch configuration a captured step bakes in: sizes, grid box and
+// every device address its kernels take by value.
+std::vector<uint64_t> graph_key(const mpms_solver *s) {
+    std::vector<uint64_t> k = {(uint64_t)s->n, (uint64_t)s->nn, (uint64_t)s->nn_cap, (uint64_t)s->ntool,
+                               (uint64_t)(uintptr_t)s->gblock, (uint64_t)(uintptr_t)s->d_tool_xyz,
+                               (uint64_t)(uintptr_t)s->d_tool_vel, (uint64_t)(uintptr_t)s->d_sa};
+    for (int a = 0; a < 3; ++a) k.push_back(((uint64_t)(uint32_t)s->lo[a] << 32) | (uint32_t)s->nb[a]);
+    for (int f = 0; f < NFIELD; ++f) k.push_back((uint64_t)(uintptr_t)s->f[f]);
+    return k;
+}
+
+// Replay the step's launches from a graph captured for this configuration,
+// capturing it first when there is none (the older of two is replaced).
+template <typename Issue> int graph_step(mpms_solver *s, Issue &&issue) {
+    std::vector<uint64_t> key = graph_key(s);
+    mpms_solver::Captured *hit = nullptr;
+    for (auto &c : s->graphs)
+        if (c.exec && c.key == key) hit = &c;
+    if (!hit) {
+        hit = &s->graphs[s->graphs[0].exec && !s->graphs[1].exec ? 1 : 0];
+        if (s->graphs[0].exec && s->graphs[1].exec) {
+            // keep the more recent one in slot 1
+            std::swap(s->graphs[0], s->graphs[1]);
+            hit = &s->graphs[1];
+        }
+        if (hit->exec) (void)hipGraphExecDestroy(hit->exec);
+        hit->exec = nullptr;
+        hit->key.clear();
+        MS_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        issue();
+        hipGraph_t graph = nullptr;
+        const hipError_t launch_err = hipGetLastError();
+        const hipError_t end_err = hipStreamEndCapture(s->stream, &graph);
+        if (launch_err != hipSuccess || end_err != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            return fail(MPMW_E_HIP, std::string("F-tracking step capture failed: ") +
+                                        hipGetErrorString(launch_err != hipSuccess ? launch_err : end_err));
+        }
+        const hipError_t inst = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+            hit->exec = nullptr;
+            (void)hipGetLastError();
+            return fail(MPMW_E_HIP, std::string("F-tracking step graph instantiation failed: ") + hipGetErrorString(inst));
+        }
+        hit->key = std::move(key);
+        s->captures++;
+    }
+    MS_TRY(hipGraphLaunch(hit->exec, s->stream));
+    s->replays++;
+    return MPMW_OK;
+}
+
 template <typename R> int step_impl(mpms_solver *s, double dt) {
     if (s->n <= 0) return MPMW_OK;   // MPMSolver::step :1063 silent return
     int rc;
@@ -410,21 +482,33 @@ template <typename R> int step_impl(mpms_solver *s, double dt) {
     sa.gravity = -9.81;
     sa.tool_vel = s->d_tool_vel;
     sa.islands = s->d_counts + 1;
+    hipLaunchKernelGGL(k_step_args, dim3(1), dim3(64), 0, s->stream, s->d_sa, sa);
+    const StepArgs *dsa = s->d_sa;
     // resetGridKernel (:113-122): the node channels and tool claims, and the
-    // island counters of this step
+    // island counters of this step (fill kernels, no memset nodes, so the
+    // sequence can be captured)
     const int64_t words = s->nn_cap * 7 * (int64_t)(s->rsz / 4);
-    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(blocks(words), 4096)), dim3(kBlock), 0, s->stream,
-                       (uint32_t *)s->gblock, words);
-    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(blocks(s->nn), 2048)), dim3(kBlock), 0, s->stream,
-                       (uint32_t *)g.claim, s->nn);
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBlock), 0, s->stream, (uint32_t *)(s->d_counts + 1), (int64_t)4);
-    hipLaunchKernelGGL(k_p2g<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, material<R>(s), sa);
-    if (s->ntool > 0)
-        hipLaunchKernelGGL(k_raster<R>, dim3(blocks(s->ntool)), dim3(kBlock), 0, s->stream, g,
-                           (const double *)s->d_tool_xyz, s->ntool);
-    hipLaunchKernelGGL(k_grid<R>, dim3(blocks(s->nn)), dim3(kBlock), 0, s->stream, g, sa);
-    hipLaunchKernelGGL(k_g2p<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, sa, (R)s->prm.damage_threshold);
-    MS_TRY(hipGetLastError());
+    auto issue = [&]() {
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(blocks(words), 4096)), dim3(kBlock), 0,
+                           s->stream, (uint32_t *)s->gblock, words);
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(blocks(s->nn), 2048)), dim3(kBlock), 0,
+                           s->stream, (uint32_t *)g.claim, s->nn);
+        hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBlock), 0, s->stream, (uint32_t *)(s->d_counts + 1),
+                           (int64_t)4);
+        hipLaunchKernelGGL(k_p2g<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, material<R>(s), dsa);
+        if (s->ntool > 0)
+            hipLaunchKernelGGL(k_raster<R>, dim3(blocks(s->ntool)), dim3(kBlock), 0, s->stream, g,
+                               (const double *)s->d_tool_xyz, s->ntool);
+        hipLaunchKernelGGL(k_grid<R>, dim3(blocks(s->nn)), dim3(kBlock), 0, s->stream, g, dsa);
+        hipLaunchKernelGGL(k_g2p<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, dsa,
+                           (R)s->prm.damage_threshold);
+    };
+    if (!s->graph_on) {
+        issue();
+        MS_TRY(hipGetLastError());
+    } else if ((rc = graph_step(s, issue))) {
+        return rc;
+    }
     s->time += dt;
     s->step++;
     return MPMW_OK;
@@ -506,7 +590,8 @@ int mpms_create(const mpms_params *p, mpms_solver **out) {
     }
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&s->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&s->d_part, sizeof(double) * kStatBlocks * kStats) != hipSuccess) {
+        hipMalloc(&s->d_part, sizeof(double) * kStatBlocks * kStats) != hipSuccess ||
+        hipMalloc(&s->d_sa, sizeof(StepArgs)) != hipSuccess) {
         (void)hipGetLastError();
         delete s;
         return fail(MPMW_E_NOMEM, "F-tracking solver: device set-up failed");
@@ -520,9 +605,10 @@ int mpms_destroy(mpms_solver *s) {
     if (!s) return MPMW_OK;
     (void)hipSetDevice(s->prm.device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
     free_fields(s->f, s);
     free_fields(s->alt, s);
-    for (void *p : {(void *)s->gblock, (void *)s->d_counts, (void *)s->d_part, (void *)s->d_tool_xyz,
+    for (void *p : {(void *)s->gblock, (void *)s->d_counts, (void *)s->d_part, (void *)s->d_sa, (void *)s->d_tool_xyz,
                     (void *)s->d_tool_vel, (void *)s->keys, (void *)s->keys2, (void *)s->idx, (void *)s->idx2})
         if (p) (void)hipFree(p);
     mpmw::free_scratch(s->temp);
@@ -724,6 +810,24 @@ int mpms_get_diagnostics(mpms_solver *s, mpms_diagnostics *d) {
     d->overflow = (int64_t)cnt[0];
     d->island_kinematics = (int64_t)cnt[1];
     d->island_newton = (int64_t)cnt[2];
+    return MPMW_OK;
+}
+
+int mpms_set_graph(mpms_solver *s, int32_t on) {
+    MS_CHECK(s);
+    if (!on) {
+        MS_TRY(hipStreamSynchronize(s->stream));
+        drop_graphs(s);
+    }
+    s->graph_on = on != 0;
+    return MPMW_OK;
+}
+
+int mpms_graph_stats(mpms_solver *s, int64_t out[2]) {
+    MS_CHECK(s);
+    if (!out) return fail(MPMW_E_INVAL, "null");
+    out[0] = s->captures;
+    out[1] = s->replays;
     return MPMW_OK;
 }
 

@@ -75,6 +75,8 @@ def lib():
         "mpms_get_grid": (C.c_int, [vp, _ip, _ip, _dp, _dp, _dp]),
         "mpms_get_diagnostics": (C.c_int, [vp, C.POINTER(Diagnostics)]),
         "mpms_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
+        "mpms_set_graph": (C.c_int, [vp, C.c_int32]),
+        "mpms_graph_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -88,7 +90,7 @@ EXPORTS = ("mpms_params_default", "mpms_create", "mpms_destroy", "mpms_set_parti
            "mpms_particle_count", "mpms_init_particle_box", "mpms_init_cylinder", "mpms_set_tool_nodes",
            "mpms_set_tool_position", "mpms_apply_external_force", "mpms_step", "mpms_run", "mpms_synchronize",
            "mpms_stable_dt", "mpms_metrics", "mpms_get_bounds", "mpms_get_grid", "mpms_get_diagnostics",
-           "mpms_get_stream")
+           "mpms_get_stream", "mpms_set_graph", "mpms_graph_stats")
 
 
 def _check(rc, what):
@@ -232,6 +234,14 @@ class Solver:
         _check(lib().mpms_get_diagnostics(self.h, C.byref(d)), "diagnostics")
         return {"box_lo": tuple(d.box_lo), "box_n": tuple(d.box_n), "fits": d.fits, "sorts": d.sorts,
                 "overflow": d.overflow, "island_kinematics": d.island_kinematics, "island_newton": d.island_newton}
+
+    def set_graph(self, on=True):
+        _check(lib().mpms_set_graph(self.h, 1 if on else 0), "mpms_set_graph")
+
+    def graph_stats(self):
+        out = (C.c_int64 * 2)()
+        _check(lib().mpms_graph_stats(self.h, out), "mpms_graph_stats")
+        return {"captures": out[0], "replays": out[1]}
 
     def stream(self):
         v = C.c_void_p()

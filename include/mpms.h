@@ -129,6 +129,16 @@ int mpms_get_bounds(mpms_solver *s, double b[6]);
  * update, P2G force; NULL skips): for parity. */
 int mpms_get_grid(mpms_solver *s, int32_t lo[3], int32_t n[3], double *mass, double *mom, double *force);
 int mpms_get_diagnostics(mpms_solver *s, mpms_diagnostics *d);
+/* HIP graph of the step (MPMSolver::step records its launches as a CUDA
+ * graph and replays it, MPMSolver.cu:1077-1157): with mpms_set_graph(s, 1)
+ * the launches of a step after its fit are captured once per launch
+ * configuration (particle count, grid sub-box, buffer addresses; the two
+ * most recent kept) and replayed; dt, the tool position and the chip step
+ * reach the graph through device memory, so changing them does not capture
+ * again.  Results equal the eager step's bit for bit.  mpms_graph_stats:
+ * captures and replays so far. */
+int mpms_set_graph(mpms_solver *s, int32_t on);
+int mpms_graph_stats(mpms_solver *s, int64_t out[2]);
 /* The hipStream_t the solver launches on, as an opaque pointer. */
 int mpms_get_stream(mpms_solver *s, void **stream);
 

@@ -270,3 +270,41 @@ def test_ftrack_precision_islands():
     isl, plain, f64 = rows["fp32+islands"][0], rows["fp32 plain"][0], rows["fp64"][0]
     assert rows["fp32+islands"][1]["island_newton"] == n
     assert f64["plastic_strain"] <= 1e-9 and isl["plastic_strain"] <= 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_ftrack_graph_replay_matches_eager(precision):
+    """mpms_set_graph: the step replayed from captured graphs computes what the
+    eager step computes over 40 steps (the fits at steps 0 and 25 swap the
+    field buffers), while dt, the tool position and the tool nodes change
+    every step.  The P2G sums with float atomics, so two eager runs already
+    differ in the last bits: the bar is the parity tolerance (FP64 1e-9 on
+    every field, FP32 1e-5 on the gated ones) with identical statuses."""
+    kw, a, tool, tv = case()
+    prec = ftrack.FP64 if precision == "fp64" else ftrack.FP32
+    runs = []
+    for graph in (False, True):
+        s = ftrack.Solver(ftrack.default_params(**kw, precision=prec))
+        s.set_particles(oracle_view(a))
+        if graph:
+            s.set_graph(True)
+        for k in range(40):
+            dt = 1e-8 if k % 2 == 0 else 0.9e-8
+            s.set_tool_position(1.5e-3, 1.5e-3, 2.1e-3 - 1e-7 * k)
+            s.set_tool_nodes(tool + tv * 1e-8 * k, np.tile(tv, (len(tool), 1)))
+            s.step(dt)
+        runs.append((s.get_particles(), s.graph_stats(), s.grid()))
+    (e, es, eg), (g, gs, gg) = runs
+    print(f"\n[ftrack graph {precision}] eager={es} graph={gs}", flush=True)
+    assert es == {"captures": 0, "replays": 0}
+    assert gs["replays"] == 40 and 1 <= gs["captures"] <= 3
+    for f in ("id", "status", "lod"):
+        assert np.array_equal(e[f], g[f]), f
+    errs = {f: parity.normwise(g[f], e[f]) for f in FIELDS}
+    errs["grid_mass"] = parity.normwise(gg["mass"], eg["mass"])
+    errs["grid_momentum"] = parity.normwise(gg["mom"], eg["mom"])
+    _report(f"ftrack graph vs eager {precision}", errs, 0)
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    for k in (errs if precision == "fp64" else GATED):
+        assert errs[k] <= tol, (k, errs[k])
+    assert e["plastic_strain"].max() > 1e-4
