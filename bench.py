@@ -321,6 +321,90 @@ def ftrack_bench(steps, warmup, dims=(512, 256, 128)):
             "max_plastic_strain_proxy_max_stress": m["max_stress"], "diagnostics": d}
 
 
+def tet_tool_bench(steps, warmup, dims=(512, 256, 128)):
+    """The coupled step with a deformable tool: the F-tracking slab of the
+    ftrack leg, a corotational tet tool (a 10 mm diameter, 15 mm cylinder at
+    0.5 mm: 49,680 tets, E = 600 GPa, nu = 0.22) driven down at 5 m/s
+    through its spindle springs with its tip 0.1 mm into the slab, the tool's
+    surface nodes rasterised into the MPM grid and in XPBD hash contact with
+    the particles (cell = contact radius = 2h = 0.5 mm).  Per step, in the
+    engine's order: tool step, tool nodes to the MPM solver, MPM step,
+    contact (which reads back its totals, one synchronisation per step)."""
+    import ftrack
+    import toolfem
+    sp = 2.5e-4
+    mx = tuple((d - 1) * sp for d in dims)
+    P = ftrack.default_params(jc_A=1098e6, jc_B=1092e6, jc_n=0.93, jc_C=0.014, jc_m=1.1, young=113.8e9,
+                              poisson=0.342, density=4430.0, specific_heat=526.0, melt_temp=1878.0,
+                              ambient_temp=293.0, fixture_thickness=0.0, lod_enabled=1,
+                              lod_active_radius=0.006, lod_near_radius=0.025, max_particles=2 ** 31 - 1)
+    s = ftrack.Solver(P)
+    t0 = time.time()
+    s.init_box((0.0, 0.0, 0.0), mx, sp)
+    n = s.count()
+    cx, cy, top = 0.5 * mx[0], 0.5 * mx[1], mx[2]
+    mesh = toolfem.cylinder_tool_mesh(5e-3, 15e-3, 5e-4, base=(cx, cy, top - 1e-4))
+    tool = toolfem.Tool(toolfem.tool_params(young=600e9, poisson=0.22, spindle_damping=1.0))
+    tool.set_mesh(mesh)
+    tool.init_driven(0.2)
+    c = toolfem.Contact(toolfem.contact_params(contact_radius=5e-4, jc_A=1098e6, jc_B=1092e6, jc_n=0.93,
+                                               jc_C=0.014, jc_m=1.1))
+    s.set_tool_position(cx, cy, top)
+    setup = time.time() - t0
+    dt, v = 1e-8, 5.0
+    k = 0
+
+    def one(timers=None):
+        nonlocal k
+        tool.set_spindle_state((0.0, 0.0, -v * dt * k), (0.0, 0.0, -v))
+        t = time.perf_counter()
+        tool.step(dt)
+        toolfem.link_tool_nodes(s, tool)
+        if timers is not None:
+            tool.synchronize()
+            t1 = time.perf_counter()
+            timers["tool"] += t1 - t
+            t = t1
+        s.step(dt)
+        if timers is not None:
+            s.synchronize()
+            t1 = time.perf_counter()
+            timers["mpm"] += t1 - t
+            t = t1
+        c.resolve(s, tool, dt)
+        if timers is not None:
+            timers["contact"] += time.perf_counter() - t
+        k += 1
+
+    for _ in range(warmup):
+        one()
+    s.synchronize()
+    tool.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    s.synchronize()
+    tool.synchronize()
+    wall = time.perf_counter() - t0
+    timers = {"tool": 0.0, "mpm": 0.0, "contact": 0.0}
+    nb = max(3, steps // 4)
+    for _ in range(nb):
+        one(timers)
+    res = c.results()
+    counts = tool.counts()
+    out = {"workload": f"F-tracking slab {dims[0]}x{dims[1]}x{dims[2]} particles at 0.25 mm + corotational tet "
+                       f"tool ({counts[2]} tets, {counts[1]} tet nodes, {counts[0]} surface nodes, E 600 GPa, "
+                       f"nu 0.22) pressed at 5 m/s + XPBD hash contact (cell 0.5 mm), FP32 particles, dt 1e-8",
+           "particles": n, "steps": steps, "warmup": warmup, "ms_per_step": wall * 1e3 / steps,
+           "value": n * steps / wall, "unit": "particle-steps/s", "setup_s": setup,
+           "breakdown_ms_per_step": {kk: vv * 1e3 / nb for kk, vv in timers.items()},
+           "contact": res}
+    c.close()
+    tool.close()
+    s.close()
+    return out
+
+
 def measured_traffic():
     """HBM bytes per launch of the fused tile kernel on this workload, from
     the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
@@ -368,6 +452,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 timing of the same workload")
     ap.add_argument("--graph", type=int, default=1, help="1: also time the step replayed as HIP graphs")
+    ap.add_argument("--tet-tool-steps", type=int, default=20,
+                    help="steps of the coupled MPM + tet tool + hash contact leg (0 skips it)")
     ap.add_argument("--ftrack-steps", type=int, default=50,
                     help="steps of the F-tracking step's timing (0: skip it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -479,6 +565,8 @@ def main():
                                                          "batched_sims_per_hour")}
     if args.ftrack_steps > 0:
         out["ftrack"] = ftrack_bench(args.ftrack_steps, 5)
+    if args.tet_tool_steps > 0:
+        out["tet_tool_contact"] = tet_tool_bench(args.tet_tool_steps, 3)
     if args.serve_seconds > 0:
         sv = {"workload": f"Poisson stream of 20 requests/s for {args.serve_seconds:g} s, sizes and parameters of "
                           "the batched distribution, 200-2000 steps each; a request's result is the NDJSON "

@@ -43,7 +43,7 @@ def build(verbose=True):
     out = os.path.join(LIB_DIR, "libmpmw_hip.so")
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, "mpmw_solver.hip"), os.path.join(CSRC, "mpmw_sort.hip"),
-            os.path.join(CSRC, "mpms_solver.hip"),
+            os.path.join(CSRC, "mpms_solver.hip"), os.path.join(CSRC, "mpmc_solver.hip"),
             os.path.join(CSRC, "MPMWorkpieceSolver.cpp"), os.path.join(CSRC, "mpmw_toolmesh.cpp")]
     srcs = [s for s in srcs if os.path.exists(s)]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs])

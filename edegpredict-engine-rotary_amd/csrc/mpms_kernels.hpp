@@ -64,7 +64,7 @@ struct StepArgs {
 // kernel before the step's kernels: the kernels that read them can then be
 // replayed from a captured graph while dt, the tool position and the chip
 // step change.
-__global__ void k_step_args(StepArgs *__restrict__ d, StepArgs v) {
+static __global__ void k_step_args(StepArgs *__restrict__ d, StepArgs v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *d = v;
 }
 
@@ -785,7 +785,7 @@ template <typename T> __global__ void __launch_bounds__(kBlock) k_gather(const T
     if (i < n) dst[i] = src[perm[i]];
 }
 
-__global__ void __launch_bounds__(kBlock) k_zero_words(uint32_t *p, int64_t n) {
+static __global__ void __launch_bounds__(kBlock) k_zero_words(uint32_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = 0u;
 }
 

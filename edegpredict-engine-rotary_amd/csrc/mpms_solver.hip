@@ -19,6 +19,7 @@
 #include "mpms.h"
 #include "mpmw_sort.hpp"
 #include "mpms_kernels.hpp"
+#include "mpms_internal.hpp"
 
 namespace mpmw_internal {
 int set_error(int code, const std::string &msg);
@@ -839,3 +840,34 @@ int mpms_get_stream(mpms_solver *s, void **stream) {
 }
 
 }  // extern "C"
+
+namespace mpms_internal {
+bool is_fp64(const mpms_solver *s) { return s->fp64; }
+int32_t count(const mpms_solver *s) { return s->n; }
+int device(const mpms_solver *s) { return s->prm.device; }
+hipStream_t stream(const mpms_solver *s) { return s->stream; }
+void domain_min(const mpms_solver *s, double o[3]) {
+    for (int a = 0; a < 3; ++a) o[a] = s->prm.domain_min[a];
+}
+mpms::Particles<float> particles_f(mpms_solver *s) { return pview<float>(s); }
+mpms::Particles<double> particles_d(mpms_solver *s) { return pview<double>(s); }
+int tool_node_buffers(mpms_solver *s, int32_t n, double **xyz, double **vel) {
+    if (n > s->tool_cap) {
+        MS_TRY(hipStreamSynchronize(s->stream));
+        if (s->d_tool_xyz) (void)hipFree(s->d_tool_xyz);
+        if (s->d_tool_vel) (void)hipFree(s->d_tool_vel);
+        s->d_tool_xyz = s->d_tool_vel = nullptr;
+        s->tool_cap = 0;
+        if (hipMalloc(&s->d_tool_xyz, sizeof(double) * 3 * n) != hipSuccess ||
+            hipMalloc(&s->d_tool_vel, sizeof(double) * 3 * n) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MPMW_E_NOMEM, "hipMalloc of the tool nodes failed");
+        }
+        s->tool_cap = n;
+    }
+    s->ntool = n;
+    *xyz = s->d_tool_xyz;
+    *vel = s->d_tool_vel;
+    return MPMW_OK;
+}
+}  // namespace mpms_internal

@@ -756,3 +756,12 @@ void oms_grid(const oms_solver *s, int32_t lo[3], int32_t n[3], double *mass, do
         force[3 * i] = s->grid[i].fx; force[3 * i + 1] = s->grid[i].fy; force[3 * i + 2] = s->grid[i].fz;
     }
 }
+
+oms_particle *oms_particle_data(oms_solver *s, int *n) {
+    if (n) *n = s->n;
+    return s->p;
+}
+
+void oms_domain_min(const oms_solver *s, double out[3]) {
+    for (int a = 0; a < 3; ++a) out[a] = s->prm.domain_min[a];
+}

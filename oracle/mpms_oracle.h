@@ -86,6 +86,12 @@ void oms_init_cylinder(oms_solver *s, const double c[3], double radius, double l
  * when the arrays are non-NULL, mass, momentum (xyz), force (xyz) per node */
 void oms_grid(const oms_solver *s, int32_t lo[3], int32_t n[3], double *mass, double *mom, double *force);
 
+/* The particle array and the lattice origin, for the contact restatement
+ * (mpmc_oracle.c), which moves particles and changes their velocity and
+ * temperature in place. */
+oms_particle *oms_particle_data(oms_solver *s, int *n);
+void oms_domain_min(const oms_solver *s, double out[3]);
+
 /* Constitutive pieces, exposed for closed-form pins. */
 void oms_polar(const double F[9], double R[9], int *iters);
 void oms_p2g_stress(const oms_params *prm, oms_particle *p, double sigma[9], double *J, int *skip);

@@ -63,6 +63,59 @@ def test_ftrack_create_without_device_fails_loudly():
         ftrack.Solver()
 
 
+def test_toolfem_library_exports_every_declared_symbol():
+    """include/mpmc.h (the tool FEM and the hash contact): every declared
+    entry point is exported and bound."""
+    import toolfem
+    src = open(os.path.join(ROOT, "include", "mpmc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(mpm[tc]_[a-z0-9_]+)\s*\(", src)))
+    L = toolfem.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(toolfem.EXPORTS)
+
+
+def test_toolfem_params_default_match_oracle_defaults():
+    import mpmc as orc_mc
+    import toolfem
+    p, o = toolfem.tool_params(), orc_mc.tool_params()
+    for name, _ in orc_mc.ToolParams._fields_:
+        if name != "pad_":
+            assert getattr(p, name) == getattr(o, name), name
+    p, o = toolfem.contact_params(), orc_mc.contact_params()
+    for name, _ in orc_mc.ContactParams._fields_:
+        assert getattr(p, name) == getattr(o, name), name
+
+
+def test_toolfem_create_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    import toolfem
+    with pytest.raises(mpmw.MPMError):
+        toolfem.Tool()
+    with pytest.raises(mpmw.MPMError):
+        toolfem.Contact()
+
+
+def test_tool_mesh_generator_is_a_conforming_closed_tet_mesh():
+    """cylinder_tool_mesh: positive tet volumes summing to the voxelised
+    volume (cubes x h^3), every interior face shared by exactly two tets, the
+    surface nodes exactly the nodes of the one-tet faces."""
+    import toolfem
+    m = toolfem.cylinder_tool_mesh(1.5e-3, 4e-3, 5e-4)
+    X, el = m["tet_xyz"], m["elems"]
+    D = np.stack([X[el[:, 1]] - X[el[:, 0]], X[el[:, 2]] - X[el[:, 0]], X[el[:, 3]] - X[el[:, 0]]], 2)
+    vol = np.abs(np.linalg.det(D)) / 6
+    assert vol.min() > 0 and len(el) % 6 == 0
+    assert vol.sum() == pytest.approx(len(el) / 6 * 5e-4 ** 3, rel=1e-9)
+    faces = np.sort(np.concatenate([el[:, [0, 1, 2]], el[:, [0, 1, 3]], el[:, [0, 2, 3]], el[:, [1, 2, 3]]]), 1)
+    _, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert set(cnt.tolist()) <= {1, 2}
+    assert len(m["surf_xyz"]) < len(X) and np.all(m["emb_w"][:, 0] == 1.0)
+
+
 def test_params_default_match_oracle_defaults(orc):
     p = mpmw.default_params()
     o = orc.default_params()
