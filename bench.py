@@ -321,41 +321,41 @@ def ftrack_bench(steps, warmup, dims=(512, 256, 128)):
             "max_plastic_strain_proxy_max_stress": m["max_stress"], "diagnostics": d}
 
 
-def tet_tool_bench(steps, warmup, dims=(512, 256, 128)):
-    """The coupled step with a deformable tool: the F-tracking slab of the
-    ftrack leg, a corotational tet tool (a 10 mm diameter, 15 mm cylinder at
-    0.5 mm: 49,680 tets, E = 600 GPa, nu = 0.22) driven down at 5 m/s
-    through its spindle springs with its tip 0.1 mm into the slab, the tool's
-    surface nodes rasterised into the MPM grid and in XPBD hash contact with
-    the particles (cell = contact radius = 2h = 0.5 mm).  Per step, in the
-    engine's order: tool step, tool nodes to the MPM solver, MPM step,
-    contact (which reads back its totals, one synchronisation per step)."""
+def _tet_tool_run(steps, warmup, precision, dims):
+    """One coupled run (see tet_tool_bench); returns its timing, the contact
+    results and the particle and tool-node fields the precision study
+    compares."""
     import ftrack
     import toolfem
     sp = 2.5e-4
     mx = tuple((d - 1) * sp for d in dims)
+    # temperatures in degC here, as the tool's (ambient 25, the 1878 K melt at 1604.85)
     P = ftrack.default_params(jc_A=1098e6, jc_B=1092e6, jc_n=0.93, jc_C=0.014, jc_m=1.1, young=113.8e9,
-                              poisson=0.342, density=4430.0, specific_heat=526.0, melt_temp=1878.0,
-                              ambient_temp=293.0, fixture_thickness=0.0, lod_enabled=1,
-                              lod_active_radius=0.006, lod_near_radius=0.025, max_particles=2 ** 31 - 1)
+                              poisson=0.342, density=4430.0, specific_heat=526.0, melt_temp=1604.85,
+                              ambient_temp=25.0, fixture_thickness=0.0, lod_enabled=1,
+                              lod_active_radius=0.006, lod_near_radius=0.025, max_particles=2 ** 31 - 1,
+                              precision=precision)
     s = ftrack.Solver(P)
     t0 = time.time()
     s.init_box((0.0, 0.0, 0.0), mx, sp)
     n = s.count()
     cx, cy, top = 0.5 * mx[0], 0.5 * mx[1], mx[2]
-    mesh = toolfem.cylinder_tool_mesh(5e-3, 15e-3, 5e-4, base=(cx, cy, top - 1e-4))
+    mesh = toolfem.cylinder_tool_mesh(5e-3, 15e-3, 5e-4, base=(cx, cy, top - 1e-4), face_samples=5)
     tool = toolfem.Tool(toolfem.tool_params(young=600e9, poisson=0.22, spindle_damping=1.0))
     tool.set_mesh(mesh)
     tool.init_driven(0.2)
     c = toolfem.Contact(toolfem.contact_params(contact_radius=5e-4, jc_A=1098e6, jc_B=1092e6, jc_n=0.93,
-                                               jc_C=0.014, jc_m=1.1))
+                                               jc_C=0.014, jc_m=1.1, jc_T_ref=25.0, jc_T_melt=1604.85,
+                                               work_melt_temp=1604.85, work_specific_heat=526.0,
+                                               work_physical_density=4430.0))
     s.set_tool_position(cx, cy, top)
     setup = time.time() - t0
     dt, v = 1e-8, 5.0
     k = 0
+    most = 0
 
     def one(timers=None):
-        nonlocal k
+        nonlocal k, most
         tool.set_spindle_state((0.0, 0.0, -v * dt * k), (0.0, 0.0, -v))
         t = time.perf_counter()
         tool.step(dt)
@@ -372,6 +372,7 @@ def tet_tool_bench(steps, warmup, dims=(512, 256, 128)):
             timers["mpm"] += t1 - t
             t = t1
         c.resolve(s, tool, dt)
+        most = max(most, c.results()["contacts"])
         if timers is not None:
             timers["contact"] += time.perf_counter() - t
         k += 1
@@ -392,16 +393,67 @@ def tet_tool_bench(steps, warmup, dims=(512, 256, 128)):
         one(timers)
     res = c.results()
     counts = tool.counts()
-    out = {"workload": f"F-tracking slab {dims[0]}x{dims[1]}x{dims[2]} particles at 0.25 mm + corotational tet "
-                       f"tool ({counts[2]} tets, {counts[1]} tet nodes, {counts[0]} surface nodes, E 600 GPa, "
-                       f"nu 0.22) pressed at 5 m/s + XPBD hash contact (cell 0.5 mm), FP32 particles, dt 1e-8",
-           "particles": n, "steps": steps, "warmup": warmup, "ms_per_step": wall * 1e3 / steps,
-           "value": n * steps / wall, "unit": "particle-steps/s", "setup_s": setup,
+    parts = s.get_particles(fields=("pos", "temperature", "plastic_strain"))
+    surf = tool.surface()
+    out = {"particles": n, "steps": steps, "warmup": warmup, "ms_per_step": wall * 1e3 / steps,
+           "value": n * steps / wall, "setup_s": setup,
            "breakdown_ms_per_step": {kk: vv * 1e3 / nb for kk, vv in timers.items()},
-           "contact": res}
+           "contact": dict(res, most_contacts=most), "counts": counts, "total_steps": k}
     c.close()
     tool.close()
     s.close()
+    return out, parts, surf
+
+
+def tet_tool_bench(steps, warmup, dims=(512, 256, 128), precision_study=True):
+    """The coupled step with a deformable tool: the F-tracking slab of the
+    ftrack leg, a corotational tet tool (a 10 mm diameter, 15 mm cylinder at
+    0.5 mm: 49,680 tets, 9,703 tet nodes; 35,258 surface nodes, the
+    boundary triangles' barycentric points embedded in the tets; E = 600
+    GPa, nu = 0.22) driven down at 5 m/s through its spindle springs with
+    its tip 0.1 mm into the slab, the tool's surface nodes rasterised into
+    the MPM grid and in XPBD hash contact with the particles (cell = contact
+    radius = 2h = 0.5 mm).  Per step, in the engine's order: tool step, tool
+    nodes to the MPM solver, MPM step, contact (which reads back its totals,
+    one synchronisation per step).  The precision study runs the same steps
+    with FP64 particles and reports the FP32 run's per-field error against
+    it (positions, plastic strain, temperature, contact force)."""
+    import ftrack
+    r32, p32, s32 = _tet_tool_run(steps, warmup, ftrack.FP32, dims)
+    counts = r32.pop("counts")
+    out = {"workload": f"F-tracking slab {dims[0]}x{dims[1]}x{dims[2]} particles at 0.25 mm + corotational tet "
+                       f"tool ({counts[2]} tets, {counts[1]} tet nodes, {counts[0]} surface nodes, E 600 GPa, "
+                       f"nu 0.22) pressed at 5 m/s + XPBD hash contact (cell 0.5 mm), FP32 particles, dt 1e-8",
+           "unit": "particle-steps/s", **r32}
+    if not precision_study:
+        return out
+    r64, p64, s64 = _tet_tool_run(steps, warmup, ftrack.FP64, dims)
+    r64.pop("counts")
+
+    def byid(p):
+        o = np.argsort(p["id"], kind="stable")
+        return {k: v[o] for k, v in p.items()}
+
+    a, b = byid(p32), byid(p64)
+    same = a["status"] == b["status"]
+
+    def rel(x, y):
+        d = np.abs(x - y).max() if x.size else 0.0
+        return float(d / max(np.abs(y).max(), 1e-300))
+
+    out["fp64"] = {"ms_per_step": r64["ms_per_step"], "value": r64["value"],
+                   "breakdown_ms_per_step": r64["breakdown_ms_per_step"], "contact": r64["contact"]}
+    out["precision_study"] = {
+        "what": f"FP32 particles (FP64 positions and islands) against the FP64 run of the same "
+                f"{r32['total_steps']} coupled steps, normwise max|d| / max|ref| over particles of equal status",
+        "status_mismatch": int((~same).sum()),
+        "positions": rel(a["pos"][same], b["pos"][same]),
+        "plastic_strain": rel(a["plastic_strain"][same], b["plastic_strain"][same]),
+        "temperature": rel(a["temperature"][same], b["temperature"][same]),
+        "tool_node_force": rel(s32["force"], s64["force"]),
+        "tool_node_temperature": rel(s32["temperature"], s64["temperature"]),
+        "contact_force_total": abs(r32["contact"]["force"] - r64["contact"]["force"]) / max(abs(r64["contact"]["force"]), 1e-300),
+    }
     return out
 
 

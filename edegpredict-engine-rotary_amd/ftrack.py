@@ -118,7 +118,7 @@ def make_arrays(n):
 
 def _struct(d):
     a = Arrays()
-    a.n = len(d["mass"])
+    a.n = len(d["mass"]) if "mass" in d else len(d["id"])
     keep = {}
     for k in SHAPES:
         v = d.get(k)
@@ -167,8 +167,13 @@ class Solver:
         _check(lib().mpms_particle_count(self.h, C.byref(n)), "mpms_particle_count")
         return n.value
 
-    def get_particles(self):
+    def get_particles(self, fields=None):
+        """Every field, or only those named (id and status always): a
+        16 M-particle state is ~7 GB of FP64 arrays."""
         d = make_arrays(self.count())
+        if fields is not None:
+            want = set(fields) | {"id", "status"}
+            d = {k: v for k, v in d.items() if k in want}
         a, keep = _struct(d)
         _check(lib().mpms_get_particles(self.h, C.byref(a)), "mpms_get_particles")
         for k, v in keep.items():

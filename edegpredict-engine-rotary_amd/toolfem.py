@@ -138,15 +138,19 @@ def contact_params(**kw):
     return p
 
 
-def cylinder_tool_mesh(radius, length, h, base=(0.0, 0.0, 0.0), shank_fraction=0.2):
+def cylinder_tool_mesh(radius, length, h, base=(0.0, 0.0, 0.0), shank_fraction=0.2, face_samples=0):
     """A cylindrical tool along +z from `base` (its tip face at base z):
     grid points at spacing h inside the radius, each grid cube whose 8
     corners are inside cut into 6 tets around its main diagonal (a conforming
     Kuhn subdivision), tet nodes = the points the tets use, surface nodes =
     the tet nodes on boundary faces (a face of one tet only), each tied to
-    its tet node with weight 1.  Tet nodes in the top shank_fraction of the
-    length are ToolRegion SHANK, the bottom 10 % CUTTING_EDGE, the rest
-    TOOL_BODY."""
+    its tet node with weight 1.  With face_samples = n > 2 every boundary
+    triangle also carries the interior points of its barycentric grid of
+    order n ((n - 1)(n - 2) / 2 points) as surface nodes, embedded in the
+    triangle's three tet nodes with those barycentric weights: a surface
+    finer than the tets, as the reference's tool meshes are.  Tet nodes in
+    the top shank_fraction of the length are ToolRegion SHANK, the bottom
+    10 % CUTTING_EDGE, the rest TOOL_BODY."""
     nr = int(np.floor(radius / h))
     nz = int(round(length / h))
     ii = np.arange(-nr, nr + 1)
@@ -186,8 +190,23 @@ def cylinder_tool_mesh(radius, length, h, base=(0.0, 0.0, 0.0), shank_fraction=0
     emb_tet[:, 0] = surf
     emb_w = np.zeros((len(surf), 4))
     emb_w[:, 0] = 1.0
-    return {"surf_xyz": xyz[surf], "surf_region": region[surf], "tet_xyz": xyz, "tet_region": region,
-            "elems": el, "emb_surf": np.arange(len(surf), dtype=np.int32), "emb_tet": emb_tet, "emb_w": emb_w}
+    sxyz, sreg = xyz[surf], region[surf]
+    if face_samples > 2:
+        n = face_samples
+        bary = np.array([(i, j, n - i - j) for i in range(1, n) for j in range(1, n - i)], float) / n
+        tri = uf[cnt == 1]                                   # (nf, 3) tet-node indices
+        w = np.repeat(bary[None], len(tri), 0).reshape(-1, 3)
+        nodes = np.repeat(tri, len(bary), 0)
+        pts = (xyz[nodes] * w[:, :, None]).sum(1)
+        e_t = np.full((len(pts), 4), -1, np.int32)
+        e_t[:, :3] = nodes
+        e_w = np.zeros((len(pts), 4))
+        e_w[:, :3] = w
+        emb_tet, emb_w = np.concatenate([emb_tet, e_t]), np.concatenate([emb_w, e_w])
+        sxyz = np.concatenate([sxyz, pts])
+        sreg = np.concatenate([sreg, region[nodes[:, 0]]])
+    return {"surf_xyz": sxyz, "surf_region": sreg.astype(np.int32), "tet_xyz": xyz, "tet_region": region,
+            "elems": el, "emb_surf": np.arange(len(sxyz), dtype=np.int32), "emb_tet": emb_tet, "emb_w": emb_w}
 
 
 class Tool:

@@ -132,7 +132,7 @@ def contact_setup(precision, steps_before=0):
     sg = ftrack.Solver(ftrack.default_params(**mp, precision=ftrack.FP64 if precision == "fp64" else ftrack.FP32))
     import test_gpu_ftrack as tf
     sg.set_particles(tf.oracle_view(a))
-    mesh = toolfem.cylinder_tool_mesh(1.0e-3, 3e-3, H, base=(1.5e-3, 1.5e-3, 2e-3 - 1.2e-4))
+    mesh = toolfem.cylinder_tool_mesh(1.0e-3, 3e-3, H, base=(1.5e-3, 1.5e-3, 2e-3 - 1.2e-4), face_samples=4)
     to, tg = tool_pair(mesh, spindle_damping=1.0)
     cp = dict(contact_radius=2e-4, contact_htc=1e5)
     co = mpmc.ContactOracle(**cp)
