@@ -451,12 +451,13 @@ __device__ __forceinline__ void apic_tile_block(const Particles<R> &P, const Gri
 }
 
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK>
-__global__ void __launch_bounds__(kBlock) k_apic_tile_batch(const TileWork<R> *__restrict__ items, int nitems) {
-    const int w = find_item(items, nitems, (int)blockIdx.x);
+__global__ void __launch_bounds__(kBlock) k_apic_tile_batch(const TileWork<R> *__restrict__ items, int nitems,
+                                                            int order) {
+    const int b = launch_block(order);
+    const int w = find_item(items, nitems, b);
     const TileWork<R> &W = items[w];
     apic_tile_block<R, DO_G2P, DO_P2G, HAS_BACK>(W.P, W.g, W.G, W.M, W.dt, W.dtd, W.zcut, W.flags, W.ref_temp,
-                                                 W.part, W.cell_start, W.tg, W.counters, W.tpart,
-                                                 (int)blockIdx.x - W.block0);
+                                                 W.part, W.cell_start, W.tg, W.counters, W.tpart, b - W.block0);
 }
 
 // The loose particles of the APIC pass, one thread each, nodes read from

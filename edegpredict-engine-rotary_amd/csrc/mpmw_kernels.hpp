@@ -345,16 +345,17 @@ __device__ __forceinline__ double grid_node(const GridView<R> &g, const Grid<R> 
 }
 
 // The grid update, one thread per node in node order (x fastest), block
-// `blk` of the grid.  Contact-force partial per block, summed in a fixed
-// order.
+// `blk` of the grid.  Contact-force partial per wave of nodes, summed in a
+// fixed order.
 template <typename R>
 __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Grid<R> &G, const TileParts<R> &tp,
                                                   const Material<R> &M, const Tool &tool, R dt, int direct_T,
-                                                  double *force_part, int blk) {
+                                                  double *force_part, int nwaves, int blk) {
     double cf = 0.0;
     // node counts are below 2^31 (the grid's index type is int32 per axis
-    // and the binning keys are 32-bit)
-    const int idx = blk * kBlock + (int)threadIdx.x;
+    // and the binning keys are 32-bit); the block size is any multiple of
+    // the wave (the autotuner picks it)
+    const int idx = blk * (int)blockDim.x + (int)threadIdx.x;
     if (idx < g.nn) {
         const int row = idx / g.nx;
         const int i = idx - row * g.nx;
@@ -363,16 +364,11 @@ __device__ __forceinline__ void grid_update_block(const GridView<R> &g, const Gr
         gather_tile_sums(tp, i, j, k, c);
         cf = grid_node(g, G, tp, M, tool, dt, direct_T, idx, i, j, k, c);
     }
-    // fixed-order block sum of the contact force
-    __shared__ double sh[kBlock / kWave];
+    // the contact force: one fixed-order partial per wave of 64 nodes, the
+    // same partials for every block size and block order
     cf = wave_sum(cf);
-    if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = cf;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = sh[0];
-        for (int w = 1; w < kBlock / kWave; ++w) r += sh[w];
-        force_part[blk] = r;
-    }
+    const int wv = idx / kWave;
+    if (threadIdx.x % kWave == 0 && wv < nwaves) force_part[wv] = cf;
 }
 
 // mpmDiffuseTemperatureKernel, MPMWorkpieceSolver.cu:208-262, Jacobi form:
@@ -466,14 +462,21 @@ template <typename R> struct GridWork {
     double *force_part;
     int32_t block0, blocks;    // grid-update blocks of the item
     int32_t dblock0, dblocks;  // diffusion blocks of the item (post pass)
+    int32_t fwaves, pad_;      // contact-force partials (one per wave of nodes)
 };
 
+// Block order of a launch: 0 = blockIdx.x (workgroups round-robin over the
+// dies), 1 = xcd_block() (one contiguous range per die).  A bijection either
+// way: the autotuner's choice changes no result.
+__device__ __forceinline__ int launch_block(int order) { return order ? xcd_block() : (int)blockIdx.x; }
+
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *__restrict__ items, int nitems) {
-    const int b = xcd_block();
+__global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> *__restrict__ items, int nitems,
+                                                              int order) {
+    const int b = launch_block(order);
     const int w = find_item(items, nitems, b);
     const GridWork<R> &W = items[w];
-    grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, b - W.block0);
+    grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, W.fwaves, b - W.block0);
 }
 
 // The fixed-order sum of an item's contact-force partials and, when
@@ -497,8 +500,8 @@ __device__ __forceinline__ void sum_block(const SumWork &W) {
 // r's contact force and reset its flags.
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_post_batch(const GridWork<R> *__restrict__ items, int nitems,
-                                                       const SumWork *__restrict__ sums, int dblocks) {
-    const int b = xcd_block();
+                                                       const SumWork *__restrict__ sums, int dblocks, int order) {
+    const int b = launch_block(order);
     if (b >= dblocks) {
         sum_block(sums[b - dblocks]);
         return;

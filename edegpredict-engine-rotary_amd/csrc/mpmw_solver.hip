@@ -105,6 +105,18 @@ struct StepStaging {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     std::vector<uint64_t> sig[2];
     int64_t captures = 0, replays = 0;
+    // Launch configuration of the step's kernels and its autotuner.  Every
+    // choice is a bijection of the launch's blocks or a block size whose
+    // partials do not depend on it, so no choice changes a result.
+    struct Tuner {
+        int32_t tile_order = 0, grid_order = 1, grid_block = kBlock, post_order = 1;   // in use
+        int32_t trials = 0;        // steps per candidate; 0: not tuning
+        int32_t state = 0;         // 0 default / set, 1 tuning, 2 tuned
+        int32_t phase = 0, cand = 0, trial = 0;
+        float ms[3][6] = {};       // median per candidate: tile (2), grid (6), post (2)
+        float sample[16] = {};
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    } tune;
 };
 
 struct mpmw_solver {
@@ -679,7 +691,7 @@ int rebuild_grid_from_stats(mpmw_solver *s) {
         s->snap_valid = false;
     }
     s->nn = nn;
-    int rc = grow(s, s->fpart, s->fpart_cap, blocks_for(nn));
+    int rc = grow(s, s->fpart, s->fpart_cap, (nn + kWave - 1) / kWave);   // one contact-force partial per wave
     if (rc) return rc;
     s->p2g_ready = false;
     return MPMW_OK;
@@ -1003,6 +1015,9 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
                                    hipStream_t stream, StepStaging &st);
 void free_staging(StepStaging &st);
 void drop_graphs(StepStaging &st);
+int set_autotune(StepStaging &st, int32_t trials);
+int set_launch_config(StepStaging &st, const mpmw_tuning *c);
+void get_tuning(const StepStaging &st, mpmw_tuning *o);
 
 // MPMWorkpieceSolver::step, MPMWorkpieceSolver.cu:727-791: a step of a batch
 // of one, so a solver runs the same kernels alone and in a batch.
@@ -2020,6 +2035,23 @@ int mpmw_set_graph(mpmw_solver *s, int32_t on) {
     return MPMW_OK;
 }
 
+int mpmw_set_autotune(mpmw_solver *s, int32_t trials) {
+    CHECK_S(s);
+    return set_autotune(s->staging, trials);
+}
+
+int mpmw_set_launch_config(mpmw_solver *s, const mpmw_tuning *cfg) {
+    CHECK_S(s);
+    return set_launch_config(s->staging, cfg);
+}
+
+int mpmw_get_tuning(mpmw_solver *s, mpmw_tuning *out) {
+    CHECK_S(s);
+    if (!out) return fail(MPMW_E_INVAL, "null");
+    get_tuning(s->staging, out);
+    return MPMW_OK;
+}
+
 int mpmw_graph_stats(mpmw_solver *s, int64_t out[2]) {
     CHECK_S(s);
     if (!out) return fail(MPMW_E_INVAL, "null");
@@ -2236,23 +2268,23 @@ void batch_detach(mpmw_batch *b, mpmw_solver *s) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename R> void launch_tile_batch(hipStream_t stream, int variant, const TileWork<R> *items, int n,
-                                             int blocks) {
+                                             int blocks, int order) {
     const dim3 grid(blocks), blk(kBlock);
     switch (variant) {
-    case 0: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, false>), grid, blk, 0, stream, items, n); break;
-    case 1: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, false>), grid, blk, 0, stream, items, n); break;
-    case 2: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, true>), grid, blk, 0, stream, items, n); break;
-    case 3: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, true>), grid, blk, 0, stream, items, n); break;
-    case 4: hipLaunchKernelGGL((k_tile_batch<R, true, false, false, false>), grid, blk, 0, stream, items, n); break;
-    case 5: hipLaunchKernelGGL((k_tile_batch<R, true, false, true, false>), grid, blk, 0, stream, items, n); break;
-    case 6: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, false>), grid, blk, 0, stream, items, n); break;
-    case 7: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, true>), grid, blk, 0, stream, items, n); break;
-    case 8: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, false>), grid, blk, 0, stream, items, n); break;
-    case 9: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, true>), grid, blk, 0, stream, items, n); break;
-    case 10: hipLaunchKernelGGL((k_tile_batch<R, false, true, false, false>), grid, blk, 0, stream, items, n); break;
-    case 11: hipLaunchKernelGGL((k_tile_batch<R, false, true, true, false>), grid, blk, 0, stream, items, n); break;
-    case 12: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, false>), grid, blk, 0, stream, items, n); break;
-    default: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, true>), grid, blk, 0, stream, items, n); break;
+    case 0: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, false>), grid, blk, 0, stream, items, n, order); break;
+    case 1: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, false>), grid, blk, 0, stream, items, n, order); break;
+    case 2: hipLaunchKernelGGL((k_tile_batch<R, true, true, false, true>), grid, blk, 0, stream, items, n, order); break;
+    case 3: hipLaunchKernelGGL((k_tile_batch<R, true, true, true, true>), grid, blk, 0, stream, items, n, order); break;
+    case 4: hipLaunchKernelGGL((k_tile_batch<R, true, false, false, false>), grid, blk, 0, stream, items, n, order); break;
+    case 5: hipLaunchKernelGGL((k_tile_batch<R, true, false, true, false>), grid, blk, 0, stream, items, n, order); break;
+    case 6: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, false>), grid, blk, 0, stream, items, n, order); break;
+    case 7: hipLaunchKernelGGL((k_apic_tile_batch<R, true, true, true>), grid, blk, 0, stream, items, n, order); break;
+    case 8: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, false>), grid, blk, 0, stream, items, n, order); break;
+    case 9: hipLaunchKernelGGL((k_apic_tile_batch<R, true, false, true>), grid, blk, 0, stream, items, n, order); break;
+    case 10: hipLaunchKernelGGL((k_tile_batch<R, false, true, false, false>), grid, blk, 0, stream, items, n, order); break;
+    case 11: hipLaunchKernelGGL((k_tile_batch<R, false, true, true, false>), grid, blk, 0, stream, items, n, order); break;
+    case 12: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, false>), grid, blk, 0, stream, items, n, order); break;
+    default: hipLaunchKernelGGL((k_apic_tile_batch<R, false, true, true>), grid, blk, 0, stream, items, n, order); break;
     }
 }
 
@@ -2292,6 +2324,8 @@ void drop_graphs(StepStaging &st) {
 // Free the staging areas of a solver or a batch.
 void free_staging(StepStaging &st) {
     drop_graphs(st);
+    for (auto &e : st.tune.ev)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
     for (auto &a : st.area) {
         if (a.copied) (void)hipEventDestroy(a.copied);
         pool::put(a.h);
@@ -2300,6 +2334,45 @@ void free_staging(StepStaging &st) {
     pool::put(st.d_blob);
     st.d_blob = nullptr;
     st.d_cap = 0;
+}
+
+int set_autotune(StepStaging &st, int32_t trials) {
+    if (trials < 0 || trials > 16) return fail(MPMW_E_INVAL, "autotune trials must be in [0, 16]");
+    StepStaging::Tuner &t = st.tune;
+    t.trials = trials;
+    t.phase = t.cand = t.trial = 0;
+    std::memset(t.ms, 0, sizeof(t.ms));
+    t.state = trials ? 1 : 0;
+    return MPMW_OK;
+}
+
+int set_launch_config(StepStaging &st, const mpmw_tuning *c) {
+    if (!c) return fail(MPMW_E_INVAL, "null");
+    if ((c->tile_order | c->grid_order | c->post_order) & ~1)
+        return fail(MPMW_E_INVAL, "block orders are 0 (blockIdx) or 1 (per-XCD ranges)");
+    if (c->grid_block != 64 && c->grid_block != 128 && c->grid_block != 256)
+        return fail(MPMW_E_INVAL, "grid_block must be 64, 128 or 256");
+    StepStaging::Tuner &t = st.tune;
+    t.tile_order = c->tile_order;
+    t.grid_order = c->grid_order;
+    t.grid_block = c->grid_block;
+    t.post_order = c->post_order;
+    t.trials = 0;
+    t.state = 0;
+    return MPMW_OK;
+}
+
+void get_tuning(const StepStaging &st, mpmw_tuning *o) {
+    const StepStaging::Tuner &t = st.tune;
+    std::memset(o, 0, sizeof(*o));
+    o->state = t.state;
+    o->tile_order = t.tile_order;
+    o->grid_order = t.grid_order;
+    o->grid_block = t.grid_block;
+    o->post_order = t.post_order;
+    o->trials = t.trials;
+    for (int k = 0; k < 2; ++k) { o->tile_ms[k] = t.ms[0][k]; o->post_ms[k] = t.ms[2][k]; }
+    for (int k = 0; k < 6; ++k) o->grid_ms[k] = t.ms[1][k];
 }
 
 // Stages `end` bytes, written by `fill` into one of the two pinned areas,
@@ -2520,7 +2593,7 @@ template <typename R> int step_begin_many(mpmw_solver *const *members, size_t nm
             for (int v = 0; v < kTileVariants; ++v)
                 if (!pw[v].empty())
                     launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off[v]), (int)pw[v].size(),
-                                         pblocks[v]);
+                                         pblocks[v], st.tune.tile_order);
         }
         HIP_TRY(hipGetLastError());
         for (int m : pm) {
@@ -2550,6 +2623,44 @@ template <typename R> int step_begin_many(mpmw_solver *const *members, size_t nm
     return MPMW_OK;
 }
 
+// The autotuner's candidates, one phase per kernel class: the tile pass's
+// block order; the grid update's block order and block size; the post
+// pass's block order.
+constexpr int kTunePhases = 3;
+constexpr int kTuneCands[kTunePhases] = {2, 6, 2};
+constexpr int kGridCand[6][2] = {{1, 256}, {1, 128}, {1, 64}, {0, 256}, {0, 128}, {0, 64}};
+
+// The configuration to run a step with: while tuning, the best so far with
+// the phase's knob set to the candidate under test.
+StepStaging::Tuner step_config(const StepStaging::Tuner &t) {
+    StepStaging::Tuner c = t;
+    if (t.state != 1) return c;
+    if (t.phase == 0) c.tile_order = t.cand;
+    if (t.phase == 1) { c.grid_order = kGridCand[t.cand][0]; c.grid_block = kGridCand[t.cand][1]; }
+    if (t.phase == 2) c.post_order = t.cand == 0 ? 1 : 0;
+    return c;
+}
+
+// One timed sample of the tuned phase (ms of its kernel class on a step
+// without a rebuild); after `trials` samples the candidate's median is
+// kept, after the phase's last candidate the fastest is adopted.
+void tune_record(StepStaging::Tuner &t, float ms) {
+    t.sample[t.trial++] = ms;
+    if (t.trial < t.trials) return;
+    std::sort(t.sample, t.sample + t.trial);
+    t.ms[t.phase][t.cand] = t.sample[t.trial / 2];
+    t.trial = 0;
+    if (++t.cand < kTuneCands[t.phase]) return;
+    int best = 0;
+    for (int k = 1; k < kTuneCands[t.phase]; ++k)
+        if (t.ms[t.phase][k] < t.ms[t.phase][best]) best = k;
+    if (t.phase == 0) t.tile_order = best;
+    if (t.phase == 1) { t.grid_order = kGridCand[best][0]; t.grid_block = kGridCand[best][1]; }
+    if (t.phase == 2) t.post_order = best == 0 ? 1 : 0;
+    t.cand = 0;
+    if (++t.phase == kTunePhases) t.state = 2;
+}
+
 // One step of every solver in `members` (all of precision R, on `stream`):
 // the host phases of each solver's own step, and each kernel of the step
 // launched once for all of them over a flattened block list.
@@ -2563,6 +2674,11 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         if (plans[m].run) run.push_back((int)m);
     if (run.empty()) return MPMW_OK;
     const int nr = (int)run.size();
+    // a tuning sample only on a step whose members all run the fused pass
+    bool clean = true;
+    for (int r = 0; r < nr; ++r) clean = clean && !plans[run[r]].next_rebuild;
+    const bool sampling = st.tune.state == 1 && clean;
+    const StepStaging::Tuner cfg = sampling ? step_config(st.tune) : st.tune;
 
     // work items
     std::vector<GridWork<R>> gw(nr);
@@ -2585,13 +2701,15 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         g.diffuse = sp.diffusion ? 1 : 0;
         g.force_part = s->fpart;
         g.block0 = gblocks;
-        g.blocks = blocks_for(s->nn);
+        g.blocks = blocks_for(s->nn, cfg.grid_block);
+        g.fwaves = (int32_t)((s->nn + kWave - 1) / kWave);
+        g.pad_ = 0;
         gblocks += g.blocks;
         g.dblock0 = dblocks;
         g.dblocks = blocks_for(s->nn, kDiffuseBlock);
         dblocks += g.dblocks;
         sw[r].part = s->fpart;
-        sw[r].nparts = g.blocks;
+        sw[r].nparts = g.fwaves;
         sw[r].modes = s->d_modes + kRed;
         sw[r].out = s->d_cf;
         sw[r].counters = sp.next_rebuild ? nullptr : s->sort.oow;
@@ -2622,28 +2740,44 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         // with `time_it` each class is bracketed by the solver's timing events
         auto launch = [&](char *d, bool time_it) {
             mpmw_solver *tm = time_it ? one : nullptr;
+            hipEvent_t *ev = sampling ? st.tune.ev : nullptr;
             const GridWork<R> *d_gw = (const GridWork<R> *)d;
+            if (ev) (void)hipEventRecord(ev[0], stream);
             {
                 Timed t(tm, MPMW_K_GRID);
-                hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(kBlock), 0, stream, d_gw, nr);
+                hipLaunchKernelGGL(k_grid_update_batch<R>, dim3(gblocks), dim3(cfg.grid_block), 0, stream, d_gw, nr,
+                                   cfg.grid_order);
             }
+            if (ev) (void)hipEventRecord(ev[1], stream);
             {
                 // diffusion, contact-force sums and flag resets in one launch
                 Timed t(tm, MPMW_K_DIFFUSE);
                 hipLaunchKernelGGL(k_post_batch<R>, dim3(dblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
-                                   (const SumWork *)(d + off_sw), dblocks);
+                                   (const SumWork *)(d + off_sw), dblocks, cfg.post_order);
             }
+            if (ev) (void)hipEventRecord(ev[2], stream);
             {
                 Timed t(tm, MPMW_K_G2P_P2G);
                 for (int v = 0; v < kTileVariants; ++v)
                     if (!tw[v].empty())
                         launch_tile_batch<R>(stream, v, (const TileWork<R> *)(d + off_tw[v]), (int)tw[v].size(),
-                                             tblocks[v]);
+                                             tblocks[v], cfg.tile_order);
             }
+            if (ev) (void)hipEventRecord(ev[3], stream);
         };
-        // kernel timing needs events between the launches: eager then
-        if (st.graph_on && !(one && one->timing)) {
-            std::vector<uint64_t> sig = {(uint64_t)nr, (uint64_t)gblocks, (uint64_t)dblocks, (uint64_t)off_sw};
+        if (sampling) {
+            for (auto &e : st.tune.ev)
+                if (!e && hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return fail(MPMW_E_HIP, "autotuner: event creation failed");
+                }
+        }
+        // kernel timing and tuning samples need events between the
+        // launches: eager then
+        if (st.graph_on && !(one && one->timing) && !sampling) {
+            std::vector<uint64_t> sig = {(uint64_t)nr, (uint64_t)gblocks, (uint64_t)dblocks, (uint64_t)off_sw,
+                                         (uint64_t)cfg.tile_order, (uint64_t)cfg.grid_order,
+                                         (uint64_t)cfg.grid_block, (uint64_t)cfg.post_order};
             for (int v = 0; v < kTileVariants; ++v) {
                 sig.push_back((uint64_t)tw[v].size());
                 sig.push_back((uint64_t)tblocks[v]);
@@ -2655,6 +2789,13 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
             char *d = nullptr;
             if ((rc = stage_blob(st, stream, end, fill, &d))) return rc;
             launch(d, true);
+            if (sampling) {
+                HIP_TRY(hipEventSynchronize(st.tune.ev[3]));
+                float ms = 0.0f;
+                const int a = st.tune.phase == 0 ? 2 : (st.tune.phase == 1 ? 0 : 1);
+                HIP_TRY(hipEventElapsedTime(&ms, st.tune.ev[a], st.tune.ev[a + 1]));
+                tune_record(st.tune, ms);
+            }
         }
     }
     HIP_TRY(hipGetLastError());
@@ -2758,6 +2899,24 @@ int mpmw_batch_set_graph(mpmw_batch *b, int32_t on) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->staging.graph_on = on != 0;
     if (!on) drop_graphs(b->staging);
+    return MPMW_OK;
+}
+
+int mpmw_batch_set_autotune(mpmw_batch *b, int32_t trials) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    return set_autotune(b->staging, trials);
+}
+
+int mpmw_batch_set_launch_config(mpmw_batch *b, const mpmw_tuning *cfg) {
+    int rc = check_b(b);
+    if (rc) return rc;
+    return set_launch_config(b->staging, cfg);
+}
+
+int mpmw_batch_get_tuning(mpmw_batch *b, mpmw_tuning *out) {
+    if (!b || !out) return fail(MPMW_E_INVAL, "null");
+    get_tuning(b->staging, out);
     return MPMW_OK;
 }
 

@@ -862,11 +862,12 @@ template <typename R> struct TileWork {
 
 template <typename R, bool DO_G2P, bool DO_P2G, bool HAS_BACK, bool DET>
 __global__ void __launch_bounds__(kBlock, TileWaves<R>::value)
-    k_tile_batch(const TileWork<R> *__restrict__ items, int nitems) {
-    // blockIdx.x order, not xcd_block(): tiles own disjoint particle ranges
-    // and with neighbouring tiles spread over the dies the pass measured 8 %
-    // faster than with the per-die ranges
-    const int b = (int)blockIdx.x;
+    k_tile_batch(const TileWork<R> *__restrict__ items, int nitems, int order) {
+    // order 0 (blockIdx.x, the default): tiles own disjoint particle ranges
+    // and with neighbouring tiles spread over the dies the slab's pass
+    // measured 8 % faster than with the per-die ranges of order 1; the
+    // autotuner measures both on the workload at hand
+    const int b = launch_block(order);
     const int w = find_item(items, nitems, b);
     const TileWork<R> &W = items[w];
     tile_block<R, DO_G2P, DO_P2G, HAS_BACK, DET>(W.P, W.g, W.G, W.M, W.dt, W.dtd, W.zcut, W.flags, W.ref_temp, W.part,

@@ -39,6 +39,8 @@ EXPORTS = (
     "mpmw_set_transfer", "mpmw_get_affine", "mpmw_set_affine", "mpmw_export_ndjson", "mpmw_export_vtk",
     "mpmw_export_ndjson_noalloc", "mpmw_morph_tool_sdf", "mpmw_set_graph", "mpmw_graph_stats",
     "mpmw_batch_set_graph", "mpmw_batch_graph_stats", "mpmw_tool_mesh_params_default", "mpmw_parametric_tool_mesh",
+    "mpmw_set_autotune", "mpmw_set_launch_config", "mpmw_get_tuning", "mpmw_batch_set_autotune",
+    "mpmw_batch_set_launch_config", "mpmw_batch_get_tuning",
 )
 
 
@@ -124,6 +126,25 @@ class Diagnostics(C.Structure):
                 ("pad_", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    """mpmw_tuning: the launch configuration in use and the autotuner's
+    state and medians."""
+    _fields_ = [("state", C.c_int32), ("tile_order", C.c_int32), ("grid_order", C.c_int32),
+                ("grid_block", C.c_int32), ("post_order", C.c_int32), ("trials", C.c_int32),
+                ("tile_ms", C.c_float * 2), ("grid_ms", C.c_float * 6), ("post_ms", C.c_float * 2)]
+
+    def as_dict(self):
+        return {"state": self.state, "tile_order": self.tile_order, "grid_order": self.grid_order,
+                "grid_block": self.grid_block, "post_order": self.post_order, "trials": self.trials,
+                "tile_ms": list(self.tile_ms), "grid_ms": list(self.grid_ms), "post_ms": list(self.post_ms)}
+
+
+def launch_config(tile_order=0, grid_order=1, grid_block=256, post_order=1):
+    t = Tuning()
+    t.tile_order, t.grid_order, t.grid_block, t.post_order = tile_order, grid_order, grid_block, post_order
+    return t
+
+
 class MPMError(RuntimeError):
     pass
 
@@ -205,6 +226,12 @@ def lib():
         "mpmw_graph_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "mpmw_batch_set_graph": (C.c_int, [vp, C.c_int32]),
         "mpmw_batch_graph_stats": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mpmw_set_autotune": (C.c_int, [vp, C.c_int32]),
+        "mpmw_set_launch_config": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "mpmw_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "mpmw_batch_set_autotune": (C.c_int, [vp, C.c_int32]),
+        "mpmw_batch_set_launch_config": (C.c_int, [vp, C.POINTER(Tuning)]),
+        "mpmw_batch_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -472,6 +499,19 @@ class Solver:
         _check(lib().mpmw_graph_stats(self.h, v), "graph_stats")
         return {"captures": v[0], "replays": v[1]}
 
+    def set_autotune(self, trials):
+        """Tune the step's launch configuration over the next steps (trials
+        steps per candidate; 0 stops); results are unchanged by it."""
+        _check(lib().mpmw_set_autotune(self.h, trials), "set_autotune")
+
+    def set_launch_config(self, cfg):
+        _check(lib().mpmw_set_launch_config(self.h, C.byref(cfg)), "set_launch_config")
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().mpmw_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
     def set_deterministic(self, on):
         _check(lib().mpmw_set_deterministic(self.h, 1 if on else 0), "set_deterministic")
 
@@ -571,3 +611,14 @@ class Batch:
         v = (C.c_int64 * 2)()
         _check(lib().mpmw_batch_graph_stats(self.h, v), "batch_graph_stats")
         return {"captures": v[0], "replays": v[1]}
+
+    def set_autotune(self, trials):
+        _check(lib().mpmw_batch_set_autotune(self.h, trials), "batch_set_autotune")
+
+    def set_launch_config(self, cfg):
+        _check(lib().mpmw_batch_set_launch_config(self.h, C.byref(cfg)), "batch_set_launch_config")
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().mpmw_batch_get_tuning(self.h, C.byref(t)), "batch_get_tuning")
+        return t.as_dict()

@@ -260,6 +260,32 @@ int mpmw_set_deterministic(mpmw_solver *s, int32_t on);
 int mpmw_set_graph(mpmw_solver *s, int32_t on);
 int mpmw_graph_stats(mpmw_solver *s, int64_t out[2]);
 
+/* Launch-configuration autotuner (no counterpart in the reference).  The
+ * step's kernels take a block order (0: blockIdx.x, workgroups round-robin
+ * over the 8 XCDs; 1: one contiguous range of blocks per XCD) for the tile
+ * pass, the grid update and the post pass, and a block size of 64, 128 or
+ * 256 for the grid update; every choice computes the same results bit for
+ * bit (the orders are bijections of the blocks; the contact-force partials
+ * are one per wave of nodes whatever the block size).  Defaults: tile 0,
+ * grid 1 / 256, post 1.  mpmw_set_autotune(s, trials) with trials > 0 tunes
+ * on the next steps: each kernel class in turn, each candidate for `trials`
+ * steps without a grid rebuild, timed with events (such steps run without
+ * graph replay and wait for their kernels), the median kept, the fastest
+ * adopted; 0 stops tuning and keeps the configuration in use.
+ * mpmw_set_launch_config sets one; mpmw_get_tuning reports the one in use,
+ * the state and the medians. */
+typedef struct mpmw_tuning {
+    int32_t state;          /* 0 default or set, 1 tuning, 2 tuned */
+    int32_t tile_order, grid_order, grid_block, post_order;
+    int32_t trials;
+    float tile_ms[2];       /* medians per candidate: tile order 0, 1 */
+    float grid_ms[6];       /* grid (order, block): (1,256) (1,128) (1,64) (0,256) (0,128) (0,64) */
+    float post_ms[2];       /* post order 1, 0 */
+} mpmw_tuning;
+int mpmw_set_autotune(mpmw_solver *s, int32_t trials);
+int mpmw_set_launch_config(mpmw_solver *s, const mpmw_tuning *cfg);
+int mpmw_get_tuning(mpmw_solver *s, mpmw_tuning *out);
+
 /* --- APIC transfer variant: quadratic B-spline weights over 27 nodes with
  *     base = floor(x/h - 0.5) (getWeightsAndGradients, MPMSolver.cu:20-34),
  *     P2G momentum m w (v + C (x_i - x_p)) with none from FIXED_BOUNDARY
@@ -366,6 +392,11 @@ int mpmw_batch_synchronize(mpmw_batch *b);
  * every member's launches; a member joining or leaving captures anew). */
 int mpmw_batch_set_graph(mpmw_batch *b, int32_t on);
 int mpmw_batch_graph_stats(mpmw_batch *b, int64_t out[2]);
+/* mpmw_set_autotune / mpmw_set_launch_config / mpmw_get_tuning for the
+ * batch's step (one configuration for all its members). */
+int mpmw_batch_set_autotune(mpmw_batch *b, int32_t trials);
+int mpmw_batch_set_launch_config(mpmw_batch *b, const mpmw_tuning *cfg);
+int mpmw_batch_get_tuning(mpmw_batch *b, mpmw_tuning *out);
 
 /* Layout diagnostics of the particle binning. */
 typedef struct mpmw_diagnostics {
