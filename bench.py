@@ -213,27 +213,33 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25, graph=Fals
     }
 
 
-def batched_bench(cfgs, precision, steps, graph=False):
+def batched_bench(cfgs, precision, steps, graph=False, autotune=0, serial=None):
     """The batched workload (64 workpieces of uneven size): wall time of the
     serial plain solves (one solver after another, each set up, stepped and
     synchronised) against the same set-ups stepped together in one batch.
-    Set-up (generating each slab on the device) counts in both."""
+    Set-up (generating each slab on the device) counts in both.  With
+    `autotune` the batch tunes its launch configuration during its first
+    steps (counted in the time); `serial` reuses an earlier serial time."""
     P = lambda c: dict(c["params"], precision=mpmw.FP64 if precision == "fp64" else mpmw.FP32)
     nsim = len(cfgs)
-    parts = 0
-    t0 = time.perf_counter()
-    for c in cfgs:
-        s = mpmw.Solver(**P(c))
-        workloads.apply_to_solver(mpmw, s, c)
-        parts += s.count()
-        s.run(steps, c["dt"])
-        s.synchronize()
-        s.close()
-    serial = time.perf_counter() - t0
+    parts = sum(int(np.prod(c["cells"])) * 8 for c in cfgs)
+    if serial is None:
+        parts = 0
+        t0 = time.perf_counter()
+        for c in cfgs:
+            s = mpmw.Solver(**P(c))
+            workloads.apply_to_solver(mpmw, s, c)
+            parts += s.count()
+            s.run(steps, c["dt"])
+            s.synchronize()
+            s.close()
+        serial = time.perf_counter() - t0
     t0 = time.perf_counter()
     solvers = []
     b = mpmw.Batch()
     b.set_graph(graph)
+    if autotune:
+        b.set_autotune(autotune)
     for c in cfgs:
         s = mpmw.Solver(**P(c))
         workloads.apply_to_solver(mpmw, s, c)
@@ -245,6 +251,7 @@ def batched_bench(cfgs, precision, steps, graph=False):
     b.synchronize()
     batched = time.perf_counter() - t0
     graph_stats = b.graph_stats()
+    tuning = b.tuning() if autotune else None
     b.close()
     for s in solvers:
         s.close()
@@ -257,6 +264,7 @@ def batched_bench(cfgs, precision, steps, graph=False):
         "speedup": serial / batched,
         "batched_particle_steps_per_s": parts * steps / batched,
         "graph": graph_stats if graph else None,
+        "tuning": tuning,
     }
 
 
@@ -504,6 +512,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 timing of the same workload")
     ap.add_argument("--graph", type=int, default=1, help="1: also time the step replayed as HIP graphs")
+    ap.add_argument("--autotune", type=int, default=2,
+                    help="trials per candidate of the autotuned batched leg (0 skips it)")
     ap.add_argument("--tet-tool-steps", type=int, default=20,
                     help="steps of the coupled MPM + tet tool + hash contact leg (0 skips it)")
     ap.add_argument("--ftrack-steps", type=int, default=50,
@@ -615,6 +625,13 @@ def main():
             g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True)
             out["batched"]["graph"] = {k: g[k] for k in ("serial_s", "batched_s", "speedup", "graph",
                                                          "batched_sims_per_hour")}
+        if args.autotune > 0:
+            # the autotuned batch (tuning inside its timed steps), against the
+            # same serial time
+            a = batched_bench(bcfg, args.precision, args.batch_steps, autotune=args.autotune,
+                              serial=out["batched"]["serial_s"])
+            out["batched"]["autotuned"] = {k: a[k] for k in ("batched_s", "speedup", "batched_sims_per_hour",
+                                                             "tuning")}
     if args.ftrack_steps > 0:
         out["ftrack"] = ftrack_bench(args.ftrack_steps, 5)
     if args.tet_tool_steps > 0:
