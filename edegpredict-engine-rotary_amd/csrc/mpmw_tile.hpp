@@ -569,6 +569,8 @@ template <typename R> struct TileWaves {
 
 // Slots of the per-wave count table of an uneven tile (tile_block).
 constexpr int kSlotTab = 32;
+// Entries of a tile's queue of particles scattered after its slot loop.
+constexpr int kExcQueue = 512;
 
 // Zeros as the window of a tile without particles: the grid update adds
 // every window that holds a node without testing the tile for particles.
@@ -606,6 +608,8 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
     __shared__ int32_t s_cnt[2][kBlock / kWave];
     __shared__ int32_t s_red[3][kBlock / kWave];
     __shared__ int32_t s_tab[kBlock / kWave][kSlotTab];   // uneven tiles: cells per wave holding more than s
+    __shared__ int32_t s_q[DO_P2G && !DET ? kExcQueue : 1];  // particles whose P2G left their home cell
+    __shared__ int32_t s_qn;
     vec4_t<R> *s_node = reinterpret_cast<vec4_t<R> *>(smem);
     R *s_acc = DO_P2G ? reinterpret_cast<R *>(smem + kNodeBytes) : nullptr;
 
@@ -644,6 +648,7 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
         if (DO_G2P) load_g2p_in(P, cnt > 0 && b0 + tid < b1 ? b0 + tid : b0, nxt);
         if (DO_P2G) {
             for (int w = tid; w < 8 * WN; w += kBlock) s_acc[w] = R(0);
+            if (tid == 0) s_qn = 0;  // before the count reduction's barrier
         }
         if (DO_G2P) {
 #pragma unroll
@@ -813,10 +818,24 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
                     ++my_exc;
                 } else {
                     ++my_oow;
-                    my_acc |= scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz, p.T,
-                                                p.s, fex, fey, fez, stress_force, ext_only)
-                                  ? 1
-                                  : 0;
+                    // queued for the scatter after the slot loop, where the
+                    // queued particles take the lanes together instead of
+                    // every slot of every wave holding one waiting on its
+                    // compare-and-swap adds; scattered here when the queue
+                    // is full
+                    const unsigned long long em = __ballot(1);
+                    const int leader = __ffsll((unsigned long long)em) - 1;
+                    int qb = 0;
+                    if (lane == leader) qb = atomicAdd(&s_qn, __popcll(em));
+                    const int q = __shfl(qb, leader, kWave) + __popcll(em & lt);
+                    if (q < kExcQueue) {
+                        s_q[q] = i;
+                    } else {
+                        my_acc |= scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz,
+                                                    p.T, p.s, fex, fey, fez, stress_force, ext_only)
+                                      ? 1
+                                      : 0;
+                    }
                 }
             }
         };
@@ -847,6 +866,32 @@ __device__ __forceinline__ void tile_block(const Particles<R> &P, const GridView
             __syncthreads();  // layer z written by wave z only
             combine_layer(A, s_acc, hx, hy, hz, 1);
             __syncthreads();  // layer z+1 written by wave z only
+            if (!DET) {
+                // the queued particles, one per thread: their state as the
+                // G2P above stored it (the barriers order those stores
+                // before these loads); the external force was consumed by
+                // the G2P of a fused pass
+                const int nq = min(s_qn, kExcQueue);
+                if (nq > 0) {  // uniform over the workgroup
+                    for (int j = tid; j < nq; j += kBlock) {
+                        const int i = s_q[j];
+                        PState<R> p;
+                        p.status = P.status[i];
+                        p.x = P.x[i]; p.y = P.y[i]; p.z = P.z[i];
+                        R fex = R(0), fey = R(0), fez = R(0);
+                        load_p2g_state(P, i, p, fex, fey, fez);
+                        if (DO_G2P) fex = fey = fez = R(0);
+                        const R mass = P.m ? P.m[i] : P.m0, vol = P.m ? P.V[i] : P.V0;
+                        const Frac<R> f = base_and_frac(g, p.x, p.y, p.z);
+                        my_acc |= scatter_exception(g, G.acc, s_acc, f, wx0, wy0, wz0, mass, vol, p.vx, p.vy, p.vz,
+                                                    p.T, p.s, fex, fey, fez, p.status == kActive || p.status == kFixed,
+                                                    p.status == kChip)
+                                      ? 1
+                                      : 0;
+                    }
+                    __syncthreads();  // the queued scatters are in
+                }
+            }
             store_window<R, WN>(tpart + (int64_t)t * (8 * WN), s_acc, tid);
         }
         int cntv = my_oow, cnte = my_exc, cnta = my_acc;
