@@ -57,7 +57,7 @@ struct StepArgs {
     int32_t lod_enabled, chip_step;
     double fixture, fixture_damping, gravity;
     const double *tool_vel;      // 3 per tool node
-    unsigned long long *islands;   // [0] kinematics, [1] Newton
+    unsigned long long *islands;   // [0] kinematics, [1] Newton, [2] P2G particles outside their tile's window
 };
 
 // The step's scalars live in device memory, written by this one-thread
@@ -408,22 +408,72 @@ __device__ __forceinline__ void constitutive(const Particles<R> &P, int i, const
     P.s[3][i] = sig[1]; P.s[4][i] = sig[2]; P.s[5][i] = sig[5];
 }
 
+// Eight channel values as one record (two 16-byte or 32-byte accesses).
+__device__ __forceinline__ void store_record(float *p, const float v[8]) {
+    reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<float4 *>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ void store_record(double *p, const double v[8]) {
+#pragma unroll
+    for (int h = 0; h < 8; h += 2) reinterpret_cast<double2 *>(p)[h / 2] = make_double2(v[h], v[h + 1]);
+}
+__device__ __forceinline__ void load_record(const float *p, float v[8]) {
+    const float4 a = reinterpret_cast<const float4 *>(p)[0], b = reinterpret_cast<const float4 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void load_record(const double *p, double v[8]) {
+#pragma unroll
+    for (int h = 0; h < 8; h += 2) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[h / 2];
+        v[h] = a.x; v[h + 1] = a.y;
+    }
+}
+
+// Base-node tiles of the P2G: TX x TY x TZ base nodes of the grid sub-box,
+// the particles of a tile contiguous after each fit's sort (key: tile, then
+// the base node inside it, x fastest), their stencils inside a window of
+// (TX + 2) x (TY + 2) x (TZ + 2) nodes.
+constexpr int kTX = 8, kTY = 8, kTZ = 4;
+constexpr int kWX = kTX + 2, kWY = kTY + 2, kWZ = kTZ + 2, kWN = kWX * kWY * kWZ;
+constexpr int kRounds = 3;   // window-add rounds of a wave's heads sharing a base node
+
 // p2gKernel (MPMSolver.cu:124-412) with the LOD classification of the step
-// (lodClassifyKernel :655-676) folded in: one thread per particle.  The
-// particles are sorted by their stencil's base node, so consecutive lanes of
-// a wave mostly share a base node and hence all 27 target nodes: each of a
-// node's 7 channel values is summed over each run of equal base nodes in the
-// wave (a segmented suffix sum by shuffles, log2 64 steps) and only the
-// run's first lane issues the device atomics, ~8 times fewer than one per
-// particle and without lanes of a wave contending for one address.  A run
-// ends wherever the base node changes, so an unsorted order gives the same
-// sums (in another order), only with more atomics.
+// (lodClassifyKernel :655-676) folded in: one workgroup per base-node tile,
+// its particles [tile_start[t], tile_start[t + 1]) one per thread in chunks
+// of the workgroup.  Consecutive lanes mostly share a base node and hence
+// all 27 target nodes: each of a node's 7 channel values is summed over
+// each run of equal base nodes in the wave (a segmented suffix sum by
+// shuffles) and the run's first lane adds the sums to its wave's LDS window
+// of the tile with plain read-add-writes, heads sharing a base node in
+// successive rounds (a particle whose base node has left the tile since the
+// sort adds to the device grid directly).  The four windows'
+// sum is stored whole to tpart[t] (kWN records of 8 channels, the last
+// unused), which k_grid adds up: no device atomics for the window, whose
+// nodes are mostly shared with neighbouring tiles (the device atomics were
+// 85 % of a per-particle form's time, and a window flush by atomics still a
+// third of this kernel's).
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R> M, const StepArgs *__restrict__ sap) {
+__global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R> M, const StepArgs *__restrict__ sap,
+                                                const int32_t *__restrict__ tile_start, R *__restrict__ tpart) {
+    constexpr int kWaves = kBlock / 64;
+    __shared__ R s_w[kWaves * 7 * kWN];
+    __shared__ int32_t s_own[kWaves * kTX * kTY * kTZ];   // per wave: the head owning a base node this round
     const StepArgs sa = *sap;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int t = blockIdx.x;
+    const int ts = tile_start[t], te = tile_start[t + 1];
+    if (ts >= te) return;   // uniform over the workgroup
+    const int ntx = (g.nb[0] + kTX - 1) / kTX, nty = (g.nb[1] + kTY - 1) / kTY;
+    // the tile's first base node in lattice coordinates
+    const int tx0 = g.lo[0] + (t % ntx) * kTX, ty0 = g.lo[1] + ((t / ntx) % nty) * kTY,
+              tz0 = g.lo[2] + (t / (ntx * nty)) * kTZ;
+    for (int w = threadIdx.x; w < kWaves * 7 * kWN; w += kBlock) s_w[w] = R(0);
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    bool live = i < P.n;
+    R *const s_wave = s_w + (threadIdx.x / 64) * (7 * kWN);
+    int missed = 0;
+    for (int c0 = ts; c0 < te; c0 += kBlock) {   // uniform over the workgroup
+    const int i = c0 + threadIdx.x;
+    bool live = i < te;
     int32_t st = live ? P.status[i] : kInactive;
     live = live && st != kInactive;
     double x = 0, y = 0, z = 0;
@@ -490,7 +540,32 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     const bool head = live && starts;
     const R pvx = fixed ? R(0) : vx, pvy = fixed ? R(0) : vy, pvz = fixed ? R(0) : vz;
     const R mm = fixed ? R(0) : m;   // FIXED_BOUNDARY particles add no momentum
-    int missed = 0;
+    // the run's base node in the tile: its 27 nodes are in the window
+    const int lbx = ax.b - tx0, lby = ay.b - ty0, lbz = az.b - tz0;
+    // Heads of runs with the same base node (a particle that crossed a
+    // base-node plane since the sort starts a run of its own) must not add
+    // to one window node in one instruction: each head claims its base node
+    // in the wave's owner table, one winner per node and round, and the
+    // rounds add one after another (a wave's LDS operations complete in
+    // order).  A head with no round left, or whose base node has left the
+    // tile, adds to the device grid.
+    const bool inwin = (unsigned)lbx < (unsigned)kTX && (unsigned)lby < (unsigned)kTY && (unsigned)lbz < (unsigned)kTZ;
+    int myround = head && inwin ? -1 : kRounds;
+    {
+        volatile int32_t *own = s_own + (threadIdx.x / 64) * (kTX * kTY * kTZ) + ((lbz * kTY + lby) * kTX + lbx);
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if (myround == -1) *own = lane;
+            if (myround == -1 && *own == lane) myround = r;
+        }
+    }
+    const bool lds_ok = myround >= 0 && myround < kRounds;
+    const int nrounds = __ballot(myround == 2) ? 3 : (__ballot(myround == 1) ? 2 : (__ballot(myround == 0) ? 1 : 0));
+    static_assert(kRounds == 3, "round count");
+    // particles whose run goes to the device grid (the host re-sorts when
+    // they grow)
+    if (head && !lds_ok) atomicAdd(sa.islands + 2, (unsigned long long)(run_end - lane));
+    R *const wp = s_wave + (lbz * kWY + lby) * kWX + lbx;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -525,11 +600,21 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                         if (same[k]) ch[q] += o;
                     }
                 }
+                const int64_t n = head ? box_index(g, gx, gy, gz) : -1;
+                if (lds_ok && n >= 0) {
+                    R *const q = wp + (c * kWY + b) * kWX + a;
+#pragma unroll
+                    for (int r = 0; r < kRounds; ++r) {
+                        if (r < nrounds && myround == r) {
+#pragma unroll
+                            for (int h = 0; h < 7; ++h) q[h * kWN] += ch[h];
+                        }
+                    }
+                }
                 if (head) {
-                    const int64_t n = box_index(g, gx, gy, gz);
                     if (n < 0) {
                         ++missed;
-                    } else {
+                    } else if (!lds_ok) {
                         unsafeAtomicAdd(g.mass + n, ch[0]);
                         if (ch[1] != R(0) || ch[2] != R(0) || ch[3] != R(0)) {
                             unsafeAtomicAdd(g.p[0] + n, ch[1]);
@@ -542,7 +627,62 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                     }
                 }
             }
+    }
+    __syncthreads();
+    // the window, its waves' parts summed in wave order (a node outside the
+    // sub-box holds zeros: its visits were counted as missed)
+    R *const dst = tpart + (int64_t)t * (kWN * 8);
+    for (int w = threadIdx.x; w < kWN; w += kBlock) {
+        R v[8];
+#pragma unroll
+        for (int h = 0; h < 7; ++h) {
+            v[h] = s_w[h * kWN + w];
+#pragma unroll
+            for (int k = 1; k < kWaves; ++k) v[h] += s_w[(k * 7 + h) * kWN + w];
+        }
+        v[7] = R(0);
+        store_record(dst + 8 * w, v);
+    }
     if (missed) atomicAdd(g.overflow, (unsigned long long)missed);
+}
+
+// Adds the windows of the non-empty tiles whose window holds lattice node
+// (gx, gy, gz) (box coordinates i, j, k) to c[0..7): tile (tx, ty, tz) holds
+// nodes [kTX tx, kTX tx + kWX) x ... of the box, so at most two tiles per
+// axis, in a fixed order.
+template <typename R>
+__device__ __forceinline__ void gather_windows(const Grid<R> &g, const R *__restrict__ tpart,
+                                               const int32_t *__restrict__ tile_start, int i, int j, int k, R c[7]) {
+    const int ntx = (g.nb[0] + kTX - 1) / kTX, nty = (g.nb[1] + kTY - 1) / kTY;
+    const int x1 = i / kTX, y1 = j / kTY, z1 = k / kTZ;
+    const int nx = (i % kTX < kWX - kTX && x1 > 0) ? 2 : 1, ny = (j % kTY < kWY - kTY && y1 > 0) ? 2 : 1,
+              nz = (k % kTZ < kWZ - kTZ && z1 > 0) ? 2 : 1;
+    for (int dz = 0; dz < nz; ++dz)
+        for (int dy = 0; dy < ny; ++dy)
+            for (int dx = 0; dx < nx; ++dx) {
+                const int tx = x1 - dx, ty = y1 - dy, tz = z1 - dz;
+                const int t = (tz * nty + ty) * ntx + tx;
+                if (tile_start[t] == tile_start[t + 1]) continue;   // an empty tile stores no window
+                const int w = ((k - kTZ * tz) * kWY + (j - kTY * ty)) * kWX + (i - kTX * tx);
+                R v[8];
+                load_record(tpart + ((int64_t)t * kWN + w) * 8, v);
+#pragma unroll
+                for (int h = 0; h < 7; ++h) c[h] += v[h];
+            }
+}
+
+// The tile ranges of the sorted keys: tile_start[t] = first particle of
+// tile t (keys of INACTIVE particles, 0xFFFFFFFF, sort after every tile);
+// thread i in [0, n] writes the starts of the tiles after its predecessor's
+// up to its own, so every t in [0, ntiles] is written once.
+static __global__ void __launch_bounds__(kBlock) k_tile_starts(const uint32_t *__restrict__ keys, int n, int ntiles,
+                                                        int32_t *__restrict__ tile_start) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > n) return;
+    auto tile = [&](int k) { return keys[k] == 0xFFFFFFFFu ? ntiles : (int)(keys[k] >> 8); };
+    const int ti = i < n ? tile(i) : ntiles;
+    const int tp = i > 0 ? tile(i - 1) : -1;
+    for (int t = tp + 1; t <= ti; ++t) tile_start[t] = i;
 }
 
 // rasterizeToolKernel (:444-473): each tool node marks the 2 x 2 x 2 nodes
@@ -563,19 +703,31 @@ template <typename R> __global__ void __launch_bounds__(kBlock) k_raster(Grid<R>
 
 // updateGridKernel (:424-442), applyFixtureBCKernel (:81-111) and
 // applyGridCollisionKernel (:475-492) on one node of the sub-box.
-template <typename R> __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g, const StepArgs *__restrict__ sap) {
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g, const StepArgs *__restrict__ sap, const R *__restrict__ tpart,
+                                                 const int32_t *__restrict__ tile_start) {
     const StepArgs sa = *sap;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (n >= g.nn) return;
-    const R m = g.mass[n];
-    R px = g.p[0][n], py = g.p[1][n], pz = g.p[2][n], vx = 0, vy = 0, vz = 0;
+    // the P2G sums: the tile windows holding the node and the device
+    // atomics of the particles outside their tile; the totals are kept for
+    // the G2P (mass) and the grid read-back (mass, force)
+    R c[7] = {g.mass[n], g.p[0][n], g.p[1][n], g.p[2][n], g.f[0][n], g.f[1][n], g.f[2][n]};
+    {
+        const int i = (int)(n % g.nb[0]), j = (int)((n / g.nb[0]) % g.nb[1]), k = (int)(n / ((int64_t)g.nb[0] * g.nb[1]));
+        gather_windows(g, tpart, tile_start, i, j, k, c);
+    }
+    g.mass[n] = c[0];
+    g.f[0][n] = c[4]; g.f[1][n] = c[5]; g.f[2][n] = c[6];
+    const R m = c[0];
+    R px = c[1], py = c[2], pz = c[3], vx = 0, vy = 0, vz = 0;
     const R dt = (R)sa.dt;
     const bool live = m > R(1e-12);
     if (live) {
         pz += m * (R)sa.gravity * dt;
-        px += g.f[0][n] * dt;
-        py += g.f[1][n] * dt;
-        pz += g.f[2][n] * dt;
+        px += c[4] * dt;
+        py += c[5] * dt;
+        pz += c[6] * dt;
         vx = px / m; vy = py / m; vz = pz / m;
     }
     if (sa.fixture > 0.0) {
@@ -591,11 +743,11 @@ template <typename R> __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g
             }
         }
     }
-    const int32_t c = g.claim[n];
-    if (live && c > 0) {
-        vx = (R)sa.tool_vel[3 * (c - 1)];
-        vy = (R)sa.tool_vel[3 * (c - 1) + 1];
-        vz = (R)sa.tool_vel[3 * (c - 1) + 2];
+    const int32_t cl = g.claim[n];
+    if (live && cl > 0) {
+        vx = (R)sa.tool_vel[3 * (cl - 1)];
+        vy = (R)sa.tool_vel[3 * (cl - 1) + 1];
+        vz = (R)sa.tool_vel[3 * (cl - 1) + 2];
         px = m * vx; py = m * vy; pz = m * vz;
     }
     g.p[0][n] = px; g.p[1][n] = py; g.p[2][n] = pz;
@@ -774,7 +926,9 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(Particles<R> P, Grid<R> g,
         const int by = (int)floor((P.y[i] - g.oy) * g.inv_dx - 0.5) - g.lo[1];
         const int bz = (int)floor((P.z[i] - g.oz) * g.inv_dx - 0.5) - g.lo[2];
         const int cx = min(max(bx, 0), g.nb[0] - 1), cy = min(max(by, 0), g.nb[1] - 1), cz = min(max(bz, 0), g.nb[2] - 1);
-        k = (uint32_t)(((int64_t)cz * g.nb[1] + cy) * g.nb[0] + cx);
+        const int ntx = (g.nb[0] + kTX - 1) / kTX, nty = (g.nb[1] + kTY - 1) / kTY;
+        const int tile = ((cz / kTZ) * nty + cy / kTY) * ntx + cx / kTX;
+        k = (uint32_t)tile * (uint32_t)(kTX * kTY * kTZ) + (uint32_t)(((cz % kTZ) * kTY + cy % kTY) * kTX + cx % kTX);
     }
     keys[i] = k;
     idx[i] = i;

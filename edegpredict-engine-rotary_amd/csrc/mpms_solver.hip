@@ -46,6 +46,7 @@ enum Field {
 constexpr int kFirstInt = ID;
 constexpr int kFitEvery = 25;     // steps between grid fits (and particle sorts)
 constexpr int kMargin = 4;        // nodes of margin around the particles' stencils
+constexpr double kResortFraction = 0.05;   // particles outside their tile window that trigger a re-sort
 constexpr int kStatBlocks = 1024;
 
 inline int blocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
@@ -65,7 +66,8 @@ struct mpms_solver {
     int64_t nn = 0, nn_cap = 0;
     void *gblock = nullptr;           // mass, p xyz, f xyz, v xyz (R), claim (int32)
     bool fitted = false;
-    unsigned long long *d_counts = nullptr;   // [0] overflow, [1] island kinematics, [2] island Newton
+    unsigned long long *d_counts = nullptr;   // [0] overflow, [1] island kinematics, [2] island Newton,
+                                              // [3] P2G particles outside their tile's window (last step)
     // tool
     double *d_tool_xyz = nullptr, *d_tool_vel = nullptr;
     int32_t ntool = 0, tool_cap = 0;
@@ -73,11 +75,17 @@ struct mpms_solver {
     // stepping
     double time = 0.0;
     int64_t step = 0, fits = 0, sorts = 0;
+    int64_t last_fit = 0;             // step of the last fit
     // reductions and sort scratch
     double *d_part = nullptr;
     uint32_t *keys = nullptr, *keys2 = nullptr;
     int32_t *idx = nullptr, *idx2 = nullptr;
     int32_t sort_cap = 0;
+    // the P2G's base-node tiles: first particle of each (ntiles + 1)
+    int32_t *tile_start = nullptr;
+    int32_t ntiles = 0, tile_cap = 0;
+    void *tpart = nullptr;            // the tiles' P2G windows (kWN records of 8 channels each)
+    int64_t tpart_cap = 0;            // bytes
     mpmw::ScratchBuf temp;
     // the step's scalars on the device (k_step_args) and its captured
     // launches: one graph per launch configuration, the two most recent kept
@@ -324,7 +332,8 @@ template <typename R> int stats(mpms_solver *s, double out[kStats]) {
 
 // The grid fit (every kFitEvery steps): the overflow count of the last
 // interval must be zero; the sub-box covers every non-INACTIVE particle's
-// stencil plus kMargin nodes; the particles are sorted by base node.
+// stencil plus kMargin nodes; the particles are sorted by base-node tile and
+// base node, and the tiles' particle ranges found.
 template <typename R> int fit(mpms_solver *s) {
     unsigned long long cnt[3] = {0, 0, 0};
     MS_TRY(hipMemcpyAsync(cnt, s->d_counts, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
@@ -385,6 +394,38 @@ template <typename R> int fit(mpms_solver *s) {
                        s->idx);
     MS_TRY(hipGetLastError());
     MS_TRY(mpmw::radix_sort_pairs(s->temp, s->keys, s->keys2, s->idx, s->idx2, n, 32, s->stream));
+    {
+        int64_t nt = 1;
+        const int tdim[3] = {kTX, kTY, kTZ};
+        for (int a = 0; a < 3; ++a) nt *= (s->nb[a] + tdim[a] - 1) / tdim[a];
+        if (nt * (kTX * kTY * kTZ) >= (int64_t)0xFFFFFFFFu) return fail(MPMW_E_RANGE, "grid sub-box too large for the sort keys");
+        if (nt + 1 > s->tile_cap) {
+            if (s->tile_start) (void)hipFree(s->tile_start);
+            s->tile_start = nullptr;
+            s->tile_cap = 0;
+            const int64_t c = nt + 1 + nt / 4;
+            if (hipMalloc(&s->tile_start, sizeof(int32_t) * (size_t)c) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MPMW_E_NOMEM, "hipMalloc of the tile ranges failed");
+            }
+            s->tile_cap = (int32_t)c;
+        }
+        s->ntiles = (int32_t)nt;
+        const int64_t tb = nt * (int64_t)kWN * 8 * (int64_t)s->rsz;
+        if (tb > s->tpart_cap) {
+            if (s->tpart) (void)hipFree(s->tpart);
+            s->tpart = nullptr;
+            s->tpart_cap = 0;
+            const int64_t c = tb + tb / 4;
+            if (hipMalloc(&s->tpart, (size_t)c) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MPMW_E_NOMEM, "hipMalloc of the tile windows failed");
+            }
+            s->tpart_cap = c;
+        }
+        hipLaunchKernelGGL(k_tile_starts, dim3(blocks(n + 1)), dim3(kBlock), 0, s->stream, (const uint32_t *)s->keys2,
+                           n, s->ntiles, s->tile_start);
+    }
     for (int k = 0; k < NFIELD; ++k) {
         if (k <= Z || (s->fp64 && k < kFirstInt))
             hipLaunchKernelGGL(k_gather<double>, dim3(blocks(n)), dim3(kBlock), 0, s->stream, (const double *)s->f[k],
@@ -399,6 +440,7 @@ template <typename R> int fit(mpms_solver *s) {
     s->fitted = true;
     s->fits++;
     s->sorts++;
+    s->last_fit = s->step;
     return MPMW_OK;
 }
 
@@ -415,7 +457,9 @@ void drop_graphs(mpms_solver *s) {
 std::vector<uint64_t> graph_key(const mpms_solver *s) {
     std::vector<uint64_t> k = {(uint64_t)s->n, (uint64_t)s->nn, (uint64_t)s->nn_cap, (uint64_t)s->ntool,
                                (uint64_t)(uintptr_t)s->gblock, (uint64_t)(uintptr_t)s->d_tool_xyz,
-                               (uint64_t)(uintptr_t)s->d_tool_vel, (uint64_t)(uintptr_t)s->d_sa};
+                               (uint64_t)(uintptr_t)s->d_tool_vel, (uint64_t)(uintptr_t)s->d_sa,
+                               (uint64_t)s->ntiles, (uint64_t)(uintptr_t)s->tile_start,
+                               (uint64_t)(uintptr_t)s->tpart};
     for (int a = 0; a < 3; ++a) k.push_back(((uint64_t)(uint32_t)s->lo[a] << 32) | (uint32_t)s->nb[a]);
     for (int f = 0; f < NFIELD; ++f) k.push_back((uint64_t)(uintptr_t)s->f[f]);
     return k;
@@ -467,8 +511,21 @@ template <typename Issue> int graph_step(mpms_solver *s, Issue &&issue) {
 template <typename R> int step_impl(mpms_solver *s, double dt) {
     if (s->n <= 0) return MPMW_OK;   // MPMSolver::step :1063 silent return
     int rc;
-    if (!s->fitted || s->step % kFitEvery == 0)
+    if (!s->fitted || s->step % kFitEvery == 0) {
         if ((rc = fit<R>(s))) return rc;
+    } else if (s->step - s->last_fit == 2 || s->step - s->last_fit == 6 || s->step - s->last_fit == 12) {
+        // The sort order decays as particles cross base-node planes: their
+        // runs break up, or leave the tile, and take the device atomics.
+        // A lattice start decays at once (half the particles sit on the
+        // planes and the first motion moves them across).  Three checks
+        // per fit interval read the last step's count (one synchronisation
+        // each) and fit again when more than 5 % of the particles left.
+        unsigned long long out = 0;
+        MS_TRY(hipMemcpyAsync(&out, s->d_counts + 3, sizeof(out), hipMemcpyDeviceToHost, s->stream));
+        MS_TRY(hipStreamSynchronize(s->stream));
+        if ((double)out > kResortFraction * (double)s->n)
+            if ((rc = fit<R>(s))) return rc;
+    }
     const Particles<R> P = pview<R>(s);
     const Grid<R> g = gview<R>(s);
     StepArgs sa;
@@ -495,12 +552,14 @@ template <typename R> int step_impl(mpms_solver *s, double dt) {
         hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<int64_t>(blocks(s->nn), 2048)), dim3(kBlock), 0,
                            s->stream, (uint32_t *)g.claim, s->nn);
         hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBlock), 0, s->stream, (uint32_t *)(s->d_counts + 1),
-                           (int64_t)4);
-        hipLaunchKernelGGL(k_p2g<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, material<R>(s), dsa);
+                           (int64_t)6);
+        hipLaunchKernelGGL(k_p2g<R>, dim3((unsigned)s->ntiles), dim3(kBlock), 0, s->stream, P, g, material<R>(s), dsa,
+                           (const int32_t *)s->tile_start, (R *)s->tpart);
         if (s->ntool > 0)
             hipLaunchKernelGGL(k_raster<R>, dim3(blocks(s->ntool)), dim3(kBlock), 0, s->stream, g,
                                (const double *)s->d_tool_xyz, s->ntool);
-        hipLaunchKernelGGL(k_grid<R>, dim3(blocks(s->nn)), dim3(kBlock), 0, s->stream, g, dsa);
+        hipLaunchKernelGGL(k_grid<R>, dim3(blocks(s->nn)), dim3(kBlock), 0, s->stream, g, dsa, (const R *)s->tpart,
+                           (const int32_t *)s->tile_start);
         hipLaunchKernelGGL(k_g2p<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, dsa,
                            (R)s->prm.damage_threshold);
     };
@@ -610,7 +669,8 @@ int mpms_destroy(mpms_solver *s) {
     free_fields(s->f, s);
     free_fields(s->alt, s);
     for (void *p : {(void *)s->gblock, (void *)s->d_counts, (void *)s->d_part, (void *)s->d_sa, (void *)s->d_tool_xyz,
-                    (void *)s->d_tool_vel, (void *)s->keys, (void *)s->keys2, (void *)s->idx, (void *)s->idx2})
+                    (void *)s->d_tool_vel, (void *)s->keys, (void *)s->keys2, (void *)s->idx, (void *)s->idx2,
+                    (void *)s->tile_start, (void *)s->tpart})
         if (p) (void)hipFree(p);
     mpmw::free_scratch(s->temp);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -802,10 +862,11 @@ int mpms_get_grid(mpms_solver *s, int32_t lo[3], int32_t n[3], double *mass, dou
 int mpms_get_diagnostics(mpms_solver *s, mpms_diagnostics *d) {
     MS_CHECK(s);
     if (!d) return fail(MPMW_E_INVAL, "null");
-    unsigned long long cnt[3] = {0, 0, 0};
+    unsigned long long cnt[4] = {0, 0, 0, 0};
     MS_TRY(hipMemcpyAsync(cnt, s->d_counts, sizeof(cnt), hipMemcpyDeviceToHost, s->stream));
     MS_TRY(hipStreamSynchronize(s->stream));
     for (int a = 0; a < 3; ++a) { d->box_lo[a] = s->lo[a]; d->box_n[a] = s->nb[a]; }
+    d->out_of_tile = (int64_t)cnt[3];
     d->fits = s->fits;
     d->sorts = s->sorts;
     d->overflow = (int64_t)cnt[0];

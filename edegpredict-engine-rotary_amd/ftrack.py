@@ -39,7 +39,8 @@ class Arrays(C.Structure):
 
 class Diagnostics(C.Structure):
     _fields_ = [("box_lo", C.c_int32 * 3), ("box_n", C.c_int32 * 3), ("fits", C.c_int64), ("sorts", C.c_int64),
-                ("overflow", C.c_int64), ("island_kinematics", C.c_int64), ("island_newton", C.c_int64)]
+                ("overflow", C.c_int64), ("island_kinematics", C.c_int64), ("island_newton", C.c_int64),
+                ("out_of_tile", C.c_int64)]
 
 
 SHAPES = {"pos": 3, "vel": 3, "fext": 3, "mass": 1, "volume": 1, "temperature": 1, "F": 9, "Fp": 9, "C": 9,
@@ -238,7 +239,8 @@ class Solver:
         d = Diagnostics()
         _check(lib().mpms_get_diagnostics(self.h, C.byref(d)), "diagnostics")
         return {"box_lo": tuple(d.box_lo), "box_n": tuple(d.box_n), "fits": d.fits, "sorts": d.sorts,
-                "overflow": d.overflow, "island_kinematics": d.island_kinematics, "island_newton": d.island_newton}
+                "overflow": d.overflow, "island_kinematics": d.island_kinematics, "island_newton": d.island_newton,
+                "out_of_tile": d.out_of_tile}
 
     def set_graph(self, on=True):
         _check(lib().mpms_set_graph(self.h, 1 if on else 0), "mpms_set_graph")

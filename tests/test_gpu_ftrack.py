@@ -308,3 +308,41 @@ def test_ftrack_graph_replay_matches_eager(precision):
     for k in (errs if precision == "fp64" else GATED):
         assert errs[k] <= tol, (k, errs[k])
     assert e["plastic_strain"].max() > 1e-4
+
+
+def test_ftrack_resorts_when_tile_order_decays():
+    """The P2G sums each base-node tile's particles in an LDS window; a
+    particle whose base node has left its tile since the sort adds to the
+    device grid directly, and two, six and twelve steps after a fit the
+    step re-sorts when such particles pass 5 %.  A jittered block moving
+    down at 0.6 cells per step (a quarter of its particles leave their
+    4-node-deep tile every ~2 steps) is re-sorted at step 2; the same block
+    at rest keeps its fit.  A lattice block (half the particles on
+    base-node planes, crossing them at the first motion) is re-sorted too:
+    the crossings inside a tile are absorbed (runs of one base node in one
+    wave add in rounds), those out of its 4-node depth are 11 %."""
+    kw, a, tool, tv = case()
+    rng = np.random.default_rng(3)
+    dx = 5e-4
+    for jitter, vz, fits in ((0.2, -0.6 * dx / 1e-8, 2), (0.2, 0.0, 1), (0.0, 0.0, 2)):
+        d = oracle_view(a)
+        d["pos"] = d["pos"] + rng.uniform(-jitter, jitter, d["pos"].shape) * 2.5e-4
+        d["vel"] = np.tile([0.0, 0.0, vz], (len(d["mass"]), 1))
+        s = ftrack.Solver(ftrack.default_params(**kw, precision=ftrack.FP32))
+        s.set_particles(d)
+        outs = []
+        for _ in range(4):
+            s.step(1e-8)
+            outs.append(s.diagnostics()["out_of_tile"])
+        dg = s.diagnostics()
+        print(f"\n[ftrack tiles jitter={jitter} vz={vz:.3g}] out of tile per step {outs} of {len(d['mass'])}, "
+              f"fits {dg['fits']}", flush=True)
+        assert dg["fits"] == fits, (jitter, vz, dg)
+        if fits == 2:   # re-sorted at step 2, whose P2G starts from fresh tiles
+            assert outs[1] > 0.05 * len(d["mass"]) and outs[2] == 0, outs
+            if not vz:
+                assert outs[3] <= 0.01 * len(d["mass"]), outs
+        else:
+            assert max(outs) <= 0.05 * len(d["mass"]), (jitter, outs)
+        assert dg["overflow"] == 0
+        s.close()
