@@ -754,19 +754,29 @@ __global__ void __launch_bounds__(kBlock) k_grid(Grid<R> g, const StepArgs *__re
     g.v[0][n] = vx; g.v[1][n] = vy; g.v[2][n] = vz;
 }
 
+// A grid node's velocity and mass, as the G2P reads them.
+template <typename R> struct alignas(4 * sizeof(R)) NodeVM {
+    R x, y, z, m;
+};
+
 // g2pKernel (:506-605), resetExternalForceKernel (:682-688) and, on even
 // steps, MPMSolver's chipFractureAndSeparationKernel (:1007-1060, critical
-// damage 1e-6, cut depth 1.5 dx) on one particle.
+// damage 1e-6, cut depth 1.5 dx) on particle i.  The 27 nodes come from the
+// tile window `win` (tile origin t0) when the particle's base node is in
+// the tile, else from the grid; the same values in the same order either
+// way.
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, const StepArgs *__restrict__ sap, R dth) {
-    const StepArgs sa = *sap;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= P.n) return;
+__device__ __forceinline__ void g2p_particle(const Particles<R> &P, const Grid<R> &g, const StepArgs &sa, R dth, int i,
+                                             const NodeVM<R> *win, int tx0, int ty0, int tz0) {
     const int32_t st = P.status[i];
     if (st != kInactive && st != kFixed) {
         const double x = P.x[i], y = P.y[i], z = P.z[i];
         const Axis<R> ax = axis_weights<R>(x - g.ox, g.inv_dx), ay = axis_weights<R>(y - g.oy, g.inv_dx),
                       az = axis_weights<R>(z - g.oz, g.inv_dx);
+        const int lbx = ax.b - tx0, lby = ay.b - ty0, lbz = az.b - tz0;
+        const bool inwin = win && (unsigned)lbx < (unsigned)kTX && (unsigned)lby < (unsigned)kTY &&
+                           (unsigned)lbz < (unsigned)kTZ;
+        const NodeVM<R> *wp = win + (inwin ? (lbz * kWY + lby) * kWX + lbx : 0);
         R v[3] = {0, 0, 0}, B[9], L[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) B[c] = L[c] = R(0);
@@ -780,11 +790,17 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, const
                     const int gx = ax.b + a, gy = ay.b + b, gz = az.b + c;
                     const int64_t n = box_index(g, gx, gy, gz);
                     if (n < 0) { ++missed; continue; }
-                    if (g.mass[n] < R(1e-12)) continue;
+                    R gv[3], gm;
+                    if (inwin) {
+                        const NodeVM<R> r = wp[(c * kWY + b) * kWX + a];
+                        gv[0] = r.x; gv[1] = r.y; gv[2] = r.z; gm = r.m;
+                    } else {
+                        gv[0] = g.v[0][n]; gv[1] = g.v[1][n]; gv[2] = g.v[2][n]; gm = g.mass[n];
+                    }
+                    if (gm < R(1e-12)) continue;
                     const R w = ax.w[a] * ay.w[b] * az.w[c];
                     const R gw[3] = {ax.dw[a] * ay.w[b] * az.w[c], ax.w[a] * ay.dw[b] * az.w[c],
                                      ax.w[a] * ay.w[b] * az.dw[c]};
-                    const R gv[3] = {g.v[0][n], g.v[1][n], g.v[2][n]};
                     const R dp[3] = {(R)((gx * g.dx + g.ox) - x), (R)((gy * g.dx + g.oy) - y),
                                      (R)((gz * g.dx + g.oz) - z)};
 #pragma unroll
@@ -852,6 +868,40 @@ __global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, const
         P.v[0][i] = P.v[1][i] = P.v[2][i] = R(0);
     }
     P.ef[0][i] = P.ef[1][i] = P.ef[2][i] = R(0);
+}
+
+// One workgroup per base-node tile (the P2G's tiles): the tile's window of
+// node velocities and masses is read into LDS once, then its particles one
+// per thread in chunks.  Blocks from ntiles on take the particles after
+// every tile (INACTIVE at the sort: their external force is reset).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_g2p(Particles<R> P, Grid<R> g, const StepArgs *__restrict__ sap, R dth,
+                                                const int32_t *__restrict__ tile_start, int ntiles, int tail_blocks) {
+    const StepArgs sa = *sap;
+    const int t = blockIdx.x;
+    if (t >= ntiles) {   // uniform over the workgroup
+        for (int i = tile_start[ntiles] + (t - ntiles) * kBlock + (int)threadIdx.x; i < P.n; i += tail_blocks * kBlock)
+            g2p_particle<R>(P, g, sa, dth, i, nullptr, 0, 0, 0);
+        return;
+    }
+    __shared__ NodeVM<R> s_n[kWN];
+    const int ts = tile_start[t], te = tile_start[t + 1];
+    if (ts >= te) return;   // uniform over the workgroup
+    const int ntx = (g.nb[0] + kTX - 1) / kTX, nty = (g.nb[1] + kTY - 1) / kTY;
+    const int tx0 = g.lo[0] + (t % ntx) * kTX, ty0 = g.lo[1] + ((t / ntx) % nty) * kTY,
+              tz0 = g.lo[2] + (t / (ntx * nty)) * kTZ;
+    for (int w = threadIdx.x; w < kWN; w += kBlock) {
+        const int64_t n = box_index(g, tx0 + w % kWX, ty0 + (w / kWX) % kWY, tz0 + w / (kWX * kWY));
+        NodeVM<R> r;
+        r.x = r.y = r.z = r.m = R(0);
+        if (n >= 0) { r.x = g.v[0][n]; r.y = g.v[1][n]; r.z = g.v[2][n]; r.m = g.mass[n]; }
+        s_n[w] = r;
+    }
+    __syncthreads();
+    for (int c0 = ts; c0 < te; c0 += kBlock) {
+        const int i = c0 + (int)threadIdx.x;
+        if (i < te) g2p_particle<R>(P, g, sa, dth, i, s_n, tx0, ty0, tz0);
+    }
 }
 
 // applyExternalForceKernel (:607-627)

@@ -47,6 +47,7 @@ constexpr int kFirstInt = ID;
 constexpr int kFitEvery = 25;     // steps between grid fits (and particle sorts)
 constexpr int kMargin = 4;        // nodes of margin around the particles' stencils
 constexpr double kResortFraction = 0.05;   // particles outside their tile window that trigger a re-sort
+constexpr int kTailBlocks = 64;   // G2P blocks over the particles after every tile (INACTIVE at the sort)
 constexpr int kStatBlocks = 1024;
 
 inline int blocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
@@ -560,8 +561,8 @@ template <typename R> int step_impl(mpms_solver *s, double dt) {
                                (const double *)s->d_tool_xyz, s->ntool);
         hipLaunchKernelGGL(k_grid<R>, dim3(blocks(s->nn)), dim3(kBlock), 0, s->stream, g, dsa, (const R *)s->tpart,
                            (const int32_t *)s->tile_start);
-        hipLaunchKernelGGL(k_g2p<R>, dim3(blocks(s->n)), dim3(kBlock), 0, s->stream, P, g, dsa,
-                           (R)s->prm.damage_threshold);
+        hipLaunchKernelGGL(k_g2p<R>, dim3((unsigned)(s->ntiles + kTailBlocks)), dim3(kBlock), 0, s->stream, P, g, dsa,
+                           (R)s->prm.damage_threshold, (const int32_t *)s->tile_start, s->ntiles, kTailBlocks);
     };
     if (!s->graph_on) {
         issue();
