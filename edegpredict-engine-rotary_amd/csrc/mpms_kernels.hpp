@@ -408,6 +408,19 @@ __device__ __forceinline__ void constitutive(const Particles<R> &P, int i, const
     P.s[3][i] = sig[1]; P.s[4][i] = sig[2]; P.s[5][i] = sig[5];
 }
 
+// The value of lane l - D of the same 16-lane row (DPP row_shr:D), zero for
+// the row's first D lanes.
+template <int D> __device__ __forceinline__ float row_shr(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x110 + D, 0xf, 0xf,
+                                                                 true));
+}
+template <int D> __device__ __forceinline__ double row_shr(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), 0x110 + D, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), 0x110 + D, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
 // Eight channel values as one record (two 16-byte or 32-byte accesses).
 __device__ __forceinline__ void store_record(float *p, const float v[8]) {
     reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
@@ -531,13 +544,21 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     const unsigned long long starts_mask = __ballot(starts);
     const unsigned long long later = lane == 63 ? 0ull : starts_mask & (~0ull << (lane + 1));
     const int run_end = later ? (int)__builtin_ctzll(later) : 64;
-    bool same[6], need[6];   // need[k]: some lane of the wave adds at distance 2^k (wave-uniform)
+    const unsigned long long upto = starts_mask & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int run_start = 63 - __builtin_clzll(upto);
+    // the run's part in this lane's 16-lane row: [seg_start, lane] summed by
+    // DPP row shifts of 1, 2, 4 and 8 lanes (mk[k] = 1 when lane - 2^k is in
+    // it); its last lane holds the part's sum and is its head
+    const int seg_start = max(run_start, lane & ~15);
+    R mk[4];
+    bool need[4];   // need[k]: some lane of the wave adds at distance 2^k (wave-uniform)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        same[k] = lane + (1 << k) < run_end;
-        need[k] = __ballot(same[k]) != 0ull;
+    for (int k = 0; k < 4; ++k) {
+        const bool same = lane - (1 << k) >= seg_start;
+        mk[k] = same ? R(1) : R(0);
+        need[k] = __ballot(same) != 0ull;
     }
-    const bool head = live && starts;
+    const bool head = live && (lane + 1 == run_end || (lane & 15) == 15);
     const R pvx = fixed ? R(0) : vx, pvy = fixed ? R(0) : vy, pvz = fixed ? R(0) : vz;
     const R mm = fixed ? R(0) : m;   // FIXED_BOUNDARY particles add no momentum
     // the run's base node in the tile: its 27 nodes are in the window
@@ -564,7 +585,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     static_assert(kRounds == 3, "round count");
     // particles whose run goes to the device grid (the host re-sorts when
     // they grow)
-    if (head && !lds_ok) atomicAdd(sa.islands + 2, (unsigned long long)(run_end - lane));
+    if (head && !lds_ok) atomicAdd(sa.islands + 2, (unsigned long long)(lane - seg_start + 1));
     R *const wp = s_wave + (lbz * kWY + lby) * kWX + lbx;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -591,14 +612,21 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                     ch[5] += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
                     ch[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
                 }
+                if (need[0]) {
 #pragma unroll
-                for (int k = 0; k < 6; ++k) {
-                    if (!need[k]) continue;
+                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<1>(ch[q]), mk[0], ch[q]);
+                }
+                if (need[1]) {
 #pragma unroll
-                    for (int q = 0; q < 7; ++q) {
-                        const R o = __shfl_down(ch[q], 1 << k, 64);
-                        if (same[k]) ch[q] += o;
-                    }
+                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<2>(ch[q]), mk[1], ch[q]);
+                }
+                if (need[2]) {
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<4>(ch[q]), mk[2], ch[q]);
+                }
+                if (need[3]) {
+#pragma unroll
+                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<8>(ch[q]), mk[3], ch[q]);
                 }
                 const int64_t n = head ? box_index(g, gx, gy, gz) : -1;
                 if (lds_ok && n >= 0) {
