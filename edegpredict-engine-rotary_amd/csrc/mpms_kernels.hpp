@@ -64,7 +64,7 @@ struct StepArgs {
 // kernel before the step's kernels: the kernels that read them can then be
 // replayed from a captured graph while dt, the tool position and the chip
 // step change.
-static __global__ void k_step_args(StepArgs *__restrict__ d, StepArgs v) {
+[[maybe_unused]] static __global__ void k_step_args(StepArgs *__restrict__ d, StepArgs v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *d = v;
 }
 
@@ -703,7 +703,7 @@ __device__ __forceinline__ void gather_windows(const Grid<R> &g, const R *__rest
 // tile t (keys of INACTIVE particles, 0xFFFFFFFF, sort after every tile);
 // thread i in [0, n] writes the starts of the tiles after its predecessor's
 // up to its own, so every t in [0, ntiles] is written once.
-static __global__ void __launch_bounds__(kBlock) k_tile_starts(const uint32_t *__restrict__ keys, int n, int ntiles,
+[[maybe_unused]] static __global__ void __launch_bounds__(kBlock) k_tile_starts(const uint32_t *__restrict__ keys, int n, int ntiles,
                                                         int32_t *__restrict__ tile_start) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i > n) return;
@@ -1017,7 +1017,7 @@ template <typename T> __global__ void __launch_bounds__(kBlock) k_gather(const T
     if (i < n) dst[i] = src[perm[i]];
 }
 
-static __global__ void __launch_bounds__(kBlock) k_zero_words(uint32_t *p, int64_t n) {
+[[maybe_unused]] static __global__ void __launch_bounds__(kBlock) k_zero_words(uint32_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = 0u;
 }
 
