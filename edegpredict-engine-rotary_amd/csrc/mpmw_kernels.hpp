@@ -480,36 +480,45 @@ __global__ void __launch_bounds__(kBlock) k_grid_update_batch(const GridWork<R> 
     grid_update_block(W.g, W.G, W.tp, W.M, W.tool, W.dt, W.direct_T, W.force_part, W.fwaves, b - W.block0);
 }
 
-// The fixed-order sum of an item's contact-force partials and, when
-// `counters` is set, the reset of its P2G-pass flags (counters[1] and [2],
-// read by the grid update, written by the next particle pass).
+// The contact-force partials of an item are summed in chunks of kSumChunk,
+// one block per chunk, each in a fixed order; the chunk sums are added, in
+// order, only when the force is read (k_finalize), which keeps the step free
+// of a dependent second pass.  Chunk 0's block also resets the item's
+// P2G-pass flags when `counters` is set (counters[1] and [2], read by the
+// grid update, written by the next particle pass).
+constexpr int kSumChunk = 16 * kBlock;
+
 struct SumWork {
     const double *part;
-    int32_t nparts, pad_;
+    int32_t nparts, chunks;    // partials; ceil(nparts / kSumChunk)
+    int32_t block0, pad_;      // first sum block of the item
     const int32_t *modes;
-    double *out;
+    double *chunk;             // chunks chunk sums
     int32_t *counters;
 };
 
-__device__ __forceinline__ void sum_block(const SumWork &W) {
-    finalize_block(W.part, W.nparts, 1, W.modes, W.out);
-    if (W.counters && threadIdx.x == 0) { W.counters[1] = 0; W.counters[2] = 0; }
+__device__ __forceinline__ void sum_block(const SumWork &W, int k) {
+    const int first = k * kSumChunk;
+    finalize_block(W.part + first, min(kSumChunk, W.nparts - first), 1, W.modes, W.chunk + k);
+    if (k == 0 && W.counters && threadIdx.x == 0) { W.counters[1] = 0; W.counters[2] = 0; }
 }
 
-// The pass after the grid update, one launch: blocks [0, dblocks) diffuse
-// the members' grids (items with diffusion), blocks dblocks + r sum member
-// r's contact force and reset its flags.
+// The pass after the grid update, one launch: blocks [0, sblocks) sum the
+// members' contact-force chunks, blocks sblocks + d diffuse the members'
+// grids (items with diffusion).
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_post_batch(const GridWork<R> *__restrict__ items, int nitems,
-                                                       const SumWork *__restrict__ sums, int dblocks, int order) {
+                                                       const SumWork *__restrict__ sums, int sblocks, int order) {
     const int b = launch_block(order);
-    if (b >= dblocks) {
-        sum_block(sums[b - dblocks]);
+    if (b < sblocks) {
+        const int r = find_item(sums, nitems, b);
+        sum_block(sums[r], b - sums[r].block0);
         return;
     }
-    const int w = find_item_d(items, nitems, b);
+    const int d = b - sblocks;
+    const int w = find_item_d(items, nitems, d);
     const GridWork<R> &W = items[w];
-    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, b - W.dblock0);
+    if (W.diffuse) diffuse_block(W.g, W.G, W.alpha, W.dt, d - W.dblock0);
 }
 
 // ------------------------------------------------------------ small ops ---

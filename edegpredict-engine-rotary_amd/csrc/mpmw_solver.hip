@@ -146,8 +146,9 @@ struct mpmw_solver {
     // reductions
     double *part = nullptr;        // k_tile / k_reduce partials
     int64_t part_cap = 0;
-    double *fpart = nullptr;       // contact-force partials
+    double *fpart = nullptr;       // contact-force partials, then the sums of their chunks
     int64_t fpart_cap = 0;
+    int32_t cf_first = 0, cf_chunks = 0;   // chunk sums of the last step not yet added (fpart + cf_first)
     double *d_stats = nullptr;     // kRed finalized statistics
     double *d_cf = nullptr;        // contact force of the last step
     int32_t *d_modes = nullptr;    // finalize modes for kRed columns, then one "sum"
@@ -422,6 +423,18 @@ template <typename T> int grow(mpmw_solver *s, T *&p, int64_t &cap, int64_t need
     return MPMW_OK;
 }
 
+// Adds the chunk sums of the last step's contact-force partials, in order,
+// into d_cf: before the force is read, or before a larger grid replaces the
+// buffer that holds them.
+static int add_contact_chunks(mpmw_solver *s) {
+    if (s->cf_chunks == 0) return MPMW_OK;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, (const double *)(s->fpart + s->cf_first),
+                       s->cf_chunks, 1, (const int32_t *)(s->d_modes + kRed), s->d_cf);
+    HIP_TRY(hipGetLastError());
+    s->cf_chunks = 0;
+    return MPMW_OK;
+}
+
 // ---------------------------------------------------------------- views ---
 
 template <typename R> Particles<R> pview(mpmw_solver *s) {
@@ -691,8 +704,10 @@ int rebuild_grid_from_stats(mpmw_solver *s) {
         s->snap_valid = false;
     }
     s->nn = nn;
-    int rc = grow(s, s->fpart, s->fpart_cap, (nn + kWave - 1) / kWave);   // one contact-force partial per wave
-    if (rc) return rc;
+    // one contact-force partial per wave, then the sums of their chunks
+    const int64_t fwaves = (nn + kWave - 1) / kWave, need = fwaves + (fwaves + kSumChunk - 1) / kSumChunk;
+    int rc = need > s->fpart_cap ? add_contact_chunks(s) : MPMW_OK;
+    if (rc || (rc = grow(s, s->fpart, s->fpart_cap, need))) return rc;
     s->p2g_ready = false;
     return MPMW_OK;
 }
@@ -1924,8 +1939,11 @@ int mpmw_get_metrics(mpmw_solver *s, mpmw_metrics *m) {
     CHECK_S(s);
     if (!m) return fail(MPMW_E_INVAL, "null");
     if (s->metrics_pending || s->cf_pending) {
-        if (s->cf_pending)
+        if (s->cf_pending) {
+            int rc = add_contact_chunks(s);
+            if (rc) return rc;
             HIP_TRY(hipMemcpyAsync(s->h_cf, s->d_cf, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        }
         HIP_TRY(hipStreamSynchronize(s->stream));
         if (s->metrics_pending) absorb_stats(s);
         if (s->cf_pending) s->contact_force = *s->h_cf;
@@ -2684,7 +2702,7 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
     std::vector<GridWork<R>> gw(nr);
     std::vector<SumWork> sw(nr);
     std::vector<TileWork<R>> tw[kTileVariants];
-    int gblocks = 0, dblocks = 0, tblocks[kTileVariants] = {};
+    int gblocks = 0, dblocks = 0, sblocks = 0, tblocks[kTileVariants] = {};
     for (int r = 0; r < nr; ++r) {
         mpmw_solver *s = members[run[r]];
         const StepPlan &sp = plans[run[r]];
@@ -2710,8 +2728,14 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
         dblocks += g.dblocks;
         sw[r].part = s->fpart;
         sw[r].nparts = g.fwaves;
+        sw[r].chunks = (g.fwaves + kSumChunk - 1) / kSumChunk;
+        sw[r].block0 = sblocks;
+        sw[r].pad_ = 0;
+        sblocks += sw[r].chunks;
         sw[r].modes = s->d_modes + kRed;
-        sw[r].out = s->d_cf;
+        sw[r].chunk = s->fpart + g.fwaves;
+        s->cf_first = g.fwaves;
+        s->cf_chunks = sw[r].chunks;
         sw[r].counters = sp.next_rebuild ? nullptr : s->sort.oow;
         const int v = tile_variant(s, sp);
         TileWork<R> t = tile_work<R>(s, dt, sp.zcut, sp.fl);
@@ -2752,8 +2776,8 @@ template <typename R> int step_many(mpmw_solver *const *members, size_t nm, cons
             {
                 // diffusion, contact-force sums and flag resets in one launch
                 Timed t(tm, MPMW_K_DIFFUSE);
-                hipLaunchKernelGGL(k_post_batch<R>, dim3(dblocks + nr), dim3(kBlock), 0, stream, d_gw, nr,
-                                   (const SumWork *)(d + off_sw), dblocks, cfg.post_order);
+                hipLaunchKernelGGL(k_post_batch<R>, dim3(sblocks + dblocks), dim3(kBlock), 0, stream, d_gw, nr,
+                                   (const SumWork *)(d + off_sw), sblocks, cfg.post_order);
             }
             if (ev) (void)hipEventRecord(ev[2], stream);
             {
