@@ -421,6 +421,37 @@ template <int D> __device__ __forceinline__ double row_shr(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
 }
 
+// One step of the segmented row sum over a particle's 7 channel values:
+// ch[q] += mk * (ch[q] of lane l - D in the same 16-lane row), zero from
+// before the row's start.  FP32: one v_fmac_f32 per channel reading its
+// first operand through DPP row_shr:D (the compiler's form is a DPP move,
+// the multiply-add and a wait state: three issue slots); the s_nop ahead of
+// the seven covers the wait states a DPP read needs after a VALU write of
+// its source and after an EXEC write, whatever the compiler placed before.
+// The same fused multiply-adds as the compiler's form, so the same sums.
+template <int D, typename R> __device__ __forceinline__ void seg_step(R (&ch)[7], R mk) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<D>(ch[q]), mk, ch[q]);
+}
+#define MPMS_SEG_STEP_F32(D)                                                                                   \
+    template <> __device__ __forceinline__ void seg_step<D, float>(float (&ch)[7], float mk) {                  \
+        asm("s_nop 4\n\t"                                                                                       \
+            "v_fmac_f32_dpp %0, %0, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %1, %1, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %2, %2, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %3, %3, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %4, %4, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %5, %5, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"               \
+            "v_fmac_f32_dpp %6, %6, %7 row_shr:" #D " row_mask:0xf bank_mask:0xf bound_ctrl:1"                     \
+            : "+v"(ch[0]), "+v"(ch[1]), "+v"(ch[2]), "+v"(ch[3]), "+v"(ch[4]), "+v"(ch[5]), "+v"(ch[6])         \
+            : "v"(mk));                                                                                         \
+    }
+MPMS_SEG_STEP_F32(1)
+MPMS_SEG_STEP_F32(2)
+MPMS_SEG_STEP_F32(4)
+MPMS_SEG_STEP_F32(8)
+#undef MPMS_SEG_STEP_F32
+
 // Eight channel values as one record (two 16-byte or 32-byte accesses).
 __device__ __forceinline__ void store_record(float *p, const float v[8]) {
     reinterpret_cast<float4 *>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
@@ -612,22 +643,10 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                     ch[5] += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
                     ch[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
                 }
-                if (need[0]) {
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<1>(ch[q]), mk[0], ch[q]);
-                }
-                if (need[1]) {
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<2>(ch[q]), mk[1], ch[q]);
-                }
-                if (need[2]) {
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<4>(ch[q]), mk[2], ch[q]);
-                }
-                if (need[3]) {
-#pragma unroll
-                    for (int q = 0; q < 7; ++q) ch[q] = fma(row_shr<8>(ch[q]), mk[3], ch[q]);
-                }
+                if (need[0]) seg_step<1, R>(ch, mk[0]);
+                if (need[1]) seg_step<2, R>(ch, mk[1]);
+                if (need[2]) seg_step<4, R>(ch, mk[2]);
+                if (need[3]) seg_step<8, R>(ch, mk[3]);
                 const int64_t n = head ? box_index(g, gx, gy, gz) : -1;
                 if (lds_ok && n >= 0) {
                     R *const q = wp + (c * kWY + b) * kWX + a;
