@@ -479,7 +479,7 @@ __device__ __forceinline__ void load_record(const double *p, double v[8]) {
 // (TX + 2) x (TY + 2) x (TZ + 2) nodes.
 constexpr int kTX = 8, kTY = 8, kTZ = 4;
 constexpr int kWX = kTX + 2, kWY = kTY + 2, kWZ = kTZ + 2, kWN = kWX * kWY * kWZ;
-constexpr int kRounds = 3;   // window-add rounds of a wave's heads sharing a base node
+constexpr int kRounds = 3;   // window-add rounds of a wave's heads sharing a base-node column
 
 // p2gKernel (MPMSolver.cu:124-412) with the LOD classification of the step
 // (lodClassifyKernel :655-676) folded in: one workgroup per base-node tile,
@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                                                 const int32_t *__restrict__ tile_start, R *__restrict__ tpart) {
     constexpr int kWaves = kBlock / 64;
     __shared__ R s_w[kWaves * 7 * kWN];
-    __shared__ int32_t s_own[kWaves * kTX * kTY * kTZ];   // per wave: the head owning a base node this round
+    __shared__ int32_t s_own[kWaves * kTX * kTY];   // per wave: the head owning a base-node column this round
     const StepArgs sa = *sap;
     const int t = blockIdx.x;
     const int ts = tile_start[t], te = tile_start[t + 1];
@@ -594,17 +594,20 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     const R mm = fixed ? R(0) : m;   // FIXED_BOUNDARY particles add no momentum
     // the run's base node in the tile: its 27 nodes are in the window
     const int lbx = ax.b - tx0, lby = ay.b - ty0, lbz = az.b - tz0;
-    // Heads of runs with the same base node (a particle that crossed a
-    // base-node plane since the sort starts a run of its own) must not add
-    // to one window node in one instruction: each head claims its base node
-    // in the wave's owner table, one winner per node and round, and the
+    // A head adds its 27 nodes to the window a column (x, y offsets a, b) at
+    // a time: the three nodes' reads, then their writes.  Heads whose base
+    // nodes share the column (x, y) (runs split by a row boundary, a particle
+    // that crossed a base-node plane since the sort, runs of two z layers)
+    // would then race on a node, so each head claims its base node's column
+    // in the wave's owner table, one winner per column and round, and the
     // rounds add one after another (a wave's LDS operations complete in
-    // order).  A head with no round left, or whose base node has left the
-    // tile, adds to the device grid.
+    // order); heads of one round target distinct nodes at every offset.  A
+    // head with no round left, or whose base node has left the tile, adds
+    // to the device grid.
     const bool inwin = (unsigned)lbx < (unsigned)kTX && (unsigned)lby < (unsigned)kTY && (unsigned)lbz < (unsigned)kTZ;
     int myround = head && inwin ? -1 : kRounds;
     {
-        volatile int32_t *own = s_own + (threadIdx.x / 64) * (kTX * kTY * kTZ) + ((lbz * kTY + lby) * kTX + lbx);
+        volatile int32_t *own = s_own + (threadIdx.x / 64) * (kTX * kTY) + (lby * kTX + lbx);
 #pragma unroll
         for (int r = 0; r < kRounds; ++r) {
             if (myround == -1) *own = lane;
@@ -621,7 +624,8 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
+        for (int b = 0; b < 3; ++b) {
+            R ch[3][7];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int gx = ax.b + a, gy = ay.b + b, gz = az.b + c;
@@ -632,52 +636,67 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                 const R dpx = (R)((gx * g.dx + g.ox) - x), dpy = (R)((gy * g.dx + g.oy) - y),
                         dpz = (R)((gz * g.dx + g.oz) - z);
                 const R wm = w * mm;
-                R ch[7];
-                ch[0] = w * m;
-                ch[1] = wm * (pvx + C[0] * dpx + C[1] * dpy + C[2] * dpz);
-                ch[2] = wm * (pvy + C[3] * dpx + C[4] * dpy + C[5] * dpz);
-                ch[3] = wm * (pvz + C[6] * dpx + C[7] * dpy + C[8] * dpz);
-                ch[4] = w * efx; ch[5] = w * efy; ch[6] = w * efz;
+                R *const h = ch[c];
+                h[0] = w * m;
+                h[1] = wm * (pvx + C[0] * dpx + C[1] * dpy + C[2] * dpz);
+                h[2] = wm * (pvy + C[3] * dpx + C[4] * dpy + C[5] * dpz);
+                h[3] = wm * (pvz + C[6] * dpx + C[7] * dpy + C[8] * dpz);
+                h[4] = w * efx; h[5] = w * efy; h[6] = w * efz;
                 if (!skip) {
-                    ch[4] += -VJ * (sig[0] * wgx + sig[1] * wgy + sig[2] * wgz);
-                    ch[5] += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
-                    ch[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
+                    h[4] += -VJ * (sig[0] * wgx + sig[1] * wgy + sig[2] * wgz);
+                    h[5] += -VJ * (sig[3] * wgx + sig[4] * wgy + sig[5] * wgz);
+                    h[6] += -VJ * (sig[6] * wgx + sig[7] * wgy + sig[8] * wgz);
                 }
-                if (need[0]) seg_step<1, R>(ch, mk[0]);
-                if (need[1]) seg_step<2, R>(ch, mk[1]);
-                if (need[2]) seg_step<4, R>(ch, mk[2]);
-                if (need[3]) seg_step<8, R>(ch, mk[3]);
-                const int64_t n = head ? box_index(g, gx, gy, gz) : -1;
-                if (lds_ok && n >= 0) {
-                    R *const q = wp + (c * kWY + b) * kWX + a;
+                if (need[0]) seg_step<1, R>(ch[c], mk[0]);
+                if (need[1]) seg_step<2, R>(ch[c], mk[1]);
+                if (need[2]) seg_step<4, R>(ch[c], mk[2]);
+                if (need[3]) seg_step<8, R>(ch[c], mk[3]);
+            }
+            // the column's three nodes into the window (a node outside the
+            // sub-box too: k_grid reads no window entry outside it)
+            if (lds_ok) {
+                R *const q = wp + b * kWX + a;
 #pragma unroll
-                    for (int r = 0; r < kRounds; ++r) {
-                        if (r < nrounds && myround == r) {
+                for (int r = 0; r < kRounds; ++r) {
+                    if (r < nrounds && myround == r) {
+                        R old[3][7];
 #pragma unroll
-                            for (int h = 0; h < 7; ++h) q[h * kWN] += ch[h];
-                        }
-                    }
-                }
-                if (head) {
-                    if (n < 0) {
-                        ++missed;
-                    } else if (!lds_ok) {
-                        unsafeAtomicAdd(g.mass + n, ch[0]);
-                        if (ch[1] != R(0) || ch[2] != R(0) || ch[3] != R(0)) {
-                            unsafeAtomicAdd(g.p[0] + n, ch[1]);
-                            unsafeAtomicAdd(g.p[1] + n, ch[2]);
-                            unsafeAtomicAdd(g.p[2] + n, ch[3]);
-                        }
-                        unsafeAtomicAdd(g.f[0] + n, ch[4]);
-                        unsafeAtomicAdd(g.f[1] + n, ch[5]);
-                        unsafeAtomicAdd(g.f[2] + n, ch[6]);
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int k = 0; k < 7; ++k) old[c][k] = q[c * (kWX * kWY) + k * kWN];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int k = 0; k < 7; ++k) q[c * (kWX * kWY) + k * kWN] = old[c][k] + ch[c][k];
                     }
                 }
             }
+            if (head) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int64_t n = box_index(g, ax.b + a, ay.b + b, az.b + c);
+                    if (n < 0) {
+                        ++missed;
+                    } else if (!lds_ok) {
+                        const R *const h = ch[c];
+                        unsafeAtomicAdd(g.mass + n, h[0]);
+                        if (h[1] != R(0) || h[2] != R(0) || h[3] != R(0)) {
+                            unsafeAtomicAdd(g.p[0] + n, h[1]);
+                            unsafeAtomicAdd(g.p[1] + n, h[2]);
+                            unsafeAtomicAdd(g.p[2] + n, h[3]);
+                        }
+                        unsafeAtomicAdd(g.f[0] + n, h[4]);
+                        unsafeAtomicAdd(g.f[1] + n, h[5]);
+                        unsafeAtomicAdd(g.f[2] + n, h[6]);
+                    }
+                }
+            }
+        }
     }
     __syncthreads();
-    // the window, its waves' parts summed in wave order (a node outside the
-    // sub-box holds zeros: its visits were counted as missed)
+    // the window, its waves' parts summed in wave order (entries of nodes
+    // outside the sub-box are never read: their visits were counted as
+    // missed)
     R *const dst = tpart + (int64_t)t * (kWN * 8);
     for (int w = threadIdx.x; w < kWN; w += kBlock) {
         R v[8];
