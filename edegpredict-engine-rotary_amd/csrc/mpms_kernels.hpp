@@ -1050,9 +1050,28 @@ __global__ void __launch_bounds__(kBlock) k_sort_keys(Particles<R> P, Grid<R> g,
     idx[i] = i;
 }
 
-template <typename T> __global__ void __launch_bounds__(kBlock) k_gather(const T *src, T *dst, const int32_t *perm, int n) {
+// Every particle field permuted in one launch: field k of particle i takes
+// the value of particle perm[i]; fields [0, NW) are 8 bytes wide, [NW, NF)
+// 4 bytes.  A thread issues its particle's NF loads before its stores.
+template <int NF> struct FieldPtrs {
+    const void *src[NF];
+    void *dst[NF];
+};
+template <int NF, int NW>
+__global__ void __launch_bounds__(kBlock) k_gather_fields(FieldPtrs<NF> G, const int32_t *__restrict__ perm, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+    if (i >= n) return;
+    const int j = perm[i];
+    double w[NW];
+    int32_t v[NF - NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = static_cast<const double *>(G.src[k])[j];
+#pragma unroll
+    for (int k = NW; k < NF; ++k) v[k - NW] = static_cast<const int32_t *>(G.src[k])[j];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) static_cast<double *>(G.dst[k])[i] = w[k];
+#pragma unroll
+    for (int k = NW; k < NF; ++k) static_cast<int32_t *>(G.dst[k])[i] = v[k - NW];
 }
 
 [[maybe_unused]] static __global__ void __launch_bounds__(kBlock) k_zero_words(uint32_t *p, int64_t n) {

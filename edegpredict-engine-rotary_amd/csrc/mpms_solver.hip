@@ -427,14 +427,20 @@ template <typename R> int fit(mpms_solver *s) {
         hipLaunchKernelGGL(k_tile_starts, dim3(blocks(n + 1)), dim3(kBlock), 0, s->stream, (const uint32_t *)s->keys2,
                            n, s->ntiles, s->tile_start);
     }
-    for (int k = 0; k < NFIELD; ++k) {
-        if (k <= Z || (s->fp64 && k < kFirstInt))
-            hipLaunchKernelGGL(k_gather<double>, dim3(blocks(n)), dim3(kBlock), 0, s->stream, (const double *)s->f[k],
-                               (double *)s->alt[k], s->idx2, n);
+    {
+        // positions (and in FP64 every real field) 8 bytes wide, the rest 4
+        FieldPtrs<NFIELD> G;
+        for (int k = 0; k < NFIELD; ++k) {
+            G.src[k] = s->f[k];
+            G.dst[k] = s->alt[k];
+        }
+        if (s->fp64)
+            hipLaunchKernelGGL((k_gather_fields<NFIELD, kFirstInt>), dim3(blocks(n)), dim3(kBlock), 0, s->stream, G,
+                               (const int32_t *)s->idx2, n);
         else
-            hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks(n)), dim3(kBlock), 0, s->stream,
-                               (const int32_t *)s->f[k], (int32_t *)s->alt[k], s->idx2, n);
-        std::swap(s->f[k], s->alt[k]);
+            hipLaunchKernelGGL((k_gather_fields<NFIELD, Z + 1>), dim3(blocks(n)), dim3(kBlock), 0, s->stream, G,
+                               (const int32_t *)s->idx2, n);
+        for (int k = 0; k < NFIELD; ++k) std::swap(s->f[k], s->alt[k]);
     }
     MS_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(kBlock), 0, s->stream, (uint32_t *)s->d_counts, (int64_t)6);
