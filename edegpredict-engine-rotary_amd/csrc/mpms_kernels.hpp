@@ -620,6 +620,17 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
     // particles whose run goes to the device grid (the host re-sorts when
     // they grow)
     if (head && !lds_ok) atomicAdd(sa.islands + 2, (unsigned long long)(lane - seg_start + 1));
+    if (head) {
+        // the run's stencil visits outside the sub-box: 27 less the product
+        // of the stencil's in-box node counts per axis
+        auto inbox = [&](int b0, int ax_) {
+            int k = 0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) k += (unsigned)(b0 + d - g.lo[ax_]) < (unsigned)g.nb[ax_] ? 1 : 0;
+            return k;
+        };
+        missed += 27 - inbox(ax.b, 0) * inbox(ay.b, 1) * inbox(az.b, 2);
+    }
     R *const wp = s_wave + (lbz * kWY + lby) * kWX + lbx;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -671,13 +682,11 @@ __global__ void __launch_bounds__(kBlock) k_p2g(Particles<R> P, Grid<R> g, Mat<R
                     }
                 }
             }
-            if (head) {
+            if (head && !lds_ok) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int64_t n = box_index(g, ax.b + a, ay.b + b, az.b + c);
-                    if (n < 0) {
-                        ++missed;
-                    } else if (!lds_ok) {
+                    if (n >= 0) {
                         const R *const h = ch[c];
                         unsafeAtomicAdd(g.mass + n, h[0]);
                         if (h[1] != R(0) || h[2] != R(0) || h[3] != R(0)) {
