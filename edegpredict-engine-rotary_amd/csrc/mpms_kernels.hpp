@@ -486,11 +486,12 @@ constexpr int kRounds = 3;   // window-add rounds of a wave's heads sharing a ba
 // its particles [tile_start[t], tile_start[t + 1]) one per thread in chunks
 // of the workgroup.  Consecutive lanes mostly share a base node and hence
 // all 27 target nodes: each of a node's 7 channel values is summed over
-// each run of equal base nodes in the wave (a segmented suffix sum by
-// shuffles) and the run's first lane adds the sums to its wave's LDS window
-// of the tile with plain read-add-writes, heads sharing a base node in
-// successive rounds (a particle whose base node has left the tile since the
-// sort adds to the device grid directly).  The four windows'
+// each run of equal base nodes inside a 16-lane row (a segmented prefix sum
+// by DPP row shifts) and the run's last lane in the row adds the sums to its
+// wave's LDS window of the tile with plain read-add-writes, a column of
+// three nodes at a time, heads sharing a base-node column in successive
+// rounds (a particle whose base node has left the tile since the sort adds
+// to the device grid directly).  The four windows'
 // sum is stored whole to tpart[t] (kWN records of 8 channels, the last
 // unused), which k_grid adds up: no device atomics for the window, whose
 // nodes are mostly shared with neighbouring tiles (the device atomics were
