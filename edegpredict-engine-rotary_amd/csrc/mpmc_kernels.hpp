@@ -551,151 +551,198 @@ struct ContactArgs {
     double *totals;               // [0] heat, [1] force
 };
 
-// xpbdContactSpatialHashKernel (ContactSolver.cu:159-408), one thread per
-// particle.  A particle outside the tool's box grown by the contact radius
-// can touch no node and skips the 27-cell probe (same results).  Node
-// accumulators go to the original node index (the reference adds into its
-// sorted copy, then scatters with atomics: the same terms).
+// The particles that can touch a tool node: not INACTIVE or
+// FIXED_BOUNDARY, and inside the tool's box grown by the contact radius
+// (a particle outside it can touch no node).  Their indices are appended to
+// cand (*ncand of them, one atomic per wave that has any, in no fixed
+// order); each thread tests kCullPer particles, their loads issued first,
+// so the pass over the whole workpiece streams its 28 bytes per particle.
+constexpr int kCullPer = 4;
 template <typename R>
-__global__ void __launch_bounds__(kBlock) k_contact(mpms::Particles<R> P, Nodes S, ContactParams C, ContactArgs A) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    bool counted = false, probed = false;
+__global__ void __launch_bounds__(kBlock) k_contact_cull(mpms::Particles<R> P, double r,
+                                                         const unsigned long long *__restrict__ box,
+                                                         int32_t *__restrict__ cand, int32_t *__restrict__ ncand) {
+    const int lane = threadIdx.x & 63;
+    const int base = blockIdx.x * (kBlock * kCullPer) + threadIdx.x;
+    int32_t st[kCullPer];
+    double x[kCullPer], y[kCullPer], z[kCullPer];
+#pragma unroll
+    for (int u = 0; u < kCullPer; ++u) {
+        const int i = base + u * kBlock;
+        st[u] = mpms::kInactive;
+        x[u] = y[u] = z[u] = 0.0;
+        if (i < P.n) { st[u] = P.status[i]; x[u] = P.x[i]; y[u] = P.y[i]; z[u] = P.z[i]; }
+    }
+    const double g = r * (1.0 + 1e-9) + 1e-12;
+    const double bx0 = from_ordered_bits(box[0]), by0 = from_ordered_bits(box[1]), bz0 = from_ordered_bits(box[2]),
+                 bx1 = from_ordered_bits(box[3]), by1 = from_ordered_bits(box[4]), bz1 = from_ordered_bits(box[5]);
+#pragma unroll
+    for (int u = 0; u < kCullPer; ++u) {
+        const bool c = st[u] != mpms::kInactive && st[u] != mpms::kFixed &&
+                       !(x[u] < bx0 - g || x[u] > bx1 + g || y[u] < by0 - g || y[u] > by1 + g || z[u] < bz0 - g ||
+                         z[u] > bz1 + g);
+        const unsigned long long m = __ballot(c);
+        if (m) {   // uniform over the wave
+            const int leader = __ffsll((long long)m) - 1;
+            int b = 0;
+            if (lane == leader) b = atomicAdd(ncand, __popcll(m));
+            b = __shfl(b, leader, 64);
+            if (c) cand[b + __popcll(m & ((1ull << lane) - 1ull))] = base + u * kBlock;
+        }
+    }
+}
+
+// xpbdContactSpatialHashKernel (ContactSolver.cu:159-408) on the particles
+// k_contact_cull kept (cand[0, *ncand)); the others can touch no node and
+// skip the 27-cell probe (same results).  One candidate per 32 lanes: lane
+// c < 27 probes cell c of the particle's 3 x 3 x 3 neighbourhood (c = 9 (dz
+// + 1) + 3 (dy + 1) + dx + 1), so the cells' dependent loads (bucket, its
+// range, its records) run side by side instead of one after another; the
+// lanes' position corrections, contact counts and heat for the particle are
+// summed over the 32 lanes and the group's first lane updates the
+// particle.  Node accumulators go to the original node index (the
+// reference adds into its sorted copy, then scatters with atomics: the same
+// terms).
+template <typename R>
+__global__ void __launch_bounds__(kBlock) k_contact(mpms::Particles<R> P, Nodes S, ContactParams C, ContactArgs A,
+                                                    const int32_t *__restrict__ cand,
+                                                    const int32_t *__restrict__ ncand) {
+    const int nc = *ncand;
+    const int sub = threadIdx.x & 31;
+    const int groups = gridDim.x * (kBlock / 32);
+    int counted = 0, probed = 0;
     double heat_gen = 0.0, force_sum = 0.0;
-    if (i < P.n) {
-        const int32_t st = P.status[i];
-        if (st != mpms::kInactive && st != mpms::kFixed) {
-            const double px = P.x[i], py = P.y[i], pz = P.z[i];
-            const double g = C.r * (1.0 + 1e-9) + 1e-12;
-            const double bx0 = from_ordered_bits(A.box[0]), by0 = from_ordered_bits(A.box[1]),
-                         bz0 = from_ordered_bits(A.box[2]), bx1 = from_ordered_bits(A.box[3]),
-                         by1 = from_ordered_bits(A.box[4]), bz1 = from_ordered_bits(A.box[5]);
-            if (!(px < bx0 - g || px > bx1 + g || py < by0 - g || py > by1 + g || pz < bz0 - g || pz > bz1 + g)) {
-                probed = true;
-                const double cell = C.r, r = C.r;
-                const int pcx = (int)floor((px - A.ox) / cell), pcy = (int)floor((py - A.oy) / cell),
-                          pcz = (int)floor((pz - A.oz) / cell);
-                const double vx = (double)P.v[0][i], vy = (double)P.v[1][i], vz = (double)P.v[2][i];
-                const double mass = (double)P.m[i], vol = (double)P.V[i], Tp = (double)P.T[i];
-                double cx = 0, cy = 0, cz = 0, heat_to_p = 0;
-                int count = 0;
-                bool has = false;
-                const double kN = fmax(C.k, 0.0);
-                const double cN = 2.0 * fmax(C.zeta, 0.0) * sqrt(kN * fmax(mass, 1e-15));
-                // the particle volume (its density is not held: the reference's
-                // fallback mass / max(density, 1) with density 0)
-                const double pv = vol > 1e-24 ? vol : mass;
-                const double prad = pow(3.0 * pv / (4.0 * kPi), 1.0 / 3.0);
-                const double flow =
-                    C.jc_enabled ? jc_flow(C, (double)P.eps[i], (double)P.rate[i], Tp) : C.shear_yield;
-                const double shear_yield = flow / 1.7320508;
-                const double mina = 0.01 * pow(fmax(pv, 1e-24), 2.0 / 3.0);
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int qx = pcx + dx, qy = pcy + dy, qz = pcz + dz;
-                            const uint32_t h = (uint32_t)(cell_mix(qx > 0 ? qx : 0, qy > 0 ? qy : 0, qz > 0 ? qz : 0) %
-                                                          (uint64_t)A.table);
-                            const int s0 = A.start[h];
-                            if (s0 < 0) continue;
-                            const int s1 = A.end[h];
-                            for (int j = s0; j < s1; ++j) {
-                                const NodeRec rec = A.rec[j];
-                                const long long tag = __double_as_longlong(rec.b.w);
-                                if ((int32_t)(tag >> 32) == kNodeFailed) continue;
-                                const int node = (int)(uint32_t)(tag & 0xffffffff);
-                                const double ndx = px - rec.a.x, ndy = py - rec.a.y, ndz = pz - rec.a.z;
-                                const double dist = sqrt(ndx * ndx + ndy * ndy + ndz * ndz);
-                                if (dist >= r || dist < 1e-12 || isnan(dist)) continue;
-                                has = true;
-                                double pen = r - dist;
-                                if (pen > A.max_pen) pen = A.max_pen;
-                                const double id = 1.0 / dist;
-                                const double nx = ndx * id, ny = ndy * id, nz = ndz * id;
-                                cx += pen * nx; cy += pen * ny; cz += pen * nz;
-                                count++;
-                                const double rvx = vx - rec.b.x, rvy = vy - rec.b.y, rvz = vz - rec.b.z;
-                                const double vn = rvx * nx + rvy * ny + rvz * nz;
-                                const double fn = (kN * pen + cN * fmax(0.0, -vn)) * A.scale;
-                                unsafeAtomicAdd(S.fx + node, -fn * nx);
-                                unsafeAtomicAdd(S.fy + node, -fn * ny);
-                                unsafeAtomicAdd(S.fz + node, -fn * nz);
-                                unsafeAtomicAdd(S.fn + node, fn);
-                                // penetrations are positive: their bits order like them
-                                atomicMax((unsigned long long *)(S.pen + node),
-                                          (unsigned long long)__double_as_longlong(pen));
-                                const double vtx = rvx - vn * nx, vty = rvy - vn * ny, vtz = rvz - vn * nz;
-                                const double vt = sqrt(vtx * vtx + vty * vty + vtz * vtz);
-                                const double Tavg = 0.5 * (Tp + rec.a.w);
-                                const double Tr = fmin(1.0, fmax(0.0, (Tavg - 200.0) / 600.0));
-                                const double mu_eff = C.friction * (1.0 - 0.4 * Tr);
-                                const double cpp = pen > 0.0 ? pen : 0.0;
-                                const double ea = kPi * (2.0 * prad * cpp - cpp * cpp);
-                                const double pa = flow > 1.0 ? fn / flow : 0.0;
-                                double area = ea + pa;
-                                if (area < mina) area = mina;
-                                const double sn = fn / fmax(area, 1e-24);
-                                const double scrit = shear_yield / fmax(mu_eff, 1e-6);
-                                const double stick = 0.5 * (1.0 + tanh(10.0 * (sn / fmax(scrit, 1e-12) - 1.0)));
-                                const double tau = (1.0 - stick) * (mu_eff * sn) + stick * shear_yield;
-                                double ft = tau * area;
-                                const double fst = vt > 1e-10 ? mass * vt / A.dt : ft;
-                                ft = fmin(ft, fst);
-                                const double htc = C.htc * (1.0 + 5.0 * (1.0 - exp(-sn / flow)));
-                                const double hs = ft * vt * A.dt;
-                                const double qc = htc * area * (Tp - rec.a.w) * A.dt;
-                                heat_to_p += (hs * (1.0 - C.hp) - qc) * A.scale;
-                                unsafeAtomicAdd(S.heat + node, (hs * C.hp + qc) * A.scale);
-                                if (vt > 1e-15) {
-                                    unsafeAtomicAdd(S.fx + node, ft * (vtx / vt));
-                                    unsafeAtomicAdd(S.fy + node, ft * (vty / vt));
-                                    unsafeAtomicAdd(S.fz + node, ft * (vtz / vt));
-                                }
-                                S.in_contact[node] = 1;
-                                heat_gen += hs * A.scale;
-                                force_sum += fn;
-                            }
-                        }
-                if (has) {
-                    const double w = 1.0 / count;
-                    cx *= w; cy *= w; cz *= w;
-                    P.x[i] = px + cx; P.y[i] = py + cy; P.z[i] = pz + cz;
-                    const double cm = sqrt(cx * cx + cy * cy + cz * cz);
-                    if (cm > 1e-15) {
-                        const double ic = 1.0 / cm;
-                        const double nx = cx * ic, ny = cy * ic, nz = cz * ic;
-                        const double vd = vx * nx + vy * ny + vz * nz;
-                        if (vd < 0.0) {
-                            P.v[0][i] = (R)(vx - vd * nx);
-                            P.v[1][i] = (R)(vy - vd * ny);
-                            P.v[2][i] = (R)(vz - vd * nz);
-                        }
-                    }
-                    if (heat_to_p > 0.0) {
-                        const double pm = fmax(C.work_rho, 1.0) * fmax(pv, 1e-24);
-                        const double dT = heat_to_p / (fmax(pm, 1e-15) * C.work_cp);
-                        P.T[i] = (R)fmin(Tp + dT, C.work_melt);
-                    }
-                    counted = true;
+    for (int k = (blockIdx.x * kBlock + (int)threadIdx.x) >> 5; k < nc; k += groups) {   // uniform per 32 lanes
+        const int i = cand[k];
+        const double px = P.x[i], py = P.y[i], pz = P.z[i];
+        const double vx = (double)P.v[0][i], vy = (double)P.v[1][i], vz = (double)P.v[2][i];
+        const double mass = (double)P.m[i], vol = (double)P.V[i], Tp = (double)P.T[i];
+        // the particle volume (its density is not held: the reference's
+        // fallback mass / max(density, 1) with density 0)
+        const double pv = vol > 1e-24 ? vol : mass;
+        double cx = 0, cy = 0, cz = 0, heat_to_p = 0;
+        int count = 0;
+        if (sub < 27) {
+            const double cell = C.r, r = C.r;
+            const int pcx = (int)floor((px - A.ox) / cell), pcy = (int)floor((py - A.oy) / cell),
+                      pcz = (int)floor((pz - A.oz) / cell);
+            const double kN = fmax(C.k, 0.0);
+            const double cN = 2.0 * fmax(C.zeta, 0.0) * sqrt(kN * fmax(mass, 1e-15));
+            const double prad = pow(3.0 * pv / (4.0 * kPi), 1.0 / 3.0);
+            const double flow =
+                C.jc_enabled ? jc_flow(C, (double)P.eps[i], (double)P.rate[i], Tp) : C.shear_yield;
+            const double shear_yield = flow / 1.7320508;
+            const double mina = 0.01 * pow(fmax(pv, 1e-24), 2.0 / 3.0);
+            const int qx = pcx + sub % 3 - 1, qy = pcy + (sub / 3) % 3 - 1, qz = pcz + sub / 9 - 1;
+            const uint32_t h = (uint32_t)(cell_mix(qx > 0 ? qx : 0, qy > 0 ? qy : 0, qz > 0 ? qz : 0) %
+                                          (uint64_t)A.table);
+            const int s0 = A.start[h];
+            const int s1 = s0 < 0 ? s0 : A.end[h];
+            for (int j = s0; j < s1; ++j) {
+                const NodeRec rec = A.rec[j];
+                const long long tag = __double_as_longlong(rec.b.w);
+                if ((int32_t)(tag >> 32) == kNodeFailed) continue;
+                const int node = (int)(uint32_t)(tag & 0xffffffff);
+                const double ndx = px - rec.a.x, ndy = py - rec.a.y, ndz = pz - rec.a.z;
+                const double dist = sqrt(ndx * ndx + ndy * ndy + ndz * ndz);
+                if (dist >= r || dist < 1e-12 || isnan(dist)) continue;
+                double pen = r - dist;
+                if (pen > A.max_pen) pen = A.max_pen;
+                const double id = 1.0 / dist;
+                const double nx = ndx * id, ny = ndy * id, nz = ndz * id;
+                cx += pen * nx; cy += pen * ny; cz += pen * nz;
+                count++;
+                const double rvx = vx - rec.b.x, rvy = vy - rec.b.y, rvz = vz - rec.b.z;
+                const double vn = rvx * nx + rvy * ny + rvz * nz;
+                const double fn = (kN * pen + cN * fmax(0.0, -vn)) * A.scale;
+                unsafeAtomicAdd(S.fx + node, -fn * nx);
+                unsafeAtomicAdd(S.fy + node, -fn * ny);
+                unsafeAtomicAdd(S.fz + node, -fn * nz);
+                unsafeAtomicAdd(S.fn + node, fn);
+                // penetrations are positive: their bits order like them
+                atomicMax((unsigned long long *)(S.pen + node), (unsigned long long)__double_as_longlong(pen));
+                const double vtx = rvx - vn * nx, vty = rvy - vn * ny, vtz = rvz - vn * nz;
+                const double vt = sqrt(vtx * vtx + vty * vty + vtz * vtz);
+                const double Tavg = 0.5 * (Tp + rec.a.w);
+                const double Tr = fmin(1.0, fmax(0.0, (Tavg - 200.0) / 600.0));
+                const double mu_eff = C.friction * (1.0 - 0.4 * Tr);
+                const double cpp = pen > 0.0 ? pen : 0.0;
+                const double ea = kPi * (2.0 * prad * cpp - cpp * cpp);
+                const double pa = flow > 1.0 ? fn / flow : 0.0;
+                double area = ea + pa;
+                if (area < mina) area = mina;
+                const double sn = fn / fmax(area, 1e-24);
+                const double scrit = shear_yield / fmax(mu_eff, 1e-6);
+                const double stick = 0.5 * (1.0 + tanh(10.0 * (sn / fmax(scrit, 1e-12) - 1.0)));
+                const double tau = (1.0 - stick) * (mu_eff * sn) + stick * shear_yield;
+                double ft = tau * area;
+                const double fst = vt > 1e-10 ? mass * vt / A.dt : ft;
+                ft = fmin(ft, fst);
+                const double htc = C.htc * (1.0 + 5.0 * (1.0 - exp(-sn / flow)));
+                const double hs = ft * vt * A.dt;
+                const double qc = htc * area * (Tp - rec.a.w) * A.dt;
+                heat_to_p += (hs * (1.0 - C.hp) - qc) * A.scale;
+                unsafeAtomicAdd(S.heat + node, (hs * C.hp + qc) * A.scale);
+                if (vt > 1e-15) {
+                    unsafeAtomicAdd(S.fx + node, ft * (vtx / vt));
+                    unsafeAtomicAdd(S.fy + node, ft * (vty / vt));
+                    unsafeAtomicAdd(S.fz + node, ft * (vtz / vt));
                 }
+                S.in_contact[node] = 1;
+                heat_gen += hs * A.scale;
+                force_sum += fn;
+            }
+        }
+        // the particle's sums over its 32 lanes
+#pragma unroll
+        for (int d = 16; d >= 1; d >>= 1) {
+            cx += __shfl_xor(cx, d, 32);
+            cy += __shfl_xor(cy, d, 32);
+            cz += __shfl_xor(cz, d, 32);
+            heat_to_p += __shfl_xor(heat_to_p, d, 32);
+            count += __shfl_xor(count, d, 32);
+        }
+        if (sub == 0) {
+            ++probed;
+            if (count > 0) {
+                const double w = 1.0 / count;
+                cx *= w; cy *= w; cz *= w;
+                P.x[i] = px + cx; P.y[i] = py + cy; P.z[i] = pz + cz;
+                const double cm = sqrt(cx * cx + cy * cy + cz * cz);
+                if (cm > 1e-15) {
+                    const double ic = 1.0 / cm;
+                    const double nx = cx * ic, ny = cy * ic, nz = cz * ic;
+                    const double vd = vx * nx + vy * ny + vz * nz;
+                    if (vd < 0.0) {
+                        P.v[0][i] = (R)(vx - vd * nx);
+                        P.v[1][i] = (R)(vy - vd * ny);
+                        P.v[2][i] = (R)(vz - vd * nz);
+                    }
+                }
+                if (heat_to_p > 0.0) {
+                    const double pm = fmax(C.work_rho, 1.0) * fmax(pv, 1e-24);
+                    const double dT = heat_to_p / (fmax(pm, 1e-15) * C.work_cp);
+                    P.T[i] = (R)fmin(Tp + dT, C.work_melt);
+                }
+                ++counted;
             }
         }
     }
-    // per-wave totals, one atomic per wave that has any (a count of the
-    // culled particles instead would put one atomic per wave of the whole
-    // workpiece on one address)
-    const unsigned long long cmask = __ballot(counted), kmask = __ballot(probed);
+    // per-wave totals, one atomic per wave that has any
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         heat_gen += __shfl_xor(heat_gen, d, 64);
         force_sum += __shfl_xor(force_sum, d, 64);
+        counted += __shfl_xor(counted, d, 64);
+        probed += __shfl_xor(probed, d, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (cmask) {
-            atomicAdd(A.counts, (unsigned long long)__popcll(cmask));
+        if (counted) {
+            atomicAdd(A.counts, (unsigned long long)counted);
             unsafeAtomicAdd(A.totals, heat_gen);
             unsafeAtomicAdd(A.totals + 1, force_sum);
         }
-        if (kmask) atomicAdd(A.counts + 1, (unsigned long long)__popcll(kmask));
+        if (probed) atomicAdd(A.counts + 1, (unsigned long long)probed);
     }
 }
 

@@ -135,7 +135,9 @@ struct mpmc_contact {
     int32_t *idx = nullptr, *idx2 = nullptr;
     NodeRec *rec = nullptr;
     int32_t *start = nullptr, *end = nullptr;
-    unsigned long long *d_misc = nullptr;   // box (6), counts (2), totals (2, as doubles)
+    unsigned long long *d_misc = nullptr;   // box (6), counts (2), totals (2, as doubles), candidates (1)
+    int32_t *cand = nullptr;                // particles that can touch the tool (k_contact_cull)
+    int32_t cand_cap = 0;
     unsigned long long *h_misc = nullptr;   // pinned read-back of counts and totals
     mpmw::ScratchBuf temp;
     mpmc_results res{};
@@ -214,18 +216,22 @@ int download_nodes(mpmt_tool *t, NodeBlock &b, int32_t n, mpmt_node_arrays *a) {
     return MPMW_OK;
 }
 
-template <typename R> void launch_contact(mpmc_contact *c, mpms_solver *s, Nodes S, const ContactArgs &A, int np,
-                                          hipStream_t st);
-template <> void launch_contact<float>(mpmc_contact *c, mpms_solver *s, Nodes S, const ContactArgs &A, int np,
-                                       hipStream_t st) {
-    hipLaunchKernelGGL(k_contact<float>, dim3(blocks(np)), dim3(kBlock), 0, st, mpms_internal::particles_f(s), S,
-                       contact_params(c->prm), A);
+// The contact pass: the particles that can touch the tool are culled into
+// c->cand by a streaming pass over the workpiece, then k_contact runs on
+// them (a grid of at most kContactBlocks blocks, striding over the list).
+constexpr int kContactBlocks = 2048;
+template <typename R> void launch_contact(mpmc_contact *c, const mpms::Particles<R> &P, Nodes S, const ContactArgs &A,
+                                          int np, int32_t *ncand, hipStream_t st) {
+    const int cull_blocks = (int)((np + (int64_t)kBlock * kCullPer - 1) / ((int64_t)kBlock * kCullPer));
+    if (cull_blocks > 0)
+        hipLaunchKernelGGL(k_contact_cull<R>, dim3(cull_blocks), dim3(kBlock), 0, st, P, c->prm.contact_radius, A.box,
+                           c->cand, ncand);
+    hipLaunchKernelGGL(k_contact<R>, dim3(std::max(1, std::min(blocks(np), kContactBlocks))), dim3(kBlock), 0, st, P,
+                       S, contact_params(c->prm), A, (const int32_t *)c->cand, (const int32_t *)ncand);
 }
-template <> void launch_contact<double>(mpmc_contact *c, mpms_solver *s, Nodes S, const ContactArgs &A, int np,
-                                        hipStream_t st) {
-    hipLaunchKernelGGL(k_contact<double>, dim3(blocks(np)), dim3(kBlock), 0, st, mpms_internal::particles_d(s), S,
-                       contact_params(c->prm), A);
-}
+template <typename R> mpms::Particles<R> contact_particles(mpms_solver *s);
+template <> mpms::Particles<float> contact_particles<float>(mpms_solver *s) { return mpms_internal::particles_f(s); }
+template <> mpms::Particles<double> contact_particles<double>(mpms_solver *s) { return mpms_internal::particles_d(s); }
 
 int ensure_hash(mpmc_contact *c, int32_t nn, hipStream_t st) {
     if (nn <= c->cap) return MPMW_OK;   // ensureCapacity (:103-115)
@@ -273,11 +279,25 @@ template <typename R> int resolve_impl(mpmc_contact *c, mpms_solver *s, mpmt_too
     MC_TRY(hipStreamWaitEvent(ms, t->ev, 0));
     int rc;
     if ((rc = ensure_hash(c, nn, ms))) return rc;
+    if (np > c->cand_cap) {
+        MC_TRY(hipStreamSynchronize(ms));
+        if (c->cand) (void)hipFree(c->cand);
+        c->cand = nullptr;
+        c->cand_cap = 0;
+        const int32_t cap = np + np / 4;
+        if (hipMalloc(&c->cand, sizeof(int32_t) * (size_t)cap) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MPMW_E_NOMEM, "hipMalloc of the contact candidates failed");
+        }
+        c->cand_cap = cap;
+    }
     Nodes S = t->surf.v;
     S.n = nn;
     unsigned long long *box = c->d_misc, *counts = c->d_misc + 6;
     double *totals = (double *)(c->d_misc + 8);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, ms, (int32_t *)counts, 0, (int64_t)8);
+    int32_t *ncand = (int32_t *)(c->d_misc + 10);
+    // counts, totals and the candidate count start at zero
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, ms, (int32_t *)counts, 0, (int64_t)10);
     hipLaunchKernelGGL(k_reset_contact, dim3(blocks(nn)), dim3(kBlock), 0, ms, S, box);
     double o[3];
     mpms_internal::domain_min(s, o);
@@ -302,7 +322,7 @@ template <typename R> int resolve_impl(mpmc_contact *c, mpms_solver *s, mpmt_too
     A.start = c->start; A.end = c->end; A.rec = c->rec; A.box = box;
     A.counts = counts;
     A.totals = totals;
-    launch_contact<R>(c, s, S, A, np, ms);
+    launch_contact<R>(c, contact_particles<R>(s), S, A, np, ncand, ms);
     hipLaunchKernelGGL(k_node_heat, dim3(blocks(nn)), dim3(kBlock), 0, ms, S, contact_params(p), dt);
     MC_TRY(hipGetLastError());
     MC_TRY(hipMemcpyAsync(c->h_misc, counts, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ms));
@@ -740,7 +760,7 @@ int mpmc_create(const mpmc_params *p, mpmc_contact **out) {
     mpmc_contact *c = new mpmc_contact();
     c->prm = *p;
     if (hipEventCreateWithFlags(&c->ev, hipEventDisableTiming) != hipSuccess ||
-        hipMalloc(&c->d_misc, 10 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_misc, 11 * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&c->h_misc, 4 * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipGetLastError();
         if (c->ev) (void)hipEventDestroy(c->ev);
@@ -757,7 +777,7 @@ int mpmc_destroy(mpmc_contact *c) {
     (void)hipSetDevice(c->prm.device);
     (void)hipDeviceSynchronize();
     for (void *q : {(void *)c->keys, (void *)c->keys2, (void *)c->idx, (void *)c->idx2, (void *)c->rec,
-                    (void *)c->start, (void *)c->end, (void *)c->d_misc})
+                    (void *)c->start, (void *)c->end, (void *)c->d_misc, (void *)c->cand})
         if (q) (void)hipFree(q);
     if (c->h_misc) (void)hipHostFree(c->h_misc);
     mpmw::free_scratch(c->temp);
