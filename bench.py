@@ -213,13 +213,17 @@ def run_mode(cfg, mode, precision, steps, warmup, breakdown_steps=25, graph=Fals
     }
 
 
-def batched_bench(cfgs, precision, steps, graph=False, autotune=0, serial=None):
+def batched_bench(cfgs, precision, steps, graph=False, autotune=0, serial=None, cold=False):
     """The batched workload (64 workpieces of uneven size): wall time of the
     serial plain solves (one solver after another, each set up, stepped and
     synchronised) against the same set-ups stepped together in one batch.
     Set-up (generating each slab on the device) counts in both.  With
     `autotune` the batch tunes its launch configuration during its first
-    steps (counted in the time); `serial` reuses an earlier serial time."""
+    steps (counted in the time); `serial` reuses an earlier serial time.
+    With `cold` the batch runs twice and both times are reported: the first
+    batch of the process also pays the device allocations of its 64
+    members' state (the serial solves hold one member at a time), the
+    second finds the allocator's pool warm, as a serving process does."""
     P = lambda c: dict(c["params"], precision=mpmw.FP64 if precision == "fp64" else mpmw.FP32)
     nsim = len(cfgs)
     parts = sum(int(np.prod(c["cells"])) * 8 for c in cfgs)
@@ -234,27 +238,32 @@ def batched_bench(cfgs, precision, steps, graph=False, autotune=0, serial=None):
             s.synchronize()
             s.close()
         serial = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    solvers = []
-    b = mpmw.Batch()
-    b.set_graph(graph)
-    if autotune:
-        b.set_autotune(autotune)
-    for c in cfgs:
-        s = mpmw.Solver(**P(c))
-        workloads.apply_to_solver(mpmw, s, c)
-        solvers.append(s)
-        b.add(s)
-    dts = np.array([c["dt"] for c in cfgs])
-    for _ in range(steps):
-        b.step(dts)
-    b.synchronize()
-    batched = time.perf_counter() - t0
-    graph_stats = b.graph_stats()
-    tuning = b.tuning() if autotune else None
-    b.close()
-    for s in solvers:
-        s.close()
+    def run_batch():
+        t0 = time.perf_counter()
+        solvers = []
+        b = mpmw.Batch()
+        b.set_graph(graph)
+        if autotune:
+            b.set_autotune(autotune)
+        for c in cfgs:
+            s = mpmw.Solver(**P(c))
+            workloads.apply_to_solver(mpmw, s, c)
+            solvers.append(s)
+            b.add(s)
+        dts = np.array([c["dt"] for c in cfgs])
+        for _ in range(steps):
+            b.step(dts)
+        b.synchronize()
+        t = time.perf_counter() - t0
+        graph_stats = b.graph_stats()
+        tuning = b.tuning() if autotune else None
+        b.close()
+        for s in solvers:
+            s.close()
+        return t, graph_stats, tuning
+
+    first = run_batch()[0] if cold else None
+    batched, graph_stats, tuning = run_batch()
     return {
         "workload": f"{nsim} workpieces of the batched distribution, {steps} steps each",
         "particles": parts, "steps": steps,
@@ -263,6 +272,9 @@ def batched_bench(cfgs, precision, steps, graph=False, autotune=0, serial=None):
         "batched_sims_per_hour": nsim / batched * 3600.0,
         "speedup": serial / batched,
         "batched_particle_steps_per_s": parts * steps / batched,
+        "first_batch": None if first is None else {
+            "batched_s": first, "speedup": serial / first,
+            "note": "the process's first batch, which also allocates its members' device state"},
         "graph": graph_stats if graph else None,
         "tuning": tuning,
     }
@@ -620,7 +632,7 @@ def main():
                   "fused_kernel_avg_ms": q["k_avg_ms"], "kernels": q["kernels"], "binning": q["diag"]}
     if args.batch_count > 0:
         bcfg = workloads.batch_config2(args.batch_count, steps=args.batch_steps)
-        out["batched"] = batched_bench(bcfg, args.precision, args.batch_steps)
+        out["batched"] = batched_bench(bcfg, args.precision, args.batch_steps, cold=True)
         if args.graph:
             g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True)
             out["batched"]["graph"] = {k: g[k] for k in ("serial_s", "batched_s", "speedup", "graph",
