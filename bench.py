@@ -634,7 +634,7 @@ def main():
         bcfg = workloads.batch_config2(args.batch_count, steps=args.batch_steps)
         out["batched"] = batched_bench(bcfg, args.precision, args.batch_steps, cold=True)
         if args.graph:
-            g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True)
+            g = batched_bench(bcfg, args.precision, args.batch_steps, graph=True, serial=out["batched"]["serial_s"])
             out["batched"]["graph"] = {k: g[k] for k in ("serial_s", "batched_s", "speedup", "graph",
                                                          "batched_sims_per_hour")}
         if args.autotune > 0:
